@@ -1,0 +1,245 @@
+/*
+ * common.h - device data layout of the path-tracer hot path and the small vector helpers the
+ * kernels use. Host code includes it for the struct layouts.
+ *
+ * Vector helpers keep the operation order of the reference's Vector3 functions
+ * (src/math/vector3.cu) so that, compiled with -ffp-contract=off, a path traced here produces
+ * the same float bits as the same path traced by a host build of the same formulas.
+ */
+#pragma once
+
+#include <stdint.h>
+#include <hip/hip_runtime.h>
+
+#include "../../include/illume_c.h"
+#include "../../include/illume_pmath.h"
+
+#define ILM_FLT_MAX 3.402823466e+38f
+#define ILM_FLT_EPSILON 1.192092896e-07f
+
+/* Largest max_depth a request may ask for (sizes the per-depth queue counters). */
+#define ILM_MAX_DEPTH 256
+
+/* ---------------- BVH node layouts ---------------- */
+/* FP32 BVH2 node, 64 B: both children's boxes live in the parent so one node fetch decides
+ * the whole step. child >= 0: inner node index. child < 0: leaf, see ilm_leaf_*(). */
+struct __attribute__((aligned(16))) BNode {
+    float4 a;   /* lo0.x hi0.x lo0.y hi0.y */
+    float4 b;   /* lo0.z hi0.z lo1.x hi1.x */
+    float4 c;   /* lo1.y hi1.y lo1.z hi1.z */
+    int4 d;     /* child0 child1 - - */
+};
+
+/* 8-bit quantised BVH4 node, 64 B. Child boxes are stored relative to the node's origin
+ * with a per-axis power-of-two scale; lo rounds down and hi rounds up, so every dequantised
+ * box contains the FP32 box it encodes. */
+struct __attribute__((aligned(16))) QNode {
+    float ox, oy, oz;       /* origin */
+    uint8_t ex, ey, ez;     /* scale = 2^(e-127) */
+    uint8_t nchild;
+    uint8_t qlo[3][4];      /* [axis][child] */
+    uint8_t qhi[3][4];
+    int32_t child[4];       /* same encoding as BNode children; unused slots = ILM_NO_CHILD */
+};
+
+#define ILM_NO_CHILD ((int32_t)0x7fffffff)
+#define ILM_LEAF_MAX 16
+__host__ __device__ inline int32_t ilm_leaf_code(uint32_t first, uint32_t count) {
+    return (int32_t)(0x80000000u | ((count - 1u) << 27) | first);
+}
+__host__ __device__ inline uint32_t ilm_leaf_first(int32_t c) { return (uint32_t)c & 0x07ffffffu; }
+__host__ __device__ inline uint32_t ilm_leaf_count(int32_t c) { return (((uint32_t)c >> 27) & 0xfu) + 1u; }
+
+/* ---------------- scene ---------------- */
+struct DTex {
+    int type;
+    int w, h, hdr;
+    float a[3];
+    float b[3];
+    float scale[2];
+    const float4* f;
+    const uchar4* u8;
+};
+
+struct DMedium {
+    float absorption[3];
+    float scattering;
+    float g;
+    int active;
+};
+
+struct DMat {
+    int type;
+    int medium;
+    int distribution;
+    int pad;
+    float ior;
+    float diffuse_fresnel;
+    float eta[3];
+    float k[3];
+    DTex albedo;
+    DTex roughness;
+};
+
+struct DInst {
+    float inv[12];      /* rows 0..2 of the inverse (world -> object) */
+    float tinv[12];     /* rows 0..2 of the inverse-transpose (normals object -> world) */
+    int mesh;
+    int material;
+    int pad[2];
+};
+
+struct DMesh {
+    const BNode* nodes;
+    const QNode* qnodes;
+    const float4* tris;     /* 3 per leaf slot: (v0, orig idx bits), (e1, 0), (e2, 0) */
+    const int* idx;         /* original triangle vertex indices, 3 per triangle */
+    const float* pos;       /* 3 per vertex */
+    const float* nrm;       /* 3 per vertex, or null */
+    const float* uv;        /* 2 per vertex, or null */
+    int root;               /* root child code (FP32 layout) */
+    int qroot;              /* root child code (quantised layout) */
+    int face_normals;
+    int has_texcoords;
+};
+
+struct DSphere {
+    float c[3];
+    float r;
+    int material;
+    int pad[3];
+};
+
+struct DScene {
+    const BNode* tlas;
+    const QNode* qtlas;
+    const int* tlas_prims;  /* TLAS leaf slot -> instance index */
+    int tlas_root;
+    int qtlas_root;
+    int n_inst;
+    int n_spheres;
+    const DInst* inst;
+    const DMesh* meshes;
+    const DSphere* spheres;
+    const DMat* mats;
+    const DMedium* mediums;
+    DTex env;
+    float cam[12];          /* rows 0..2 of the camera matrix */
+    float dof;
+    float aperture;
+    int pad[2];
+};
+
+/* One request inside a launch. */
+struct DReq {
+    const DScene* scene;
+    int width, height;
+    int max_depth;
+    float ray_bias;
+    float image_width;
+    float pixel_size, left, top;
+    int tonemap;
+    int spp;
+    float4* accum;          /* width*height Kahan sums */
+    float4* comp;           /* width*height Kahan compensations */
+};
+
+/* One slice of work in a chunk: passes [pass0, pass0+npass) of pixels [px0, px0+npx) of
+ * request req. Paths are laid out pass-major: pid = pid_base + s*npx + (pixel - px0). */
+struct DWork {
+    int req;
+    int pass0;
+    int npass;
+    uint32_t px0;
+    uint32_t npx;
+    uint32_t pid_base;      /* first path id of this slice */
+    uint32_t acc_base;      /* first accumulate index of this slice (prefix of npx) */
+    uint32_t pad;
+};
+
+#define ILM_MAX_JOBS 256
+#define ILM_MAX_WORK 512
+
+/* Everything that changes from one chunk to the next, in one device-resident block, so the
+ * kernels of a chunk can be launched with fixed arguments (and captured once as a graph). */
+struct DChunk {
+    uint32_t total_paths;
+    uint32_t total_px;
+    int n_work;
+    int depth_steps;
+    DReq reqs[ILM_MAX_JOBS];
+    DWork work[ILM_MAX_WORK];
+};
+
+/* Per-chunk control block (device). Cleared by k_ctl_init, never by a memset. */
+struct DCtl {
+    uint32_t count[ILM_MAX_DEPTH + 2];   /* live paths entering depth d */
+    uint32_t fetch[ILM_MAX_DEPTH + 2];   /* extend work cursor at depth d */
+    unsigned long long segments;
+    uint32_t total;
+    uint32_t pad;
+};
+
+/* Path state, struct of arrays; two copies ping-pong across depths. */
+struct DPaths {
+    float4* o;          /* origin, w = bits of (req << 16 | (medium + 1)) */
+    float4* d;          /* direction, w = bits of pixel */
+    float4* c;          /* throughput, w = bits of pid */
+    uint2* rng;         /* PCG32 state */
+};
+
+struct DHits {
+    float4* h;          /* t, u, v, bits of prim */
+    int* inst;          /* instance, -2-s for sphere s, -1 miss */
+};
+
+/* Everything a kernel launch needs. Passed by value. */
+struct KParams {
+    const DChunk* chunk;
+    DCtl* ctl;
+    DPaths in;
+    DPaths out;
+    DHits hits;
+    float4* contrib;        /* per pid radiance (zero when the path ends without emission) */
+    int* seg_count;         /* per pid segment counts (debug), or null */
+    const uint32_t* list;   /* explicit (pixel, pass) pairs for the debug raygen, or null */
+    unsigned long long* trav_stats;   /* [0] child-box tests, [1] triangle tests, or null */
+    int bvh_layout;
+    int tri_test;
+};
+
+/* ---------------- vector helpers (reference operation order) ---------------- */
+struct V3 { float x, y, z; };
+__host__ __device__ inline V3 mk3(float x, float y, float z) { V3 r; r.x = x; r.y = y; r.z = z; return r; }
+__host__ __device__ inline V3 add3(V3 a, V3 b) { return mk3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__host__ __device__ inline V3 adds3(V3 a, float s) { return mk3(s + a.x, s + a.y, s + a.z); }
+__host__ __device__ inline V3 sub3(V3 a, V3 b) { return mk3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__host__ __device__ inline V3 mul3(V3 a, float s) { return mk3(a.x * s, a.y * s, a.z * s); }
+__host__ __device__ inline V3 mulv3(V3 a, V3 b) { return mk3(a.x * b.x, a.y * b.y, a.z * b.z); }
+__host__ __device__ inline V3 divs3(V3 a, float s) { return mk3(a.x / s, a.y / s, a.z / s); }
+__host__ __device__ inline V3 divv3(V3 a, V3 b) { return mk3(a.x / b.x, a.y / b.y, a.z / b.z); }
+__host__ __device__ inline float dot3(V3 a, V3 b) { return (a.x * b.x) + (a.y * b.y) + (a.z * b.z); }
+__host__ __device__ inline V3 cross3(V3 a, V3 b) {
+    return mk3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+__host__ __device__ inline float len2_3(V3 v) { return (v.x * v.x) + (v.y * v.y) + (v.z * v.z); }
+__host__ __device__ inline V3 normalize3(V3 v) {
+    float inv = 1.0f / __builtin_sqrtf(len2_3(v));
+    return mk3(v.x * inv, v.y * inv, v.z * inv);
+}
+__host__ __device__ inline V3 reflect3(V3 v, V3 n) { return sub3(v, mul3(mul3(n, dot3(v, n)), 2.0f)); }
+__host__ __device__ inline V3 to_basis3(V3 v, V3 n) {
+    V3 t;
+    if (pm_fabs(n.x) > pm_fabs(n.y)) t = mk3(n.z, 0.0f, -n.x);
+    else t = mk3(0.0f, n.z, -n.y);
+    V3 b = cross3(n, t);
+    t = cross3(b, n);
+    V3 r = add3(add3(mul3(t, v.x), mul3(n, v.y)), mul3(b, v.z));
+    return normalize3(r);
+}
+/* rows 0..2 of a 4x4 matrix times (v, w), src/math/matrix4.cu:203-209 */
+__host__ __device__ inline V3 xform3(const float* m, V3 v, float w) {
+    return mk3(m[0] * v.x + m[1] * v.y + m[2] * v.z + m[3] * w,
+               m[4] * v.x + m[5] * v.y + m[6] * v.z + m[7] * w,
+               m[8] * v.x + m[9] * v.y + m[10] * v.z + m[11] * w);
+}

@@ -1,0 +1,114 @@
+/* engine.h - host-side handle types of the C ABI */
+#pragma once
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <utility>
+#include <vector>
+
+#include "common.h"
+
+#define ILM_RING 8
+
+struct ilm_scene {
+    ilm_ctx* ctx = nullptr;
+    std::atomic<int> refs{0};
+    std::vector<void*> allocs;
+    int64_t bytes = 0;
+    DScene* d_scene = nullptr;
+    DScene h_scene;
+    ilm_camera_desc camera;
+    std::vector<float> inst_mats;   /* 48 per instance: mat, inv, inverse-transpose */
+    int num_instances = 0;
+    int64_t mesh_tris = 0;
+    int stack_bound_fp32 = 0;
+    int stack_bound_q = 0;
+};
+
+/* One render request. */
+struct Job {
+    ilm_scene* scene = nullptr;
+    ilm_render_params p;
+    DReq dreq;
+    float4* d_accum = nullptr;      /* 2*W*H: Kahan sums then compensations */
+    ilm_pixel* d_px = nullptr;
+    int64_t bytes = 0;              /* device bytes while admitted */
+    int next_pass = 0;              /* issue cursor: pass, pixel */
+    int64_t next_px = 0;
+    /* served form */
+    float4* h_accum = nullptr;      /* pinned results */
+    ilm_pixel* h_px = nullptr;
+    std::atomic<int> state{0};
+    std::atomic<bool> cancel{false};
+    bool failed = false;
+    double submit_ms = 0, start_ms = 0, end_ms = 0;
+    int samples_done = 0;
+    int64_t last_ev_seq = -1;
+    int refs = 1;
+    std::condition_variable cv;
+    ~Job();
+};
+
+struct ilm_req {
+    Job* job = nullptr;
+    ilm_ctx* ctx = nullptr;
+};
+
+struct ProfileSink {
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+};
+
+struct ilm_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    ilm_opts opts;
+    int n_cu = 0;
+    int ext_grid = 0, shade_grid = 0, gen_grid = 0;
+    std::chrono::steady_clock::time_point t0;
+
+    /* device buffers */
+    int64_t cap = 0;                /* paths per chunk */
+    DPaths A, B;
+    DHits hits;
+    float4* contrib = nullptr;
+    int* seg_count = nullptr;
+    int* ovf = nullptr;
+    DCtl* ctl = nullptr;
+    DChunk* d_chunk = nullptr;
+    unsigned long long* d_seg_total = nullptr;
+    unsigned long long* d_trav = nullptr;
+    std::vector<void*> allocs;
+    int64_t own_bytes = 0;
+
+    /* chunk descriptor staging */
+    DChunk* h_chunk[ILM_RING] = {};
+    hipEvent_t ring_ev[ILM_RING] = {};
+    int ring_pos = 0;
+    unsigned long long* h_u64 = nullptr;
+
+    std::map<int, hipGraphExec_t> graphs;
+    std::mutex dev_mu;              /* one user of the device pipeline at a time */
+
+    /* serving */
+    std::mutex sched_mu;
+    std::condition_variable sched_cv;
+    std::deque<Job*> pending;
+    std::vector<Job*> active;
+    std::thread server;
+    bool stop = false;
+    int64_t budget = 0;
+    int64_t in_use = 0;
+    int64_t chunk_seq = 0;
+    int64_t chunks_retired = 0;
+    std::vector<double> latencies;
+    uint64_t stat_segments = 0, stat_done = 0, stat_rejected = 0, stat_cancelled = 0;
+    double stat_busy_ms = 0.0;
+
+    std::mutex err_mu;
+    std::string err, err_copy;
+};

@@ -1,0 +1,1201 @@
+/*
+ * engine.hip - host runtime behind the C ABI: context, device-resident scenes, the chunked
+ * wavefront driver (direct launches or one hipGraph per depth count), the plain, batched and
+ * served forms, and the parity hooks.
+ *
+ * Reference counterparts: Renderer::render_to_bitmap (src/renderer.cu:498-597) and
+ * SceneRef (src/scene/sceneref.cpp:9-148). Differences by design: scenes stay on the device
+ * across renders, path state is struct-of-arrays and compacted on the device, many passes
+ * (and many requests) share one chunk of launches, and nothing synchronises with the host
+ * between depths.
+ */
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstdio>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "bvh.h"
+#include "engine.h"
+#include "hostmath.h"
+#include "kernels.h"
+
+using namespace ilm;
+
+static thread_local std::string g_err;
+
+const char* ilm_global_error(void) { return g_err.c_str(); }
+int ilm_version(void) { return ILM_ABI_VERSION; }
+
+static int fail(ilm_ctx* ctx, int code, const std::string& msg) {
+    g_err = msg;
+    if (ctx) {
+        std::lock_guard<std::mutex> lk(ctx->err_mu);
+        ctx->err = msg;
+    }
+    return code;
+}
+#define HIPTRY(ctx, call)                                                                          \
+    do {                                                                                           \
+        hipError_t e_ = (call);                                                                    \
+        if (e_ != hipSuccess)                                                                      \
+            return fail((ctx), ILM_E_DEVICE, std::string(#call ": ") + hipGetErrorString(e_));     \
+    } while (0)
+
+static double now_ms(const ilm_ctx* ctx) {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - ctx->t0).count();
+}
+
+/* ======================= scenes ======================= */
+
+static hipError_t dalloc(ilm_scene* s, void** p, size_t bytes) {
+    *p = nullptr;
+    if (bytes == 0) return hipSuccess;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e == hipSuccess) {
+        s->allocs.push_back(*p);
+        s->bytes += (int64_t)bytes;
+    }
+    return e;
+}
+template <class T>
+static hipError_t dupload(ilm_scene* s, T** dst, const T* src, size_t count) {
+    *dst = nullptr;
+    if (count == 0) return hipSuccess;
+    hipError_t e = dalloc(s, (void**)dst, count * sizeof(T));
+    if (e != hipSuccess) return e;
+    return hipMemcpy(*dst, src, count * sizeof(T), hipMemcpyHostToDevice);
+}
+
+static int upload_texture(ilm_ctx* ctx, ilm_scene* s, const ilm_texture_desc& t, DTex& d) {
+    std::memset(&d, 0, sizeof(d));
+    d.type = t.type;
+    if (t.type == ILM_TEX_CONSTANT) {
+        for (int i = 0; i < 3; i++) d.a[i] = t.color[i];
+    } else if (t.type == ILM_TEX_CHECKERBOARD) {
+        for (int i = 0; i < 3; i++) { d.a[i] = t.on[i]; d.b[i] = t.off[i]; }
+    } else if (t.type == ILM_TEX_BITMAP) {
+        if (t.width <= 0 || t.height <= 0) return fail(ctx, ILM_E_INVALID, "bitmap texture without size");
+        d.w = t.width; d.h = t.height; d.hdr = t.is_hdr;
+        size_t n = (size_t)t.width * t.height;
+        if (t.is_hdr) {
+            if (!t.texels_f32) return fail(ctx, ILM_E_INVALID, "HDR bitmap without texels");
+            float4* p;
+            HIPTRY(ctx, dupload(s, &p, (const float4*)t.texels_f32, n));
+            d.f = p;
+        } else {
+            if (!t.texels_u8) return fail(ctx, ILM_E_INVALID, "LDR bitmap without texels");
+            uchar4* p;
+            HIPTRY(ctx, dupload(s, &p, (const uchar4*)t.texels_u8, n));
+            d.u8 = p;
+        }
+    } else {
+        return fail(ctx, ILM_E_INVALID, "unknown texture type");
+    }
+    d.scale[0] = t.scale[0];
+    d.scale[1] = t.scale[1];
+    return ILM_OK;
+}
+
+static void m4_rows(const hm::M4& m, float* out12) {
+    for (int i = 0; i < 3; i++) for (int j = 0; j < 4; j++) out12[4 * i + j] = m.m[i][j];
+}
+
+/* outward rounding of a double bound to float */
+static float round_down(double v) { float f = (float)v; return (double)f > v ? std::nextafter(f, -INFINITY) : f; }
+static float round_up(double v) { float f = (float)v; return (double)f < v ? std::nextafter(f, INFINITY) : f; }
+
+int ilm_scene_upload(ilm_ctx* ctx, const ilm_scene_desc* d, ilm_scene** out) {
+    if (!ctx || !d || !out) return fail(ctx, ILM_E_INVALID, "null argument");
+    *out = nullptr;
+    std::unique_ptr<ilm_scene> s(new ilm_scene());
+    s->ctx = ctx;
+    HIPTRY(ctx, hipSetDevice(ctx->device));
+
+    /* ---- validation ---- */
+    for (int i = 0; i < d->num_meshes; i++) {
+        const ilm_mesh_desc& m = d->meshes[i];
+        if (m.num_vertices < 0 || m.num_triangles < 0 || (m.num_triangles > 0 && (!m.indices || !m.positions)))
+            return fail(ctx, ILM_E_INVALID, "mesh " + std::to_string(i) + ": bad arrays");
+        if (!m.face_normals && !m.normals) return fail(ctx, ILM_E_INVALID, "mesh needs normals when face_normals=0");
+        if (m.has_texcoords && !m.texcoords) return fail(ctx, ILM_E_INVALID, "mesh needs texcoords");
+        if ((int64_t)m.num_triangles >= (1 << 27)) return fail(ctx, ILM_E_UNSUPPORTED, "mesh too large (2^27 triangles)");
+        for (int64_t k = 0; k < 3LL * m.num_triangles; k++)
+            if (m.indices[k] < 0 || m.indices[k] >= m.num_vertices)
+                return fail(ctx, ILM_E_INVALID, "mesh " + std::to_string(i) + ": index out of range");
+    }
+    for (int i = 0; i < d->num_instances; i++) {
+        if (d->instances[i].mesh < 0 || d->instances[i].mesh >= d->num_meshes)
+            return fail(ctx, ILM_E_INVALID, "instance mesh out of range");
+        if (d->instances[i].material < 0 || d->instances[i].material >= d->num_materials)
+            return fail(ctx, ILM_E_INVALID, "instance material out of range");
+    }
+    for (int i = 0; i < d->num_spheres; i++)
+        if (d->spheres[i].material < 0 || d->spheres[i].material >= d->num_materials)
+            return fail(ctx, ILM_E_INVALID, "sphere material out of range");
+    for (int i = 0; i < d->num_materials; i++)
+        if (d->materials[i].medium < -1 || d->materials[i].medium >= d->num_mediums)
+            return fail(ctx, ILM_E_INVALID, "material medium out of range");
+    if (d->num_instances >= 65536) return fail(ctx, ILM_E_UNSUPPORTED, "too many instances");
+
+    DScene hs;
+    std::memset(&hs, 0, sizeof(hs));
+
+    /* ---- meshes: BVH, layouts, triangle slots ---- */
+    std::vector<DMesh> dmeshes(d->num_meshes);
+    std::vector<Box> mesh_bounds(d->num_meshes);
+    int blas_depth = 0, blas_qdepth = 0;
+    BuildOpts bo;
+    for (int i = 0; i < d->num_meshes; i++) {
+        const ilm_mesh_desc& m = d->meshes[i];
+        std::vector<Box> pb(m.num_triangles);
+        Box mb;
+        mb.reset();
+        for (int t = 0; t < m.num_triangles; t++) {
+            pb[t].reset();
+            for (int j = 0; j < 3; j++) pb[t].grow(&m.positions[3 * m.indices[3 * t + j]]);
+            mb.grow(pb[t]);
+        }
+        mesh_bounds[i] = mb;
+        Tree tree = build_tree(pb, bo);
+        Layout2 L2 = make_layout2(tree);
+        LayoutQ LQ = make_layout_q(tree);
+        blas_depth = std::max(blas_depth, tree.max_depth);
+        blas_qdepth = std::max(blas_qdepth, LQ.max_depth);
+        std::vector<float4> tris(3 * (size_t)m.num_triangles);
+        for (int k = 0; k < m.num_triangles; k++) {
+            int t = tree.order[k];
+            const float* p0 = &m.positions[3 * m.indices[3 * t]];
+            const float* p1 = &m.positions[3 * m.indices[3 * t + 1]];
+            const float* p2 = &m.positions[3 * m.indices[3 * t + 2]];
+            float tb;
+            int32_t ti = t;
+            std::memcpy(&tb, &ti, 4);
+            tris[3 * k] = make_float4(p0[0], p0[1], p0[2], tb);
+            tris[3 * k + 1] = make_float4(p1[0], p1[1], p1[2], 0.0f);
+            tris[3 * k + 2] = make_float4(p2[0], p2[1], p2[2], 0.0f);
+        }
+        DMesh& dm = dmeshes[i];
+        std::memset(&dm, 0, sizeof(dm));
+        BNode* bn; QNode* qn; float4* tp; int* ip; float* pp; float* np; float* up;
+        HIPTRY(ctx, dupload(s.get(), &bn, L2.nodes.data(), L2.nodes.size()));
+        HIPTRY(ctx, dupload(s.get(), &qn, LQ.nodes.data(), LQ.nodes.size()));
+        HIPTRY(ctx, dupload(s.get(), &tp, tris.data(), tris.size()));
+        HIPTRY(ctx, dupload(s.get(), &ip, m.indices, 3 * (size_t)m.num_triangles));
+        HIPTRY(ctx, dupload(s.get(), &pp, m.positions, 3 * (size_t)m.num_vertices));
+        np = nullptr;
+        up = nullptr;
+        if (!m.face_normals) HIPTRY(ctx, dupload(s.get(), &np, m.normals, 3 * (size_t)m.num_vertices));
+        if (m.has_texcoords) HIPTRY(ctx, dupload(s.get(), &up, m.texcoords, 2 * (size_t)m.num_vertices));
+        dm.nodes = bn; dm.qnodes = qn; dm.tris = tp; dm.idx = ip; dm.pos = pp; dm.nrm = np; dm.uv = up;
+        dm.root = L2.root;
+        dm.qroot = LQ.root;
+        dm.face_normals = m.face_normals != 0;
+        dm.has_texcoords = m.has_texcoords != 0;
+        s->mesh_tris += m.num_triangles;
+    }
+
+    /* ---- instances and TLAS ---- */
+    std::vector<DInst> dinst(d->num_instances);
+    std::vector<Box> inst_boxes(d->num_instances);
+    s->inst_mats.resize(48 * (size_t)d->num_instances);
+    for (int i = 0; i < d->num_instances; i++) {
+        hm::M4 mat, inv, tinv;
+        hm::build_transform(d->instances[i].transform, mat, inv, tinv);
+        DInst& di = dinst[i];
+        std::memset(&di, 0, sizeof(di));
+        m4_rows(inv, di.inv);
+        m4_rows(tinv, di.tinv);
+        di.mesh = d->instances[i].mesh;
+        di.material = d->instances[i].material;
+        for (int k = 0; k < 16; k++) {
+            s->inst_mats[48 * i + k] = mat.m[k / 4][k % 4];
+            s->inst_mats[48 * i + 16 + k] = inv.m[k / 4][k % 4];
+            s->inst_mats[48 * i + 32 + k] = tinv.m[k / 4][k % 4];
+        }
+        /* conservative world box: 8 transformed corners in double, widened by a relative
+         * margin that covers the float inverse used by the traversal */
+        const Box& mb = mesh_bounds[di.mesh];
+        double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+        bool empty = d->meshes[di.mesh].num_triangles == 0;
+        for (int c = 0; c < 8 && !empty; c++) {
+            double p[3] = {(c & 1) ? mb.hi[0] : mb.lo[0], (c & 2) ? mb.hi[1] : mb.lo[1], (c & 4) ? mb.hi[2] : mb.lo[2]};
+            for (int a = 0; a < 3; a++) {
+                double w = (double)mat.m[a][0] * p[0] + (double)mat.m[a][1] * p[1] + (double)mat.m[a][2] * p[2] + (double)mat.m[a][3];
+                lo[a] = std::min(lo[a], w);
+                hi[a] = std::max(hi[a], w);
+            }
+        }
+        Box& ib = inst_boxes[i];
+        if (empty) { ib.reset(); continue; }
+        double ext = std::max({hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2], 1e-30});
+        for (int a = 0; a < 3; a++) {
+            double mag = std::max(std::fabs(lo[a]), std::fabs(hi[a]));
+            double pad = 1e-5 * ext + 1e-6 * mag;
+            ib.lo[a] = round_down(lo[a] - pad);
+            ib.hi[a] = round_up(hi[a] + pad);
+        }
+    }
+    int tlas_depth = 0, tlas_qdepth = 0;
+    std::vector<int> tlas_prims;
+    hs.tlas_root = ILM_NO_CHILD;
+    hs.qtlas_root = ILM_NO_CHILD;
+    if (d->num_instances > 0) {
+        BuildOpts to;
+        to.max_leaf = 1;
+        Tree tt = build_tree(inst_boxes, to);
+        Layout2 T2 = make_layout2(tt);
+        LayoutQ TQ = make_layout_q(tt);
+        tlas_depth = tt.max_depth;
+        tlas_qdepth = TQ.max_depth;
+        tlas_prims = tt.order;
+        BNode* bn; QNode* qn; int* tp;
+        HIPTRY(ctx, dupload(s.get(), &bn, T2.nodes.data(), T2.nodes.size()));
+        HIPTRY(ctx, dupload(s.get(), &qn, TQ.nodes.data(), TQ.nodes.size()));
+        HIPTRY(ctx, dupload(s.get(), &tp, tlas_prims.data(), tlas_prims.size()));
+        hs.tlas = bn; hs.qtlas = qn; hs.tlas_prims = tp;
+        hs.tlas_root = T2.root;
+        hs.qtlas_root = TQ.root;
+    }
+    /* device stack bound: pushes never exceed the depth of the current node per level */
+    const int cap = ilm_k_lds_stack() + ilm_k_ovf_stack();
+    s->stack_bound_fp32 = tlas_depth + 1 + blas_depth;
+    s->stack_bound_q = 3 * (tlas_qdepth + 1) + 1 + 3 * (blas_qdepth + 1);
+    if (s->stack_bound_fp32 > cap)
+        return fail(ctx, ILM_E_UNSUPPORTED, "BVH too deep for the traversal stack");
+
+    /* ---- materials, mediums, spheres, environment, camera ---- */
+    std::vector<DMat> dmats(d->num_materials);
+    for (int i = 0; i < d->num_materials; i++) {
+        const ilm_material_desc& m = d->materials[i];
+        DMat& dm = dmats[i];
+        std::memset(&dm, 0, sizeof(dm));
+        if (m.type < ILM_MAT_EMISSIVE || m.type > ILM_MAT_PLASTIC) return fail(ctx, ILM_E_INVALID, "bad material type");
+        dm.type = m.type;
+        dm.medium = m.medium;
+        dm.distribution = m.distribution;
+        dm.ior = m.ior;
+        for (int k = 0; k < 3; k++) { dm.eta[k] = m.eta[k]; dm.k[k] = m.k[k]; }
+        if (m.type == ILM_MAT_PLASTIC) dm.diffuse_fresnel = hm::diffuse_fresnel(1.0f / m.ior, 1000000);  /* src/material.cu:176 */
+        int rc = upload_texture(ctx, s.get(), m.albedo, dm.albedo);
+        if (rc) return rc;
+        rc = upload_texture(ctx, s.get(), m.roughness, dm.roughness);
+        if (rc) return rc;
+    }
+    std::vector<DMedium> dmed(d->num_mediums);
+    for (int i = 0; i < d->num_mediums; i++) {
+        const ilm_medium_desc& m = d->mediums[i];
+        DMedium& dm = dmed[i];
+        std::memset(&dm, 0, sizeof(dm));
+        for (int k = 0; k < 3; k++) dm.absorption[k] = m.sigma_a[k];
+        dm.scattering = m.sigma_s;
+        dm.g = m.g;
+        float l2 = (m.sigma_a[0] * m.sigma_a[0]) + (m.sigma_a[1] * m.sigma_a[1]) + (m.sigma_a[2] * m.sigma_a[2]);
+        dm.active = ((double)l2 > 1e-5 || m.sigma_s > 0);   /* src/medium.cu:12 */
+    }
+    std::vector<DSphere> dsph(d->num_spheres);
+    for (int i = 0; i < d->num_spheres; i++) {
+        std::memset(&dsph[i], 0, sizeof(DSphere));
+        for (int k = 0; k < 3; k++) dsph[i].c[k] = d->spheres[i].center[k];
+        dsph[i].r = d->spheres[i].radius;
+        dsph[i].material = d->spheres[i].material;
+    }
+    if (d->has_environment) {
+        int rc = upload_texture(ctx, s.get(), d->environment, hs.env);
+        if (rc) return rc;
+    } else {
+        std::memset(&hs.env, 0, sizeof(hs.env));
+        hs.env.type = ILM_TEX_CONSTANT;   /* black (src/texture.h:39-43) */
+    }
+    {
+        hm::M4 mat, inv, tinv;
+        hm::build_transform(d->camera.transform, mat, inv, tinv);
+        m4_rows(mat, hs.cam);
+        hs.dof = d->camera.dof;
+        hs.aperture = d->camera.aperture;
+        s->camera = d->camera;
+    }
+    DInst* di; DMesh* dmp; DSphere* dsp; DMat* dma; DMedium* dme;
+    HIPTRY(ctx, dupload(s.get(), &di, dinst.data(), dinst.size()));
+    HIPTRY(ctx, dupload(s.get(), &dmp, dmeshes.data(), dmeshes.size()));
+    HIPTRY(ctx, dupload(s.get(), &dsp, dsph.data(), dsph.size()));
+    HIPTRY(ctx, dupload(s.get(), &dma, dmats.data(), dmats.size()));
+    HIPTRY(ctx, dupload(s.get(), &dme, dmed.data(), dmed.size()));
+    hs.inst = di; hs.meshes = dmp; hs.spheres = dsp; hs.mats = dma; hs.mediums = dme;
+    hs.n_inst = d->num_instances;
+    hs.n_spheres = d->num_spheres;
+    DScene* ds;
+    HIPTRY(ctx, dupload(s.get(), &ds, &hs, 1));
+    s->d_scene = ds;
+    s->h_scene = hs;
+    s->num_instances = d->num_instances;
+    s->refs.store(1);
+    {
+        std::lock_guard<std::mutex> lk(ctx->sched_mu);
+        ctx->in_use += s->bytes;
+    }
+    *out = s.release();
+    return ILM_OK;
+}
+
+static void scene_release(ilm_scene* s) {
+    if (!s) return;
+    if (s->refs.fetch_sub(1) != 1) return;
+    ilm_ctx* ctx = s->ctx;
+    hipSetDevice(ctx->device);
+    for (void* p : s->allocs) hipFree(p);
+    {
+        std::lock_guard<std::mutex> lk(ctx->sched_mu);
+        ctx->in_use -= s->bytes;
+    }
+    delete s;
+}
+
+void ilm_scene_free(ilm_scene* s) { scene_release(s); }
+int64_t ilm_scene_device_bytes(const ilm_scene* s) { return s ? s->bytes : -1; }
+
+int ilm_debug_instance_matrices(const ilm_scene* s, int32_t i, float* out48) {
+    if (!s || !out48 || i < 0 || i >= s->num_instances) return fail(nullptr, ILM_E_INVALID, "bad instance");
+    std::memcpy(out48, &s->inst_mats[48 * (size_t)i], 48 * sizeof(float));
+    return ILM_OK;
+}
+
+/* ======================= context ======================= */
+
+static int ctx_alloc_paths(ilm_ctx* ctx, int64_t cap) {
+    auto A = [&](void** p, size_t b) -> hipError_t {
+        hipError_t e = hipMalloc(p, b);
+        if (e == hipSuccess) { ctx->allocs.push_back(*p); ctx->own_bytes += (int64_t)b; }
+        return e;
+    };
+    for (DPaths* P : {&ctx->A, &ctx->B}) {
+        HIPTRY(ctx, A((void**)&P->o, cap * sizeof(float4)));
+        HIPTRY(ctx, A((void**)&P->d, cap * sizeof(float4)));
+        HIPTRY(ctx, A((void**)&P->c, cap * sizeof(float4)));
+        HIPTRY(ctx, A((void**)&P->rng, cap * sizeof(uint2)));
+    }
+    HIPTRY(ctx, A((void**)&ctx->hits.h, cap * sizeof(float4)));
+    HIPTRY(ctx, A((void**)&ctx->hits.inst, cap * sizeof(int)));
+    HIPTRY(ctx, A((void**)&ctx->contrib, cap * sizeof(float4)));
+    ctx->cap = cap;
+    return ILM_OK;
+}
+
+int ilm_open(int device, const ilm_opts* opts, ilm_ctx** out) {
+    if (!out) return fail(nullptr, ILM_E_INVALID, "null out");
+    *out = nullptr;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n == 0) return fail(nullptr, ILM_E_DEVICE, "no HIP device available");
+    if (device < 0 || device >= n) return fail(nullptr, ILM_E_INVALID, "device index out of range");
+    std::unique_ptr<ilm_ctx> c(new ilm_ctx());
+    ilm_ctx* ctx = c.get();
+    ctx->device = device;
+    ctx->t0 = std::chrono::steady_clock::now();
+    std::memset(&ctx->opts, 0, sizeof(ctx->opts));
+    ctx->opts.use_graphs = 1;
+    if (opts) ctx->opts = *opts;
+    HIPTRY(ctx, hipSetDevice(device));
+    hipDeviceProp_t prop;
+    HIPTRY(ctx, hipGetDeviceProperties(&prop, device));
+    ctx->n_cu = prop.multiProcessorCount;
+    HIPTRY(ctx, hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    int occ = ilm_k_extend_occupancy(ctx->opts.tri_test, ctx->opts.bvh_layout);
+    ctx->ext_grid = ctx->n_cu * std::max(1, occ);
+    ctx->gen_grid = ctx->n_cu * 8;
+    ctx->shade_grid = ctx->n_cu * 8;
+    int64_t cap = ctx->opts.paths_in_flight > 0 ? ctx->opts.paths_in_flight : (int64_t)1 << 22;
+    cap = std::max<int64_t>(cap, 4096);
+    cap = std::min<int64_t>(cap, (int64_t)1 << 30);
+    int rc = ctx_alloc_paths(ctx, cap);
+    if (rc) return rc;
+    auto A = [&](void** p, size_t b) -> hipError_t {
+        hipError_t e2 = hipMalloc(p, b);
+        if (e2 == hipSuccess) { ctx->allocs.push_back(*p); ctx->own_bytes += (int64_t)b; }
+        return e2;
+    };
+    size_t ovf = (size_t)ctx->ext_grid * ilm_k_ext_block() * ilm_k_ovf_stack() * sizeof(int);
+    HIPTRY(ctx, A((void**)&ctx->ovf, ovf));
+    HIPTRY(ctx, A((void**)&ctx->ctl, sizeof(DCtl)));
+    HIPTRY(ctx, A((void**)&ctx->d_chunk, sizeof(DChunk)));
+    HIPTRY(ctx, A((void**)&ctx->d_seg_total, 2 * sizeof(unsigned long long)));
+    HIPTRY(ctx, A((void**)&ctx->d_trav, 2 * sizeof(unsigned long long)));
+    HIPTRY(ctx, hipMemset(ctx->d_seg_total, 0, 2 * sizeof(unsigned long long)));
+    HIPTRY(ctx, hipMemset(ctx->d_trav, 0, 2 * sizeof(unsigned long long)));
+    for (int i = 0; i < ILM_RING; i++) {
+        HIPTRY(ctx, hipHostMalloc((void**)&ctx->h_chunk[i], sizeof(DChunk), hipHostMallocDefault));
+        HIPTRY(ctx, hipEventCreateWithFlags(&ctx->ring_ev[i], hipEventDisableTiming));
+    }
+    HIPTRY(ctx, hipHostMalloc((void**)&ctx->h_u64, 4 * sizeof(unsigned long long), hipHostMallocDefault));
+    size_t free_b = 0, total_b = 0;
+    HIPTRY(ctx, hipMemGetInfo(&free_b, &total_b));
+    ctx->budget = ctx->opts.hbm_budget_bytes > 0 ? ctx->opts.hbm_budget_bytes : (int64_t)(0.9 * (double)free_b);
+    *out = c.release();
+    return ILM_OK;
+}
+
+static void server_stop(ilm_ctx* ctx);
+
+void ilm_close(ilm_ctx* ctx) {
+    if (!ctx) return;
+    server_stop(ctx);
+    hipSetDevice(ctx->device);
+    hipStreamSynchronize(ctx->stream);
+    for (auto& kv : ctx->graphs) hipGraphExecDestroy(kv.second);
+    for (void* p : ctx->allocs) hipFree(p);
+    if (ctx->seg_count) hipFree(ctx->seg_count);
+    for (int i = 0; i < ILM_RING; i++) {
+        if (ctx->h_chunk[i]) hipHostFree(ctx->h_chunk[i]);
+        if (ctx->ring_ev[i]) hipEventDestroy(ctx->ring_ev[i]);
+    }
+    if (ctx->h_u64) hipHostFree(ctx->h_u64);
+    hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char* ilm_last_error(ilm_ctx* ctx) {
+    if (!ctx) return g_err.c_str();
+    std::lock_guard<std::mutex> lk(ctx->err_mu);
+    ctx->err_copy = ctx->err;
+    return ctx->err_copy.c_str();
+}
+
+/* ======================= chunk pipeline ======================= */
+
+static KParams base_params(ilm_ctx* ctx) {
+    KParams P;
+    std::memset(&P, 0, sizeof(P));
+    P.chunk = ctx->d_chunk;
+    P.ctl = ctx->ctl;
+    P.in = ctx->A;
+    P.out = ctx->B;
+    P.hits = ctx->hits;
+    P.contrib = ctx->contrib;
+    P.bvh_layout = ctx->opts.bvh_layout;
+    P.tri_test = ctx->opts.tri_test;
+    P.trav_stats = ctx->opts.count_tests ? ctx->d_trav : nullptr;
+    return P;
+}
+
+/* Stage a chunk descriptor: pinned ring slot -> the fixed device DChunk (stream ordered). */
+static int stage_chunk(ilm_ctx* ctx, const DChunk& hc) {
+    int slot = ctx->ring_pos++ % ILM_RING;
+    HIPTRY(ctx, hipEventSynchronize(ctx->ring_ev[slot]));
+    std::memcpy(ctx->h_chunk[slot], &hc, sizeof(DChunk));
+    HIPTRY(ctx, hipMemcpyAsync(ctx->d_chunk, ctx->h_chunk[slot], sizeof(DChunk), hipMemcpyHostToDevice, ctx->stream));
+    HIPTRY(ctx, hipEventRecord(ctx->ring_ev[slot], ctx->stream));
+    return ILM_OK;
+}
+
+/* The launch sequence of one chunk with depth_steps depth iterations. */
+static int enqueue_pipeline(ilm_ctx* ctx, const KParams& base, int depth_steps, ProfileSink* prof) {
+    hipStream_t s = ctx->stream;
+    KParams P = base;
+    HIPTRY(ctx, ilm_k_ctl_init(P, s));
+    P.in = ctx->A;
+    HIPTRY(ctx, ilm_k_raygen(P, ctx->gen_grid, s));
+    for (int d = 0; d < depth_steps; d++) {
+        P.in = (d & 1) ? ctx->B : ctx->A;
+        P.out = (d & 1) ? ctx->A : ctx->B;
+        if (prof) {
+            hipEvent_t a, b;
+            HIPTRY(ctx, hipEventCreate(&a));
+            HIPTRY(ctx, hipEventCreate(&b));
+            HIPTRY(ctx, hipEventRecord(a, s));
+            HIPTRY(ctx, ilm_k_extend(P, d, ctx->ovf, ctx->ext_grid, s));
+            HIPTRY(ctx, hipEventRecord(b, s));
+            prof->ev.push_back({a, b});
+        } else {
+            HIPTRY(ctx, ilm_k_extend(P, d, ctx->ovf, ctx->ext_grid, s));
+        }
+        HIPTRY(ctx, ilm_k_shade(P, d, ctx->shade_grid, s));
+    }
+    HIPTRY(ctx, ilm_k_accumulate(P, ctx->gen_grid, s));
+    HIPTRY(ctx, ilm_k_chunk_end(P, ctx->d_seg_total, s));
+    return ILM_OK;
+}
+
+static int launch_chunk(ilm_ctx* ctx, const DChunk& hc, ProfileSink* prof) {
+    int rc = stage_chunk(ctx, hc);
+    if (rc) return rc;
+    KParams P = base_params(ctx);
+    const int D = hc.depth_steps;
+    const bool graphs = ctx->opts.use_graphs && !prof && !ctx->opts.count_tests;
+    if (graphs) {
+        auto it = ctx->graphs.find(D);
+        if (it == ctx->graphs.end()) {
+            hipGraph_t g = nullptr;
+            hipGraphExec_t ge = nullptr;
+            HIPTRY(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+            int rc2 = enqueue_pipeline(ctx, P, D, nullptr);
+            hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
+            if (rc2) return rc2;
+            if (ec != hipSuccess) return fail(ctx, ILM_E_DEVICE, std::string("graph capture: ") + hipGetErrorString(ec));
+            HIPTRY(ctx, hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+            hipGraphDestroy(g);
+            it = ctx->graphs.emplace(D, ge).first;
+        }
+        HIPTRY(ctx, hipGraphLaunch(it->second, ctx->stream));
+        return ILM_OK;
+    }
+    return enqueue_pipeline(ctx, P, D, prof);
+}
+
+/* ======================= jobs ======================= */
+
+static int job_prepare(ilm_ctx* ctx, Job* j) {
+    const ilm_render_params& p = j->p;
+    const int64_t wh = (int64_t)p.width * p.height;
+    HIPTRY(ctx, hipMalloc((void**)&j->d_accum, 2 * wh * sizeof(float4)));
+    HIPTRY(ctx, hipMalloc((void**)&j->d_px, wh * sizeof(ilm_pixel)));
+    HIPTRY(ctx, ilm_k_fill_zero4(j->d_accum, 2 * wh, ctx->stream));
+    std::memset(&j->dreq, 0, sizeof(j->dreq));
+    j->dreq.scene = j->scene->d_scene;
+    j->dreq.width = p.width;
+    j->dreq.height = p.height;
+    j->dreq.max_depth = p.max_depth;
+    j->dreq.ray_bias = p.ray_bias;
+    j->dreq.tonemap = p.tonemap;
+    j->dreq.spp = p.spp;
+    hm::render_info(j->scene->camera, p.width, p.height, j->dreq.image_width, j->dreq.pixel_size, j->dreq.left, j->dreq.top);
+    j->dreq.accum = j->d_accum;
+    j->dreq.comp = j->d_accum + wh;
+    j->next_pass = 0;
+    j->next_px = 0;
+    return ILM_OK;
+}
+
+static void job_release_device(Job* j) {
+    if (j->d_accum) hipFree(j->d_accum);
+    if (j->d_px) hipFree(j->d_px);
+    j->d_accum = nullptr;
+    j->d_px = nullptr;
+}
+
+static bool job_has_work(const Job* j) { return j->next_pass < j->p.spp; }
+
+/* Fill a chunk from jobs in priority order. Each job contributes at most one slice per chunk
+ * (all pixels x several passes, or a pixel range of one pass), so no pixel appears twice in a
+ * chunk and every pixel's passes are accumulated in pass order. */
+static void build_chunk(ilm_ctx* ctx, const std::vector<Job*>& jobs, DChunk& c, std::vector<Job*>& in_chunk) {
+    c.total_paths = 0;
+    c.total_px = 0;
+    c.n_work = 0;
+    c.depth_steps = 0;
+    in_chunk.clear();
+    int64_t left = ctx->cap;
+    int nreq = 0;
+    for (Job* j : jobs) {
+        if (!job_has_work(j) || j->cancel.load()) continue;
+        if (nreq >= ILM_MAX_JOBS || c.n_work >= ILM_MAX_WORK || left <= 0) break;
+        const int64_t wh = (int64_t)j->p.width * j->p.height;
+        DWork w;
+        std::memset(&w, 0, sizeof(w));
+        w.req = nreq;
+        w.pass0 = j->next_pass;
+        if (j->next_px == 0 && wh <= left) {
+            int64_t np = std::min<int64_t>(j->p.spp - j->next_pass, left / wh);
+            w.npass = (int)np;
+            w.px0 = 0;
+            w.npx = (uint32_t)wh;
+            j->next_pass += (int)np;
+        } else {
+            int64_t npx = std::min<int64_t>(wh - j->next_px, left);
+            w.npass = 1;
+            w.px0 = (uint32_t)j->next_px;
+            w.npx = (uint32_t)npx;
+            j->next_px += npx;
+            if (j->next_px >= wh) { j->next_px = 0; j->next_pass += 1; }
+        }
+        w.pid_base = c.total_paths;
+        w.acc_base = c.total_px;
+        c.total_paths += (uint32_t)((int64_t)w.npass * w.npx);
+        c.total_px += w.npx;
+        left -= (int64_t)w.npass * w.npx;
+        c.reqs[nreq] = j->dreq;
+        c.work[c.n_work++] = w;
+        c.depth_steps = std::max(c.depth_steps, j->p.max_depth);
+        nreq++;
+        in_chunk.push_back(j);
+    }
+}
+
+static int validate_params(ilm_ctx* ctx, const ilm_render_params* p) {
+    if (!p) return fail(ctx, ILM_E_INVALID, "null params");
+    if (p->width <= 0 || p->height <= 0 || p->spp <= 0 || p->max_depth <= 0)
+        return fail(ctx, ILM_E_INVALID, "width, height, spp and max_depth must be positive");
+    if (p->max_depth > ILM_MAX_DEPTH) return fail(ctx, ILM_E_INVALID, "max_depth above 256");
+    if ((int64_t)p->width * p->height >= (1LL << 31)) return fail(ctx, ILM_E_INVALID, "image too large");
+    if (p->tonemap < 0 || p->tonemap > ILM_TONEMAP_UNCHARTED) return fail(ctx, ILM_E_INVALID, "bad tonemap");
+    if (p->first_sample < 0) return fail(ctx, ILM_E_INVALID, "negative first_sample");
+    return ILM_OK;
+}
+
+static void accum4_to_rgb(const std::vector<float4>& a4, float* rgb, int64_t n) {
+    for (int64_t i = 0; i < n; i++) { rgb[3 * i] = a4[i].x; rgb[3 * i + 1] = a4[i].y; rgb[3 * i + 2] = a4[i].z; }
+}
+
+/* Run jobs to completion on the calling thread (plain and batched forms). */
+static int run_jobs_sync(ilm_ctx* ctx, std::vector<Job*>& jobs, ilm_render_info* info) {
+    std::lock_guard<std::mutex> dev(ctx->dev_mu);
+    HIPTRY(ctx, hipSetDevice(ctx->device));
+    for (Job* j : jobs) {
+        int rc = job_prepare(ctx, j);
+        if (rc) return rc;
+        /* pass i of the render seeds wang_hash(first_sample + i) */
+        j->next_pass = 0;
+    }
+    ProfileSink prof;
+    hipEvent_t e0, e1;
+    HIPTRY(ctx, hipEventCreate(&e0));
+    HIPTRY(ctx, hipEventCreate(&e1));
+    /* the segment counter is cumulative; read it before and after */
+    HIPTRY(ctx, hipMemcpyAsync(ctx->h_u64 + 3, ctx->d_seg_total, sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx->stream));
+    HIPTRY(ctx, hipMemsetAsync(ctx->d_trav, 0, 2 * sizeof(unsigned long long), ctx->stream));
+    HIPTRY(ctx, hipEventRecord(e0, ctx->stream));
+    DChunk* hc = new DChunk();
+    std::unique_ptr<DChunk> hc_own(hc);
+    std::vector<Job*> in_chunk;
+    int chunks = 0;
+    int64_t max_paths = 0;
+    for (;;) {
+        build_chunk(ctx, jobs, *hc, in_chunk);
+        if (in_chunk.empty()) break;
+        /* pass offsets: first_sample shifts every pass of the request */
+        for (int w = 0; w < hc->n_work; w++) hc->work[w].pass0 += in_chunk[hc->work[w].req]->p.first_sample;
+        int rc = launch_chunk(ctx, *hc, ctx->opts.profile ? &prof : nullptr);
+        if (rc) return rc;
+        chunks++;
+        max_paths = std::max<int64_t>(max_paths, hc->total_paths);
+    }
+    HIPTRY(ctx, hipEventRecord(e1, ctx->stream));
+    for (Job* j : jobs) {
+        const int64_t wh = (int64_t)j->p.width * j->p.height;
+        HIPTRY(ctx, ilm_k_tonemap(j->d_accum, j->d_px, (int)wh, j->p.spp, j->p.tonemap, ctx->stream));
+    }
+    HIPTRY(ctx, hipMemcpyAsync(ctx->h_u64, ctx->d_seg_total, sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx->stream));
+    HIPTRY(ctx, hipMemcpyAsync(ctx->h_u64 + 1, ctx->d_trav, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx->stream));
+    HIPTRY(ctx, hipStreamSynchronize(ctx->stream));
+    float ms = 0.0f;
+    HIPTRY(ctx, hipEventElapsedTime(&ms, e0, e1));
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    double ext_ms = 0.0;
+    for (auto& pr : prof.ev) {
+        float m = 0.0f;
+        hipEventElapsedTime(&m, pr.first, pr.second);
+        ext_ms += m;
+        hipEventDestroy(pr.first);
+        hipEventDestroy(pr.second);
+    }
+    const unsigned long long segs = ctx->h_u64[0] - ctx->h_u64[3];
+    if (info) {
+        info->segments = segs;
+        info->device_ms = ms;
+        info->extend_ms = ext_ms;
+        info->extend_launches = (int32_t)prof.ev.size();
+        info->chunks = chunks;
+        info->paths_per_chunk = max_paths;
+        info->box_tests = ctx->h_u64[1];
+        info->tri_tests = ctx->h_u64[2];
+    }
+    {
+        std::lock_guard<std::mutex> lk(ctx->sched_mu);
+        ctx->stat_busy_ms += ms;
+    }
+    return ILM_OK;
+}
+
+static int copy_job_outputs(ilm_ctx* ctx, Job* j, float* accum_rgb, ilm_pixel* out) {
+    const int64_t wh = (int64_t)j->p.width * j->p.height;
+    if (accum_rgb) {
+        std::vector<float4> a4(wh);
+        HIPTRY(ctx, hipMemcpy(a4.data(), j->d_accum, wh * sizeof(float4), hipMemcpyDeviceToHost));
+        accum4_to_rgb(a4, accum_rgb, wh);
+    }
+    if (out) HIPTRY(ctx, hipMemcpy(out, j->d_px, wh * sizeof(ilm_pixel), hipMemcpyDeviceToHost));
+    return ILM_OK;
+}
+
+int ilm_render(ilm_ctx* ctx, ilm_scene* scene, const ilm_render_params* params, float* accum_rgb, ilm_pixel* out,
+               ilm_render_info* info) {
+    if (!ctx || !scene) return fail(ctx, ILM_E_INVALID, "null context or scene");
+    int rc = validate_params(ctx, params);
+    if (rc) return rc;
+    Job j;
+    j.scene = scene;
+    j.p = *params;
+    std::vector<Job*> jobs = {&j};
+    if (info) std::memset(info, 0, sizeof(*info));
+    rc = run_jobs_sync(ctx, jobs, info);
+    if (!rc) rc = copy_job_outputs(ctx, &j, accum_rgb, out);
+    job_release_device(&j);
+    return rc;
+}
+
+int ilm_render_batch(ilm_ctx* ctx, int32_t n, ilm_scene* const* scenes, const ilm_render_params* params,
+                     float* const* accum_rgb, ilm_pixel* const* out, ilm_render_info* info) {
+    if (!ctx || n <= 0 || !scenes || !params) return fail(ctx, ILM_E_INVALID, "bad batch arguments");
+    std::vector<Job> js(n);
+    std::vector<Job*> jobs(n);
+    for (int i = 0; i < n; i++) {
+        int rc = validate_params(ctx, &params[i]);
+        if (rc) return rc;
+        if (!scenes[i]) return fail(ctx, ILM_E_INVALID, "null scene in batch");
+        js[i].scene = scenes[i];
+        js[i].p = params[i];
+        jobs[i] = &js[i];
+    }
+    if (info) std::memset(info, 0, sizeof(*info));
+    int rc = ILM_OK;
+    /* requests beyond one chunk's job table run in consecutive groups */
+    for (int g = 0; g < n && !rc; g += ILM_MAX_JOBS) {
+        std::vector<Job*> grp(jobs.begin() + g, jobs.begin() + std::min(n, g + ILM_MAX_JOBS));
+        ilm_render_info gi;
+        std::memset(&gi, 0, sizeof(gi));
+        rc = run_jobs_sync(ctx, grp, &gi);
+        if (info) {
+            info->segments += gi.segments;
+            info->device_ms += gi.device_ms;
+            info->extend_ms += gi.extend_ms;
+            info->extend_launches += gi.extend_launches;
+            info->chunks += gi.chunks;
+            info->paths_per_chunk = std::max(info->paths_per_chunk, gi.paths_per_chunk);
+            info->box_tests += gi.box_tests;
+            info->tri_tests += gi.tri_tests;
+        }
+        for (size_t k = 0; k < grp.size() && !rc; k++) {
+            int i = g + (int)k;
+            rc = copy_job_outputs(ctx, grp[k], accum_rgb ? accum_rgb[i] : nullptr, out ? out[i] : nullptr);
+        }
+        for (Job* j : grp) job_release_device(j);
+    }
+    return rc;
+}
+
+/* ======================= parity hooks ======================= */
+
+int ilm_debug_closest_hit(ilm_ctx* ctx, ilm_scene* scene, const float* rays, int32_t n, int32_t* inst,
+                          int32_t* prim, float* t) {
+    if (!ctx || !scene || !rays || n < 0 || !inst || !prim || !t) return fail(ctx, ILM_E_INVALID, "bad arguments");
+    std::lock_guard<std::mutex> dev(ctx->dev_mu);
+    HIPTRY(ctx, hipSetDevice(ctx->device));
+    std::unique_ptr<DChunk> hc(new DChunk());
+    std::memset(hc.get(), 0, sizeof(DChunk));
+    hc->reqs[0].scene = scene->d_scene;
+    std::vector<float4> o4, d4, h4;
+    std::vector<int> hi;
+    for (int64_t base = 0; base < n; base += ctx->cap) {
+        int64_t m = std::min<int64_t>(ctx->cap, n - base);
+        o4.resize(m);
+        d4.resize(m);
+        for (int64_t i = 0; i < m; i++) {
+            const float* r = rays + 6 * (base + i);
+            o4[i] = make_float4(r[0], r[1], r[2], 0.0f);
+            d4[i] = make_float4(r[3], r[4], r[5], 0.0f);
+        }
+        hc->total_paths = (uint32_t)m;
+        hc->depth_steps = 1;
+        int rc = stage_chunk(ctx, *hc);
+        if (rc) return rc;
+        HIPTRY(ctx, hipMemcpyAsync(ctx->A.o, o4.data(), m * sizeof(float4), hipMemcpyHostToDevice, ctx->stream));
+        HIPTRY(ctx, hipMemcpyAsync(ctx->A.d, d4.data(), m * sizeof(float4), hipMemcpyHostToDevice, ctx->stream));
+        KParams P = base_params(ctx);
+        P.trav_stats = nullptr;
+        HIPTRY(ctx, ilm_k_ctl_init(P, ctx->stream));
+        HIPTRY(ctx, ilm_k_extend(P, 0, ctx->ovf, ctx->ext_grid, ctx->stream));
+        h4.resize(m);
+        hi.resize(m);
+        HIPTRY(ctx, hipMemcpyAsync(h4.data(), ctx->hits.h, m * sizeof(float4), hipMemcpyDeviceToHost, ctx->stream));
+        HIPTRY(ctx, hipMemcpyAsync(hi.data(), ctx->hits.inst, m * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        HIPTRY(ctx, hipStreamSynchronize(ctx->stream));
+        for (int64_t i = 0; i < m; i++) {
+            int32_t pr;
+            std::memcpy(&pr, &h4[i].w, 4);
+            inst[base + i] = hi[i];
+            prim[base + i] = hi[i] >= 0 ? pr : -1;
+            t[base + i] = hi[i] == -1 ? 3.402823466e+38f : h4[i].x;
+        }
+    }
+    return ILM_OK;
+}
+
+int ilm_debug_path_radiance(ilm_ctx* ctx, ilm_scene* scene, const ilm_render_params* params,
+                            const uint32_t* pixel_pass, int32_t n, float* rgb, int32_t* segments) {
+    if (!ctx || !scene || !pixel_pass || n < 0 || !rgb) return fail(ctx, ILM_E_INVALID, "bad arguments");
+    int rc = validate_params(ctx, params);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> dev(ctx->dev_mu);
+    HIPTRY(ctx, hipSetDevice(ctx->device));
+    if (!ctx->seg_count) HIPTRY(ctx, hipMalloc((void**)&ctx->seg_count, ctx->cap * sizeof(int)));
+    uint32_t* d_list = nullptr;
+    HIPTRY(ctx, hipMalloc((void**)&d_list, 2 * std::min<int64_t>(ctx->cap, n) * sizeof(uint32_t) + 8));
+    std::unique_ptr<DChunk> hc(new DChunk());
+    std::memset(hc.get(), 0, sizeof(DChunk));
+    DReq& R = hc->reqs[0];
+    R.scene = scene->d_scene;
+    R.width = params->width;
+    R.height = params->height;
+    R.max_depth = params->max_depth;
+    R.ray_bias = params->ray_bias;
+    R.tonemap = params->tonemap;
+    R.spp = params->spp;
+    hm::render_info(scene->camera, params->width, params->height, R.image_width, R.pixel_size, R.left, R.top);
+    std::vector<float4> c4;
+    std::vector<int> sc;
+    for (int64_t base = 0; base < n; base += ctx->cap) {
+        int64_t m = std::min<int64_t>(ctx->cap, n - base);
+        hc->total_paths = (uint32_t)m;
+        hc->depth_steps = params->max_depth;
+        rc = stage_chunk(ctx, *hc);
+        if (rc) break;
+        if (hipMemcpyAsync(d_list, pixel_pass + 2 * base, 2 * m * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream) != hipSuccess) {
+            rc = fail(ctx, ILM_E_DEVICE, "list upload");
+            break;
+        }
+        KParams P = base_params(ctx);
+        P.trav_stats = nullptr;
+        P.list = d_list;
+        P.seg_count = ctx->seg_count;
+        if (ilm_k_ctl_init(P, ctx->stream) != hipSuccess || ilm_k_raygen(P, ctx->gen_grid, ctx->stream) != hipSuccess) {
+            rc = fail(ctx, ILM_E_DEVICE, "raygen launch");
+            break;
+        }
+        for (int d = 0; d < params->max_depth; d++) {
+            P.in = (d & 1) ? ctx->B : ctx->A;
+            P.out = (d & 1) ? ctx->A : ctx->B;
+            if (ilm_k_extend(P, d, ctx->ovf, ctx->ext_grid, ctx->stream) != hipSuccess ||
+                ilm_k_shade(P, d, ctx->shade_grid, ctx->stream) != hipSuccess) {
+                rc = fail(ctx, ILM_E_DEVICE, "depth launch");
+                break;
+            }
+        }
+        if (rc) break;
+        c4.resize(m);
+        sc.resize(m);
+        if (hipMemcpyAsync(c4.data(), ctx->contrib, m * sizeof(float4), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+            hipMemcpyAsync(sc.data(), ctx->seg_count, m * sizeof(int), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+            hipStreamSynchronize(ctx->stream) != hipSuccess) {
+            rc = fail(ctx, ILM_E_DEVICE, "readback");
+            break;
+        }
+        for (int64_t i = 0; i < m; i++) {
+            rgb[3 * (base + i)] = c4[i].x;
+            rgb[3 * (base + i) + 1] = c4[i].y;
+            rgb[3 * (base + i) + 2] = c4[i].z;
+            if (segments) segments[base + i] = sc[i];
+        }
+    }
+    hipFree(d_list);
+    return rc;
+}
+
+/* ======================= served form ======================= */
+
+static int admit_locked(ilm_ctx* ctx);
+
+static void server_loop(ilm_ctx* ctx) {
+    hipSetDevice(ctx->device);
+    std::unique_ptr<DChunk> hc(new DChunk());
+    std::vector<Job*> in_chunk;
+    struct Inflight { hipEvent_t ev; std::vector<Job*> finishing; };
+    std::deque<Inflight> inflight;
+    for (;;) {
+        std::vector<Job*> active;
+        {
+            std::unique_lock<std::mutex> lk(ctx->sched_mu);
+            ctx->sched_cv.wait(lk, [&] {
+                return ctx->stop || !ctx->pending.empty() || !ctx->active.empty() || !inflight.empty();
+            });
+            if (ctx->stop && ctx->pending.empty() && ctx->active.empty() && inflight.empty()) break;
+            admit_locked(ctx);
+            active = ctx->active;
+        }
+        /* retire cancelled jobs that have nothing in flight */
+        bool launched = false;
+        {
+            std::lock_guard<std::mutex> dev(ctx->dev_mu);
+            std::vector<Job*> issuable;
+            for (Job* j : active)
+                if (job_has_work(j) && !j->cancel.load()) issuable.push_back(j);
+            if (!issuable.empty() && inflight.size() < 2) {
+                build_chunk(ctx, issuable, *hc, in_chunk);
+                for (int w = 0; w < hc->n_work; w++) hc->work[w].pass0 += in_chunk[hc->work[w].req]->p.first_sample;
+                int rc = launch_chunk(ctx, *hc, nullptr);
+                Inflight f;
+                hipEventCreateWithFlags(&f.ev, hipEventDisableTiming);
+                for (Job* j : in_chunk) {
+                    if (!job_has_work(j)) {
+                        const int64_t wh = (int64_t)j->p.width * j->p.height;
+                        ilm_k_tonemap(j->d_accum, j->d_px, (int)wh, j->p.spp, j->p.tonemap, ctx->stream);
+                        hipMemcpyAsync(j->h_accum, j->d_accum, wh * sizeof(float4), hipMemcpyDeviceToHost, ctx->stream);
+                        hipMemcpyAsync(j->h_px, j->d_px, wh * sizeof(ilm_pixel), hipMemcpyDeviceToHost, ctx->stream);
+                        f.finishing.push_back(j);
+                    }
+                }
+                hipEventRecord(f.ev, ctx->stream);
+                for (Job* j : in_chunk) j->last_ev_seq = ctx->chunk_seq;
+                ctx->chunk_seq++;
+                if (rc) {
+                    for (Job* j : in_chunk) j->failed = true;
+                }
+                inflight.push_back(f);
+                launched = true;
+            }
+        }
+        /* retire finished chunks; block on the oldest when nothing new could be launched */
+        while (!inflight.empty()) {
+            Inflight& f = inflight.front();
+            hipError_t q = launched ? hipEventQuery(f.ev) : hipEventSynchronize(f.ev);
+            if (q == hipErrorNotReady) break;
+            double t = now_ms(ctx);
+            std::vector<Job*> done = f.finishing;
+            hipEventDestroy(f.ev);
+            inflight.pop_front();
+            ctx->chunks_retired++;
+            std::lock_guard<std::mutex> lk(ctx->sched_mu);
+            for (Job* j : done) {
+                j->end_ms = t;
+                job_release_device(j);
+                ctx->in_use -= j->bytes;
+                ctx->active.erase(std::find(ctx->active.begin(), ctx->active.end(), j));
+                ctx->latencies.push_back(t - j->submit_ms);
+                ctx->stat_done++;
+                j->state.store(j->failed ? ILM_REQ_FAILED : ILM_REQ_DONE);
+                j->samples_done = j->p.spp;
+                j->cv.notify_all();
+                if (--j->refs == 0) { scene_release(j->scene); delete j; }
+            }
+            launched = false;
+        }
+        /* cancelled jobs: release once no chunk that used them is still in flight */
+        {
+            std::lock_guard<std::mutex> lk(ctx->sched_mu);
+            for (size_t k = 0; k < ctx->active.size();) {
+                Job* j = ctx->active[k];
+                bool busy = j->last_ev_seq >= ctx->chunk_seq - (int64_t)inflight.size();
+                if (j->cancel.load() && !busy) {
+                    hipStreamSynchronize(ctx->stream);
+                    job_release_device(j);
+                    ctx->in_use -= j->bytes;
+                    ctx->active.erase(ctx->active.begin() + k);
+                    j->end_ms = now_ms(ctx);
+                    ctx->stat_cancelled++;
+                    j->state.store(ILM_REQ_CANCELLED);
+                    j->cv.notify_all();
+                    if (--j->refs == 0) { scene_release(j->scene); delete j; }
+                    continue;
+                }
+                k++;
+            }
+        }
+    }
+}
+
+/* Admit queued jobs while the HBM budget allows. Caller holds sched_mu. */
+static int admit_locked(ilm_ctx* ctx) {
+    while (!ctx->pending.empty() && (int)ctx->active.size() < ILM_MAX_JOBS) {
+        Job* j = ctx->pending.front();
+        if (j->cancel.load()) {
+            ctx->pending.pop_front();
+            j->state.store(ILM_REQ_CANCELLED);
+            j->end_ms = now_ms(ctx);
+            ctx->stat_cancelled++;
+            j->cv.notify_all();
+            if (--j->refs == 0) { scene_release(j->scene); delete j; }
+            continue;
+        }
+        if (ctx->in_use + j->bytes > ctx->budget && !ctx->active.empty()) break;
+        ctx->pending.pop_front();
+        {
+            std::lock_guard<std::mutex> dev(ctx->dev_mu);
+            if (job_prepare(ctx, j) != ILM_OK) {
+                job_release_device(j);
+                j->state.store(ILM_REQ_FAILED);
+                j->cv.notify_all();
+                if (--j->refs == 0) { scene_release(j->scene); delete j; }
+                continue;
+            }
+        }
+        ctx->in_use += j->bytes;
+        j->start_ms = now_ms(ctx);
+        j->state.store(ILM_REQ_RUNNING);
+        ctx->active.push_back(j);
+    }
+    return ILM_OK;
+}
+
+static void server_start(ilm_ctx* ctx) {
+    if (ctx->server.joinable()) return;
+    ctx->stop = false;
+    ctx->server = std::thread(server_loop, ctx);
+}
+
+static void server_stop(ilm_ctx* ctx) {
+    {
+        std::lock_guard<std::mutex> lk(ctx->sched_mu);
+        ctx->stop = true;
+        for (Job* j : ctx->pending) j->cancel.store(true);
+        for (Job* j : ctx->active) j->cancel.store(true);
+    }
+    ctx->sched_cv.notify_all();
+    if (ctx->server.joinable()) ctx->server.join();
+}
+
+int ilm_submit(ilm_ctx* ctx, ilm_scene* scene, const ilm_render_params* params, ilm_req** out) {
+    if (!ctx || !scene || !out) return fail(ctx, ILM_E_INVALID, "null argument");
+    *out = nullptr;
+    int rc = validate_params(ctx, params);
+    if (rc) return rc;
+    const int64_t wh = (int64_t)params->width * params->height;
+    Job* j = new Job();
+    j->scene = scene;
+    j->p = *params;
+    j->bytes = wh * (2 * (int64_t)sizeof(float4) + (int64_t)sizeof(ilm_pixel));
+    if (hipHostMalloc((void**)&j->h_accum, wh * sizeof(float4), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&j->h_px, wh * sizeof(ilm_pixel), hipHostMallocDefault) != hipSuccess) {
+        if (j->h_accum) hipHostFree(j->h_accum);
+        delete j;
+        return fail(ctx, ILM_E_NOMEM, "pinned result buffers");
+    }
+    {
+        std::lock_guard<std::mutex> lk(ctx->sched_mu);
+        int maxq = ctx->opts.max_queued > 0 ? ctx->opts.max_queued : 4096;
+        if ((int)ctx->pending.size() >= maxq) {
+            ctx->stat_rejected++;
+            hipHostFree(j->h_accum);
+            hipHostFree(j->h_px);
+            delete j;
+            return fail(ctx, ILM_E_BUSY, "admission queue full");
+        }
+        if (j->bytes > ctx->budget) {
+            ctx->stat_rejected++;
+            hipHostFree(j->h_accum);
+            hipHostFree(j->h_px);
+            delete j;
+            return fail(ctx, ILM_E_NOMEM, "request exceeds the HBM budget");
+        }
+        scene->refs.fetch_add(1);
+        j->refs = 2;   /* scheduler + caller handle */
+        j->submit_ms = now_ms(ctx);
+        j->state.store(ILM_REQ_QUEUED);
+        ctx->pending.push_back(j);
+        server_start(ctx);
+    }
+    ctx->sched_cv.notify_all();
+    ilm_req* r = new ilm_req();
+    r->job = j;
+    r->ctx = ctx;
+    *out = r;
+    return ILM_OK;
+}
+
+int ilm_poll(ilm_req* r, ilm_req_status* st) {
+    if (!r || !st) return fail(nullptr, ILM_E_INVALID, "null argument");
+    Job* j = r->job;
+    std::lock_guard<std::mutex> lk(r->ctx->sched_mu);
+    st->state = j->state.load();
+    st->samples_done = j->state.load() == ILM_REQ_DONE ? j->p.spp : j->next_pass;
+    st->segments = 0;
+    st->submit_ms = j->submit_ms;
+    st->start_ms = j->start_ms;
+    st->end_ms = j->end_ms;
+    return ILM_OK;
+}
+
+int ilm_wait(ilm_req* r, int32_t timeout_ms) {
+    if (!r) return fail(nullptr, ILM_E_INVALID, "null request");
+    Job* j = r->job;
+    std::unique_lock<std::mutex> lk(r->ctx->sched_mu);
+    auto done = [&] { int s = j->state.load(); return s == ILM_REQ_DONE || s == ILM_REQ_CANCELLED || s == ILM_REQ_FAILED; };
+    if (timeout_ms < 0) j->cv.wait(lk, done);
+    else if (!j->cv.wait_for(lk, std::chrono::milliseconds(timeout_ms), done)) return fail(r->ctx, ILM_E_TIMEOUT, "wait timed out");
+    int s = j->state.load();
+    if (s == ILM_REQ_CANCELLED) return fail(r->ctx, ILM_E_CANCELLED, "request cancelled");
+    if (s == ILM_REQ_FAILED) return fail(r->ctx, ILM_E_DEVICE, "request failed");
+    return ILM_OK;
+}
+
+int ilm_result(ilm_req* r, float* accum_rgb, ilm_pixel* out) {
+    if (!r) return fail(nullptr, ILM_E_INVALID, "null request");
+    Job* j = r->job;
+    if (j->state.load() != ILM_REQ_DONE) return fail(r->ctx, ILM_E_STATE, "request not done");
+    const int64_t wh = (int64_t)j->p.width * j->p.height;
+    if (accum_rgb)
+        for (int64_t i = 0; i < wh; i++) {
+            accum_rgb[3 * i] = j->h_accum[i].x;
+            accum_rgb[3 * i + 1] = j->h_accum[i].y;
+            accum_rgb[3 * i + 2] = j->h_accum[i].z;
+        }
+    if (out) std::memcpy(out, j->h_px, wh * sizeof(ilm_pixel));
+    return ILM_OK;
+}
+
+int ilm_cancel(ilm_req* r) {
+    if (!r) return fail(nullptr, ILM_E_INVALID, "null request");
+    r->job->cancel.store(true);
+    r->ctx->sched_cv.notify_all();
+    return ILM_OK;
+}
+
+void ilm_req_free(ilm_req* r) {
+    if (!r) return;
+    Job* j = r->job;
+    ilm_ctx* ctx = r->ctx;
+    bool del = false;
+    {
+        std::lock_guard<std::mutex> lk(ctx->sched_mu);
+        j->cancel.store(true);
+        del = (--j->refs == 0);
+    }
+    ctx->sched_cv.notify_all();
+    if (del) {
+        scene_release(j->scene);
+        delete j;
+    }
+    delete r;
+}
+
+Job::~Job() {
+    if (h_accum) hipHostFree(h_accum);
+    if (h_px) hipHostFree(h_px);
+}
+
+int ilm_stats_get(ilm_ctx* ctx, ilm_stats* st) {
+    if (!ctx || !st) return fail(ctx, ILM_E_INVALID, "null argument");
+    unsigned long long segs = 0;
+    hipSetDevice(ctx->device);
+    hipStreamSynchronize(ctx->stream);
+    hipMemcpy(&segs, ctx->d_seg_total, sizeof(segs), hipMemcpyDeviceToHost);
+    std::lock_guard<std::mutex> lk(ctx->sched_mu);
+    std::memset(st, 0, sizeof(*st));
+    st->segments = segs;
+    st->requests_done = ctx->stat_done;
+    st->requests_rejected = ctx->stat_rejected;
+    st->requests_cancelled = ctx->stat_cancelled;
+    st->busy_ms = ctx->stat_busy_ms;
+    std::vector<double> l = ctx->latencies;
+    if (!l.empty()) {
+        std::sort(l.begin(), l.end());
+        auto q = [&](double f) {
+            double pos = f * (double)(l.size() - 1);
+            size_t i = (size_t)pos;
+            double fr = pos - (double)i;
+            return i + 1 < l.size() ? l[i] * (1 - fr) + l[i + 1] * fr : l[i];
+        };
+        st->p50_ms = q(0.50);
+        st->p99_ms = q(0.99);
+    }
+    st->hbm_budget_bytes = ctx->budget;
+    st->hbm_in_use_bytes = ctx->in_use;
+    return ILM_OK;
+}

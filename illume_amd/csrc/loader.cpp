@@ -1,0 +1,607 @@
+/*
+ * loader.cpp - reference-schema scene JSON and OBJ meshes to an ilm_scene_desc (host only).
+ *
+ * Reads the schema of HostScene (src/scene/scene.cpp:11-99) and Renderer's render_settings
+ * (src/renderer.cu:39-59): environment_map, camera, mediums, materials, meshes, primitives.
+ * Field defaults follow the reference loaders: texture_from_json (src/texture.cu:7-71),
+ * material_from_json (src/material.cu:7-97), medium_from_json (src/medium.cu:5-14),
+ * camera_from_json (src/camera.cu:20-30), transform_from_json (src/math/transform.cu:28-44),
+ * mesh_from_json (src/primitives/mesh.cu:430-443), Tonemapper (src/tonemapper.cu:5-30).
+ * Fallbacks keep the reference's: unknown material type -> black diffuse, unknown conductor ->
+ * "Cu", unknown mesh name -> mesh 0, missing resolution -> 512x512.
+ * Additive keys (no reference counterpart): material "eta"/"k" (explicit conductor IOR) and
+ * "distribution": "ggx"; bitmap textures read PFM (float) and binary PPM (8-bit) files.
+ * The OBJ reader accepts faces with more than three vertices (fan triangulation) and long
+ * lines; vertex merging assigns indices in first-occurrence order as src/primitives/mesh.cu:264-340.
+ */
+#include <cctype>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <memory>
+#include <sstream>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/illume_c.h"
+
+namespace {
+
+/* ---------------- minimal JSON ---------------- */
+struct JV {
+    enum T { NUL, BOOL, NUM, STR, ARR, OBJ } t = NUL;
+    bool b = false;
+    double n = 0;
+    std::string s;
+    std::vector<JV> a;
+    std::vector<std::pair<std::string, JV>> o;
+    const JV* get(const char* k) const {
+        if (t != OBJ) return nullptr;
+        for (auto& kv : o) if (kv.first == k) return &kv.second;
+        return nullptr;
+    }
+};
+
+struct Parser {
+    const char* p;
+    const char* e;
+    std::string err;
+    void ws() {
+        for (;;) {
+            while (p < e && std::isspace((unsigned char)*p)) p++;
+            if (p + 1 < e && p[0] == '/' && p[1] == '/') { while (p < e && *p != '\n') p++; continue; }
+            break;
+        }
+    }
+    bool parse(JV& v) {
+        ws();
+        if (p >= e) { err = "unexpected end"; return false; }
+        char c = *p;
+        if (c == '{') {
+            v.t = JV::OBJ;
+            p++;
+            ws();
+            if (p < e && *p == '}') { p++; return true; }
+            for (;;) {
+                ws();
+                JV k;
+                if (p >= e || *p != '"' || !str(k.s)) { err = "expected key"; return false; }
+                ws();
+                if (p >= e || *p != ':') { err = "expected ':'"; return false; }
+                p++;
+                JV val;
+                if (!parse(val)) return false;
+                v.o.emplace_back(k.s, std::move(val));
+                ws();
+                if (p < e && *p == ',') { p++; ws(); if (p < e && *p == '}') { p++; return true; } continue; }
+                if (p < e && *p == '}') { p++; return true; }
+                err = "expected ',' or '}'";
+                return false;
+            }
+        }
+        if (c == '[') {
+            v.t = JV::ARR;
+            p++;
+            ws();
+            if (p < e && *p == ']') { p++; return true; }
+            for (;;) {
+                JV val;
+                if (!parse(val)) return false;
+                v.a.push_back(std::move(val));
+                ws();
+                if (p < e && *p == ',') { p++; ws(); if (p < e && *p == ']') { p++; return true; } continue; }
+                if (p < e && *p == ']') { p++; return true; }
+                err = "expected ',' or ']'";
+                return false;
+            }
+        }
+        if (c == '"') { v.t = JV::STR; return str(v.s); }
+        if (!strncmp(p, "true", 4)) { v.t = JV::BOOL; v.b = true; p += 4; return true; }
+        if (!strncmp(p, "false", 5)) { v.t = JV::BOOL; v.b = false; p += 5; return true; }
+        if (!strncmp(p, "null", 4)) { v.t = JV::NUL; p += 4; return true; }
+        char* end = nullptr;
+        v.n = std::strtod(p, &end);
+        if (end == p) { err = "bad value"; return false; }
+        v.t = JV::NUM;
+        p = end;
+        return true;
+    }
+    bool str(std::string& out) {
+        p++;
+        while (p < e && *p != '"') {
+            if (*p == '\\' && p + 1 < e) { p++; char c = *p; out.push_back(c == 'n' ? '\n' : c == 't' ? '\t' : c); p++; continue; }
+            out.push_back(*p++);
+        }
+        if (p >= e) { err = "unterminated string"; return false; }
+        p++;
+        return true;
+    }
+};
+
+/* ---------------- owned descriptor ---------------- */
+struct TexOwn {
+    std::vector<float> f;
+    std::vector<uint8_t> u8;
+};
+
+}  // namespace
+
+struct ilm_desc {
+    ilm_scene_desc desc;
+    ilm_render_params settings;
+    std::vector<ilm_mesh_desc> meshes;
+    std::vector<std::vector<float>> pos, nrm, uv;
+    std::vector<std::vector<int32_t>> idx;
+    std::vector<ilm_instance_desc> instances;
+    std::vector<ilm_sphere_desc> spheres;
+    std::vector<ilm_material_desc> materials;
+    std::vector<ilm_medium_desc> mediums;
+    std::vector<std::unique_ptr<TexOwn>> tex;
+    std::string err;
+};
+
+static thread_local std::string g_loader_err;
+
+namespace {
+
+bool get_vec3(const JV* v, float out[3]) {
+    if (!v) return false;
+    if (v->t == JV::NUM) { out[0] = out[1] = out[2] = (float)v->n; return true; }
+    if (v->t == JV::ARR && v->a.size() >= 3) {
+        for (int i = 0; i < 3; i++) out[i] = (float)v->a[i].n;
+        return true;
+    }
+    return false;
+}
+float get_num(const JV& o, const char* k, float def) {
+    const JV* v = o.get(k);
+    return (v && v->t == JV::NUM) ? (float)v->n : def;
+}
+bool get_bool(const JV& o, const char* k, bool def) {
+    const JV* v = o.get(k);
+    return (v && v->t == JV::BOOL) ? v->b : def;
+}
+std::string get_str(const JV& o, const char* k, const std::string& def = "") {
+    const JV* v = o.get(k);
+    return (v && v->t == JV::STR) ? v->s : def;
+}
+
+std::string dir_of(const std::string& path) {
+    size_t s = path.find_last_of('/');
+    return s == std::string::npos ? std::string(".") : path.substr(0, s);
+}
+std::string resolve(const std::string& file, const std::string& base) {
+    std::ifstream f(file);
+    if (f.good()) return file;
+    std::string alt = base + "/" + file;
+    std::ifstream g(alt);
+    if (g.good()) return alt;
+    /* reference scenes name paths from the repository root (e.g. "res/cube.obj") */
+    std::string up = dir_of(dir_of(base)) + "/" + file;
+    std::ifstream h(up);
+    if (h.good()) return up;
+    return file;
+}
+
+bool read_pfm(const std::string& path, int& w, int& h, std::vector<float>& rgba) {
+    FILE* f = fopen(path.c_str(), "rb");
+    if (!f) return false;
+    char magic[3] = {0};
+    double scale = 0;
+    if (fscanf(f, "%2s %d %d %lf", magic, &w, &h, &scale) != 4 || (strcmp(magic, "PF") && strcmp(magic, "Pf")) || w <= 0 || h <= 0) { fclose(f); return false; }
+    fgetc(f);
+    int ch = magic[1] == 'F' ? 3 : 1;
+    std::vector<float> raw((size_t)w * h * ch);
+    if (fread(raw.data(), 4, raw.size(), f) != raw.size()) { fclose(f); return false; }
+    fclose(f);
+    bool little = scale < 0;
+    rgba.assign((size_t)w * h * 4, 1.0f);
+    for (int y = 0; y < h; y++)
+        for (int x = 0; x < w; x++) {
+            /* PFM rows run bottom to top */
+            const float* src = &raw[((size_t)(h - 1 - y) * w + x) * ch];
+            for (int c = 0; c < 3; c++) {
+                float v = src[ch == 3 ? c : 0];
+                if (!little) { uint32_t u; memcpy(&u, &v, 4); u = __builtin_bswap32(u); memcpy(&v, &u, 4); }
+                rgba[((size_t)y * w + x) * 4 + c] = v;
+            }
+        }
+    return true;
+}
+bool read_ppm(const std::string& path, int& w, int& h, std::vector<uint8_t>& rgba) {
+    FILE* f = fopen(path.c_str(), "rb");
+    if (!f) return false;
+    char magic[3] = {0};
+    int mx = 0;
+    if (fscanf(f, "%2s %d %d %d", magic, &w, &h, &mx) != 4 || strcmp(magic, "P6") || mx != 255 || w <= 0 || h <= 0) { fclose(f); return false; }
+    fgetc(f);
+    std::vector<uint8_t> raw((size_t)w * h * 3);
+    if (fread(raw.data(), 1, raw.size(), f) != raw.size()) { fclose(f); return false; }
+    fclose(f);
+    rgba.assign((size_t)w * h * 4, 255);
+    for (size_t i = 0; i < (size_t)w * h; i++) for (int c = 0; c < 3; c++) rgba[4 * i + c] = raw[3 * i + c];
+    return true;
+}
+
+ilm_texture_desc tex_const(float r, float g, float b) {
+    ilm_texture_desc t;
+    memset(&t, 0, sizeof(t));
+    t.type = ILM_TEX_CONSTANT;
+    t.color[0] = r; t.color[1] = g; t.color[2] = b;
+    t.scale[0] = t.scale[1] = 1.0f;
+    return t;
+}
+
+/* src/texture.cu:7-71 */
+ilm_texture_desc texture_from_json(const JV& j, ilm_desc* D, const std::string& base) {
+    if (j.t == JV::NUM) return tex_const((float)j.n, (float)j.n, (float)j.n);
+    if (j.t == JV::ARR) {
+        float c[3] = {0, 0, 0};
+        get_vec3(&j, c);
+        return tex_const(c[0], c[1], c[2]);
+    }
+    std::string type = get_str(j, "type");
+    if (type == "constant") {
+        float c[3] = {0, 0, 0};
+        get_vec3(j.get("color"), c);
+        return tex_const(c[0], c[1], c[2]);
+    }
+    if (type == "checkerboard") {
+        ilm_texture_desc t = tex_const(0, 0, 0);
+        t.type = ILM_TEX_CHECKERBOARD;
+        get_vec3(j.get("on_color"), t.on);
+        get_vec3(j.get("off_color"), t.off);
+        const JV* sc = j.get("scale");
+        if (sc && sc->t == JV::ARR && sc->a.size() >= 2) { t.scale[0] = (float)sc->a[0].n; t.scale[1] = (float)sc->a[1].n; }
+        return t;
+    }
+    if (type == "bitmap") {
+        std::string file = get_str(j, "file");
+        if (file.empty()) return tex_const(0, 0, 0);
+        std::string path = resolve(file, base);
+        ilm_texture_desc t = tex_const(0, 0, 0);
+        const JV* sc = j.get("scale");
+        std::unique_ptr<TexOwn> own(new TexOwn());
+        int w = 0, h = 0;
+        bool ok = false;
+        if (path.size() > 4 && path.substr(path.size() - 4) == ".pfm") {
+            ok = read_pfm(path, w, h, own->f);
+            t.is_hdr = 1;
+        } else if (path.size() > 4 && path.substr(path.size() - 4) == ".ppm") {
+            ok = read_ppm(path, w, h, own->u8);
+            t.is_hdr = 0;
+        }
+        if (!ok) {
+            fprintf(stderr, "texture_from_json: cannot read %s, using black\n", path.c_str());
+            return tex_const(0, 0, 0);
+        }
+        t.type = ILM_TEX_BITMAP;
+        t.width = w;
+        t.height = h;
+        t.texels_f32 = own->f.empty() ? nullptr : own->f.data();
+        t.texels_u8 = own->u8.empty() ? nullptr : own->u8.data();
+        if (sc && sc->t == JV::ARR && sc->a.size() >= 2) { t.scale[0] = (float)sc->a[0].n; t.scale[1] = (float)sc->a[1].n; }
+        D->tex.push_back(std::move(own));
+        return t;
+    }
+    fprintf(stderr, "texture_from_json: invalid texture type %s\n", type.c_str());
+    return tex_const(0, 0, 0);
+}
+
+ilm_transform_desc transform_from_json(const JV* j) {
+    ilm_transform_desc t;
+    memset(&t, 0, sizeof(t));
+    t.scale[0] = t.scale[1] = t.scale[2] = 1.0f;
+    if (!j) return t;
+    get_vec3(j->get("translation"), t.translation);
+    get_vec3(j->get("scale"), t.scale);
+    get_vec3(j->get("rotation"), t.rotation);
+    if (j->get("lookat")) { t.has_lookat = 1; get_vec3(j->get("lookat"), t.lookat); }
+    return t;
+}
+
+/* src/conductor.h:18-25 (refractiveindex.info values at 700, 510, 440 nm) */
+struct CondEntry { const char* name; float eta[3]; float k[3]; };
+const CondEntry kConductors[] = {
+    {"Ag", {0.16761f, 0.14032f, 0.13529f}, {4.2868f, 2.8794f, 2.2916f}},
+    {"Al", {1.66130f, 0.82373f, 0.58461f}, {8.0444f, 5.9742f, 5.1875f}},
+    {"Au", {0.14311f, 0.37495f, 1.44247f}, {3.9831f, 2.3857f, 1.6032f}},
+    {"Cu", {0.21249f, 0.97909f, 1.3343f}, {4.1005f, 2.3691f, 2.2965f}},
+    {"Pb", {1.78000f, 1.74000f, 1.4400f}, {3.5700f, 3.3100f, 3.1800f}},
+};
+bool conductor_get(const std::string& name, float eta[3], float k[3]) {
+    for (const CondEntry& c : kConductors)
+        if (name == c.name) { for (int i = 0; i < 3; i++) { eta[i] = c.eta[i]; k[i] = c.k[i]; } return true; }
+    return false;
+}
+
+/* OBJ: src/primitives/mesh.cu:159-340, 363-388 */
+bool load_obj(const std::string& path, bool z_up, bool face_normals, bool has_tex, bool flip, ilm_desc* D, int mi) {
+    std::ifstream in(path);
+    if (!in.good()) return false;
+    std::vector<float> v, vt, vn;
+    struct FV { int p, t, n; };
+    std::vector<FV> tris;
+    std::string line;
+    auto fix = [](long i, size_t n) -> int { return i > 0 ? (int)(i - 1) : (int)((long)n + i); };
+    while (std::getline(in, line)) {
+        const char* s = line.c_str();
+        while (*s == ' ' || *s == '\t') s++;
+        if (s[0] == 'v' && s[1] == ' ') {
+            float a, b, c;
+            if (sscanf(s + 2, "%f %f %f", &a, &b, &c) == 3) {
+                if (z_up) { v.push_back(a); v.push_back(c); v.push_back(b); }
+                else { v.push_back(a); v.push_back(b); v.push_back(c); }
+            }
+        } else if (s[0] == 'v' && s[1] == 't') {
+            float a = 0, b = 0;
+            sscanf(s + 2, "%f %f", &a, &b);
+            vt.push_back(a); vt.push_back(b);
+        } else if (s[0] == 'v' && s[1] == 'n') {
+            float a, b, c;
+            if (sscanf(s + 2, "%f %f %f", &a, &b, &c) == 3) { vn.push_back(a); vn.push_back(b); vn.push_back(c); }
+        } else if (s[0] == 'f' && (s[1] == ' ' || s[1] == '\t')) {
+            std::istringstream ss(s + 2);
+            std::string tok;
+            std::vector<FV> poly;
+            while (ss >> tok) {
+                FV f = {-1, -1, -1};
+                long a = 0, b = 0, c = 0;
+                if (sscanf(tok.c_str(), "%ld/%ld/%ld", &a, &b, &c) == 3) f = {fix(a, v.size() / 3), fix(b, vt.size() / 2), fix(c, vn.size() / 3)};
+                else if (sscanf(tok.c_str(), "%ld//%ld", &a, &c) == 2) f = {fix(a, v.size() / 3), -1, fix(c, vn.size() / 3)};
+                else if (sscanf(tok.c_str(), "%ld/%ld", &a, &b) == 2) f = {fix(a, v.size() / 3), fix(b, vt.size() / 2), -1};
+                else if (sscanf(tok.c_str(), "%ld", &a) == 1) f = {fix(a, v.size() / 3), -1, -1};
+                else continue;
+                poly.push_back(f);
+            }
+            for (size_t k = 2; k < poly.size(); k++) { tris.push_back(poly[0]); tris.push_back(poly[k - 1]); tris.push_back(poly[k]); }
+        }
+    }
+    /* merge identical (position, normal, uv) vertices in first-occurrence order */
+    struct Key { uint32_t w[8]; bool operator==(const Key& o) const { return !memcmp(w, o.w, sizeof(w)); } };
+    struct KH { size_t operator()(const Key& k) const { size_t h = 1469598103934665603ull; for (uint32_t x : k.w) h = (h ^ x) * 1099511628211ull; return h; } };
+    std::unordered_map<Key, int, KH> map;
+    std::vector<float>& P = D->pos[mi];
+    std::vector<float>& N = D->nrm[mi];
+    std::vector<float>& T = D->uv[mi];
+    std::vector<int32_t>& I = D->idx[mi];
+    for (const FV& f : tris) {
+        if (f.p < 0 || 3 * (size_t)f.p + 2 >= v.size() + 0) { if (f.p < 0 || (size_t)f.p >= v.size() / 3) return false; }
+        float pos[3] = {v[3 * f.p], v[3 * f.p + 1], v[3 * f.p + 2]};
+        float nrm[3] = {0, 0, 0};
+        float tc[2] = {0, 0};
+        if (!face_normals) {
+            if (f.n < 0 || (size_t)f.n >= vn.size() / 3) return false;
+            float n0 = vn[3 * f.n], n1 = vn[3 * f.n + 1], n2 = vn[3 * f.n + 2];
+            float inv = 1.0f / std::sqrt((n0 * n0) + (n1 * n1) + (n2 * n2));
+            nrm[0] = n0 * inv; nrm[1] = n1 * inv; nrm[2] = n2 * inv;
+        }
+        if (has_tex) {
+            if (f.t < 0 || (size_t)f.t >= vt.size() / 2) return false;
+            tc[0] = vt[2 * f.t]; tc[1] = vt[2 * f.t + 1];
+        }
+        Key k;
+        memcpy(&k.w[0], pos, 12);
+        memcpy(&k.w[3], nrm, 12);
+        memcpy(&k.w[6], tc, 8);
+        auto it = map.find(k);
+        int id;
+        if (it == map.end()) {
+            id = (int)(P.size() / 3);
+            map.emplace(k, id);
+            P.insert(P.end(), pos, pos + 3);
+            N.insert(N.end(), nrm, nrm + 3);
+            T.insert(T.end(), tc, tc + 2);
+        } else {
+            id = it->second;
+        }
+        I.push_back(id);
+    }
+    if (flip) {
+        if (face_normals) for (size_t t = 0; t + 2 < I.size(); t += 3) std::swap(I[t + 1], I[t + 2]);
+        else for (float& x : N) x = -x;
+    }
+    return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ilm_desc_load_json(const char* path, ilm_desc** out) {
+    if (!path || !out) return ILM_E_INVALID;
+    *out = nullptr;
+    std::ifstream f(path, std::ios::binary);
+    if (!f.good()) { g_loader_err = std::string("cannot read ") + path; return ILM_E_IO; }
+    std::stringstream buf;
+    buf << f.rdbuf();
+    std::string text = buf.str();
+    Parser ps{text.data(), text.data() + text.size(), ""};
+    JV root;
+    if (!ps.parse(root) || root.t != JV::OBJ) {
+        g_loader_err = "JSON parse error at offset " + std::to_string(ps.p - text.data()) + ": " + ps.err;
+        return ILM_E_PARSE;
+    }
+    std::unique_ptr<ilm_desc> D(new ilm_desc());
+    const std::string base = dir_of(path);
+    memset(&D->desc, 0, sizeof(D->desc));
+    memset(&D->settings, 0, sizeof(D->settings));
+
+    if (const JV* env = root.get("environment_map")) {
+        D->desc.has_environment = 1;
+        D->desc.environment = texture_from_json(*env, D.get(), base);
+    }
+    ilm_camera_desc& cam = D->desc.camera;
+    cam.transform = transform_from_json(nullptr);
+    if (const JV* c = root.get("camera")) {
+        cam.fov = get_num(*c, "fov", 0);
+        cam.dof = get_num(*c, "depth_of_field", 0);
+        cam.aperture = get_num(*c, "aperture_radius", 0);
+        cam.transform = transform_from_json(c->get("transform"));
+    }
+    std::map<std::string, int> medium_idx;
+    if (const JV* ms = root.get("mediums")) {
+        for (const JV& m : ms->a) {
+            ilm_medium_desc md;
+            memset(&md, 0, sizeof(md));
+            float ab[3] = {0, 0, 0};
+            get_vec3(m.get("absorption"), ab);
+            for (int i = 0; i < 3; i++) md.sigma_a[i] = 1.0f - ab[i];   /* src/medium.cu:9 */
+            md.sigma_s = get_num(m, "scattering", 0);
+            md.g = get_num(m, "g", 0);
+            medium_idx[get_str(m, "name")] = (int)D->mediums.size();
+            D->mediums.push_back(md);
+        }
+    }
+    std::map<std::string, int> mat_idx;
+    auto black_diffuse = [&]() {
+        ilm_material_desc m;
+        memset(&m, 0, sizeof(m));
+        m.type = ILM_MAT_DIFFUSE;
+        m.albedo = tex_const(0, 0, 0);
+        m.roughness = tex_const(1e-4f, 1e-4f, 1e-4f);
+        m.medium = -1;
+        return m;
+    };
+    if (const JV* ms = root.get("materials")) {
+        for (const JV& j : ms->a) {
+            ilm_material_desc m = black_diffuse();
+            std::string type = get_str(j, "type");
+            if (const JV* a = j.get("albedo")) m.albedo = texture_from_json(*a, D.get(), base);
+            if (const JV* r = j.get("roughness")) m.roughness = texture_from_json(*r, D.get(), base);
+            auto mi = medium_idx.find(get_str(j, "medium"));
+            int medium = mi == medium_idx.end() ? -1 : mi->second;
+            m.ior = get_num(j, "ior", 0);
+            m.distribution = get_str(j, "distribution") == "ggx" ? ILM_DIST_GGX : ILM_DIST_BECKMANN;
+            if (type == "emissive") m.type = ILM_MAT_EMISSIVE;
+            else if (type == "lambert") m.type = ILM_MAT_DIFFUSE;
+            else if (type == "reflective") m.type = ILM_MAT_SPECULAR;
+            else if (type == "refractive") { m.type = ILM_MAT_REFRACTIVE; m.medium = medium; }
+            else if (type == "roughreflective") m.type = ILM_MAT_ROUGHREFLECTIVE;
+            else if (type == "roughrefractive") { m.type = ILM_MAT_ROUGHREFRACTIVE; m.medium = medium; }
+            else if (type == "conductor" || type == "roughconductor") {
+                m.type = type == "conductor" ? ILM_MAT_CONDUCTOR : ILM_MAT_ROUGHCONDUCTOR;
+                if (type == "conductor") m.roughness = tex_const(1e-4f, 1e-4f, 1e-4f);
+                m.albedo = tex_const(0, 0, 0);
+                if (j.get("eta") && j.get("k")) {
+                    get_vec3(j.get("eta"), m.eta);
+                    get_vec3(j.get("k"), m.k);
+                } else if (!conductor_get(get_str(j, "material"), m.eta, m.k)) {
+                    fprintf(stderr, "material_from_json: invalid conductor material, defaulting to copper\n");
+                    conductor_get("Cu", m.eta, m.k);
+                }
+            } else if (type == "plastic") m.type = ILM_MAT_PLASTIC;
+            else {
+                fprintf(stderr, "material_from_json: invalid material type %s\n", type.c_str());
+                m = black_diffuse();
+            }
+            /* material_base resets roughness for every type that does not take one (src/material.cu:99-109) */
+            if (m.type != ILM_MAT_ROUGHREFLECTIVE && m.type != ILM_MAT_ROUGHREFRACTIVE && m.type != ILM_MAT_ROUGHCONDUCTOR)
+                m.roughness = tex_const(1e-4f, 1e-4f, 1e-4f);
+            mat_idx[get_str(j, "name")] = (int)D->materials.size();
+            D->materials.push_back(m);
+        }
+    }
+    std::map<std::string, int> mesh_idx;
+    if (const JV* ms = root.get("meshes")) {
+        for (const JV& j : ms->a) {
+            int mi = (int)D->meshes.size();
+            D->pos.emplace_back(); D->nrm.emplace_back(); D->uv.emplace_back(); D->idx.emplace_back();
+            ilm_mesh_desc md;
+            memset(&md, 0, sizeof(md));
+            md.face_normals = get_bool(j, "face_normals", true);
+            md.has_texcoords = get_bool(j, "has_texcoords", false);
+            md.leaf_size = (int)get_num(j, "bvh_items_per_node", 4);
+            std::string file = resolve(get_str(j, "file"), base);
+            if (!load_obj(file, get_bool(j, "z_up", false), md.face_normals, md.has_texcoords, get_bool(j, "flip_normals", false), D.get(), mi)) {
+                g_loader_err = "cannot load mesh " + file;
+                return ILM_E_IO;
+            }
+            if (!mesh_idx.count(get_str(j, "name"))) mesh_idx[get_str(j, "name")] = mi;
+            D->meshes.push_back(md);
+        }
+    }
+    if (const JV* ps2 = root.get("primitives")) {
+        for (const JV& j : ps2->a) {
+            std::string type = get_str(j, "type");
+            std::string mname = get_str(j, "material");
+            int mat;
+            auto it = mat_idx.find(mname);
+            if (it != mat_idx.end()) mat = it->second;
+            else {
+                mat = (int)D->materials.size();
+                D->materials.push_back(black_diffuse());
+                mat_idx[mname] = mat;
+            }
+            if (type == "sphere") {
+                ilm_sphere_desc s;
+                memset(&s, 0, sizeof(s));
+                s.radius = get_num(j, "radius", 0);
+                get_vec3(j.get("center"), s.center);
+                s.material = mat;
+                D->spheres.push_back(s);
+            } else if (type == "mesh_instance") {
+                ilm_instance_desc in;
+                memset(&in, 0, sizeof(in));
+                auto mit = mesh_idx.find(get_str(j, "mesh"));
+                in.mesh = mit == mesh_idx.end() ? 0 : mit->second;
+                in.material = mat;
+                in.transform = transform_from_json(j.get("transform"));
+                if (D->meshes.empty()) { g_loader_err = "mesh_instance without meshes"; return ILM_E_PARSE; }
+                D->instances.push_back(in);
+            }
+        }
+    }
+    /* render_settings (src/renderer.cu:39-59, src/tonemapper.cu:5-30) */
+    ilm_render_params& rp = D->settings;
+    rp.width = rp.height = 512;
+    rp.spp = 1;
+    rp.max_depth = 1;
+    rp.ray_bias = 0.0f;
+    rp.tonemap = ILM_TONEMAP_LINEAR;
+    if (const JV* rs = root.get("render_settings")) {
+        rp.ray_bias = get_num(*rs, "ray_bias", 0.0f);
+        const JV* res = rs->get("resolution");
+        if (res && res->t == JV::ARR && res->a.size() >= 2) { rp.width = (int)res->a[0].n; rp.height = (int)res->a[1].n; }
+        std::string op = get_str(*rs, "tonemapper", "linear");
+        rp.tonemap = op == "Reinhard" ? ILM_TONEMAP_REINHARD : op == "Filmic" ? ILM_TONEMAP_FILMIC
+                   : op == "Uncharted" ? ILM_TONEMAP_UNCHARTED : ILM_TONEMAP_LINEAR;
+    }
+    for (size_t i = 0; i < D->meshes.size(); i++) {
+        ilm_mesh_desc& md = D->meshes[i];
+        md.num_vertices = (int32_t)(D->pos[i].size() / 3);
+        md.num_triangles = (int32_t)(D->idx[i].size() / 3);
+        md.positions = D->pos[i].data();
+        md.normals = md.face_normals ? nullptr : D->nrm[i].data();
+        md.texcoords = md.has_texcoords ? D->uv[i].data() : nullptr;
+        md.indices = D->idx[i].data();
+    }
+    ilm_scene_desc& sd = D->desc;
+    sd.num_meshes = (int32_t)D->meshes.size();
+    sd.meshes = D->meshes.data();
+    sd.num_instances = (int32_t)D->instances.size();
+    sd.instances = D->instances.data();
+    sd.num_spheres = (int32_t)D->spheres.size();
+    sd.spheres = D->spheres.data();
+    sd.num_materials = (int32_t)D->materials.size();
+    sd.materials = D->materials.data();
+    sd.num_mediums = (int32_t)D->mediums.size();
+    sd.mediums = D->mediums.data();
+    *out = D.release();
+    return ILM_OK;
+}
+
+const ilm_scene_desc* ilm_desc_scene(const ilm_desc* d) { return d ? &d->desc : nullptr; }
+int ilm_desc_settings(const ilm_desc* d, ilm_render_params* out) {
+    if (!d || !out) return ILM_E_INVALID;
+    *out = d->settings;
+    return ILM_OK;
+}
+void ilm_desc_free(ilm_desc* d) { delete d; }
+const char* ilm_loader_error(void) { return g_loader_err.c_str(); }
+
+}  /* extern "C" */

@@ -1,0 +1,307 @@
+/*
+ * illume_c.h - C ABI of the MI355X path-tracer hot path.
+ *
+ * Conventions
+ *   - Every int-returning entry point returns ILM_OK (0) or a negative ILM_E_* code.
+ *     Nothing calls exit(); the text of the last error is ilm_last_error(ctx)
+ *     (or ilm_global_error() for calls without a context).
+ *   - The caller owns every host buffer it passes in or out; the library owns device memory.
+ *   - Descriptors are read during the call only; the library copies what it keeps.
+ *   - ilm_scene handles stay device-resident until ilm_scene_free; requests in flight
+ *     keep their scene alive (reference counted).
+ *   - ilm_submit / ilm_poll / ilm_wait / ilm_cancel are thread-safe. One scheduler thread
+ *     per context drives the device.
+ *
+ * Reference interface each group replaces (matthewshang/illume):
+ *   ilm_scene_upload / ilm_scene_free    SceneRef(HostScene&) / ~SceneRef   src/scene/sceneref.cpp:9-148
+ *   ilm_desc_load_json                   HostScene(rapidjson::Document&)    src/scene/scene.cpp:11-99
+ *                                        Renderer ctor render_settings       src/renderer.cu:39-59
+ *   ilm_render                           Renderer::render_to_bitmap(Bitmap*) src/renderer.cu:498-597
+ *   ilm_pixel                            Pixel {size_t r,g,b}                src/bitmap.h:6-12
+ *   ilm_bvh_*                            bvh_create / GPUNode / BVH          src/accel/bvh.h:11-42
+ *   ilm_debug_closest_hit                get_min_hit                         src/renderer.cu:112-137
+ *   ilm_debug_path_radiance              init_rays + pathtrace_kernel        src/renderer.cu:81-110,139-388
+ *   ilm_submit .. ilm_stats, ilm_render_batch   no reference counterpart (served and batched forms)
+ */
+#ifndef ILLUME_C_H
+#define ILLUME_C_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ILM_ABI_VERSION 1
+
+enum {
+    ILM_OK = 0,
+    ILM_E_INVALID = -1,     /* bad argument or descriptor */
+    ILM_E_NOMEM = -2,       /* host or device allocation failed, or over the HBM budget */
+    ILM_E_DEVICE = -3,      /* HIP runtime error / no usable GPU */
+    ILM_E_IO = -4,          /* file could not be read */
+    ILM_E_PARSE = -5,       /* malformed scene JSON or OBJ */
+    ILM_E_BUSY = -6,        /* admission queue full */
+    ILM_E_CANCELLED = -7,   /* request was cancelled */
+    ILM_E_TIMEOUT = -8,     /* ilm_wait timed out */
+    ILM_E_UNSUPPORTED = -9,
+    ILM_E_STATE = -10       /* call not valid in the handle's current state */
+};
+
+/* Texture types (src/texture.h:8-11) */
+enum { ILM_TEX_CONSTANT = 0, ILM_TEX_CHECKERBOARD = 1, ILM_TEX_BITMAP = 2 };
+
+/* Material types, same numbering as src/material.h:12-20 */
+enum {
+    ILM_MAT_EMISSIVE = 0,
+    ILM_MAT_DIFFUSE = 1,
+    ILM_MAT_SPECULAR = 2,
+    ILM_MAT_REFRACTIVE = 3,
+    ILM_MAT_ROUGHREFLECTIVE = 4,
+    ILM_MAT_ROUGHREFRACTIVE = 5,
+    ILM_MAT_CONDUCTOR = 6,
+    ILM_MAT_ROUGHCONDUCTOR = 7,
+    ILM_MAT_PLASTIC = 8
+};
+
+/* Microfacet distribution. Beckmann is the reference's only model (src/microfacet.cu:7-52);
+ * GGX is an additive extension selected by the "distribution":"ggx" material key. */
+enum { ILM_DIST_BECKMANN = 0, ILM_DIST_GGX = 1 };
+
+/* Tonemap operators (src/tonemapper.h:11-14) */
+enum { ILM_TONEMAP_LINEAR = 0, ILM_TONEMAP_REINHARD = 1, ILM_TONEMAP_FILMIC = 2, ILM_TONEMAP_UNCHARTED = 3 };
+
+/* Node layouts of the device BVH */
+enum { ILM_BVH_FP32 = 0, ILM_BVH_QUANT8 = 1 };
+
+/* Ray/triangle test: Moller-Trumbore exactly as src/primitives/mesh.cu:446-476, or watertight
+ * (Woop, Benthin, Wald, JCGT 2013). */
+enum { ILM_TRI_MOLLER_TRUMBORE = 0, ILM_TRI_WATERTIGHT = 1 };
+
+typedef struct ilm_texture_desc {
+    int32_t type;               /* ILM_TEX_* */
+    float color[3];             /* constant */
+    float on[3];                /* checkerboard on colour */
+    float off[3];               /* checkerboard off colour */
+    float scale[2];             /* checkerboard and bitmap uv scale (default 1,1) */
+    int32_t width, height;      /* bitmap */
+    int32_t is_hdr;             /* 1: float RGBA texels, bilinear; 0: 8-bit RGBA texels, point */
+    const float* texels_f32;    /* width*height*4 when is_hdr */
+    const uint8_t* texels_u8;   /* width*height*4 when !is_hdr */
+} ilm_texture_desc;
+
+typedef struct ilm_medium_desc {
+    float sigma_a[3];           /* absorption coefficient = 1 - JSON "absorption" (src/medium.cu:9) */
+    float sigma_s;              /* scattering coefficient */
+    float g;                    /* Henyey-Greenstein asymmetry */
+} ilm_medium_desc;
+
+typedef struct ilm_material_desc {
+    int32_t type;               /* ILM_MAT_* */
+    ilm_texture_desc albedo;    /* default constant 0 */
+    ilm_texture_desc roughness; /* default constant 1e-4 (src/material.cu:33) */
+    float ior;
+    int32_t medium;             /* index into the scene's mediums, -1 = air */
+    float eta[3], k[3];         /* conductor complex IOR */
+    int32_t distribution;       /* ILM_DIST_* (rough materials) */
+} ilm_material_desc;
+
+typedef struct ilm_mesh_desc {
+    int32_t num_vertices;
+    const float* positions;     /* num_vertices*3 */
+    const float* normals;       /* num_vertices*3, required when !face_normals */
+    const float* texcoords;     /* num_vertices*2, required when has_texcoords */
+    int32_t num_triangles;
+    const int32_t* indices;     /* num_triangles*3 */
+    int32_t face_normals;       /* reference default 1 (src/primitives/mesh.cu:440) */
+    int32_t has_texcoords;
+    int32_t leaf_size;          /* reference "bvh_items_per_node" (default 4) */
+} ilm_mesh_desc;
+
+/* Object-to-world transform. With has_matrix=0 the library builds
+ * translate * (Rx*Ry*Rz) * scale from degrees (src/math/transform.cu:28-44), or a look-at
+ * frame when has_lookat=1; inverse and inverse-transpose follow src/math/transform.cu:6-16. */
+typedef struct ilm_transform_desc {
+    float translation[3];
+    float scale[3];
+    float rotation[3];          /* degrees */
+    int32_t has_lookat;
+    float lookat[3];
+    int32_t has_matrix;
+    float matrix[16];           /* row-major */
+} ilm_transform_desc;
+
+typedef struct ilm_instance_desc {
+    int32_t mesh;
+    int32_t material;
+    ilm_transform_desc transform;
+} ilm_instance_desc;
+
+typedef struct ilm_sphere_desc {
+    float center[3];
+    float radius;
+    int32_t material;
+} ilm_sphere_desc;
+
+typedef struct ilm_camera_desc {
+    float fov;                  /* horizontal, degrees */
+    float dof;                  /* image-plane distance ("depth_of_field") */
+    float aperture;             /* thin-lens radius; 0 = pinhole */
+    ilm_transform_desc transform;
+} ilm_camera_desc;
+
+typedef struct ilm_scene_desc {
+    int32_t num_meshes;        const ilm_mesh_desc* meshes;
+    int32_t num_instances;     const ilm_instance_desc* instances;
+    int32_t num_spheres;       const ilm_sphere_desc* spheres;
+    int32_t num_materials;     const ilm_material_desc* materials;
+    int32_t num_mediums;       const ilm_medium_desc* mediums;
+    int32_t has_environment;   ilm_texture_desc environment;
+    ilm_camera_desc camera;
+} ilm_scene_desc;
+
+typedef struct ilm_render_params {
+    int32_t width, height;
+    int32_t spp;
+    int32_t max_depth;
+    float ray_bias;
+    int32_t tonemap;            /* ILM_TONEMAP_* */
+    int32_t first_sample;       /* pass index of the first sample; pass i seeds wang_hash(i) */
+} ilm_render_params;
+
+/* Same layout as the reference's Pixel {size_t red, green, blue} on LP64 */
+typedef struct ilm_pixel {
+    uint64_t r, g, b;
+} ilm_pixel;
+
+typedef struct ilm_opts {
+    int32_t bvh_layout;         /* ILM_BVH_* (default FP32) */
+    int32_t tri_test;           /* ILM_TRI_* (default Moller-Trumbore) */
+    int64_t paths_in_flight;    /* 0 = automatic */
+    int64_t hbm_budget_bytes;   /* serving admission budget; 0 = 90% of free device memory */
+    int32_t use_graphs;         /* capture the per-chunk pipeline as a hipGraph */
+    int32_t max_queued;         /* serving: admission queue bound (0 = 4096) */
+    int32_t profile;            /* time every extend launch with HIP events (disables graphs) */
+    int32_t count_tests;        /* count box/triangle tests of the extend kernel (slower) */
+    int32_t verbose;
+} ilm_opts;
+
+typedef struct ilm_render_info {
+    uint64_t segments;          /* closest-hit queries, same count as src/renderer.cu:554 */
+    double device_ms;           /* device time of the render loop (HIP events) */
+    double extend_ms;           /* device time inside the extend kernel (ilm_opts.profile) */
+    int32_t extend_launches;    /* extend launches timed (ilm_opts.profile) */
+    int32_t chunks;             /* chunks the render was split into */
+    int64_t paths_per_chunk;
+    uint64_t box_tests;         /* child-box tests (ilm_opts.count_tests) */
+    uint64_t tri_tests;         /* triangle tests (ilm_opts.count_tests) */
+} ilm_render_info;
+
+typedef struct ilm_stats {
+    uint64_t segments;
+    uint64_t requests_done;
+    uint64_t requests_rejected;
+    uint64_t requests_cancelled;
+    double busy_ms;
+    double p50_ms, p99_ms;      /* request latency: completion - submission */
+    int64_t hbm_budget_bytes;
+    int64_t hbm_in_use_bytes;
+} ilm_stats;
+
+enum { ILM_REQ_QUEUED = 0, ILM_REQ_RUNNING = 1, ILM_REQ_DONE = 2, ILM_REQ_CANCELLED = 3, ILM_REQ_FAILED = 4 };
+
+typedef struct ilm_req_status {
+    int32_t state;              /* ILM_REQ_* */
+    int32_t samples_done;
+    uint64_t segments;
+    double submit_ms, start_ms, end_ms;   /* steady clock, ms since the context opened */
+} ilm_req_status;
+
+typedef struct ilm_bvh_info {
+    int32_t num_nodes;          /* inner + leaf nodes */
+    int32_t num_leaves;
+    int32_t max_depth;
+    int32_t max_leaf_size;
+    double sah_cost;            /* sum over nodes of SA(node)/SA(root) * (inner: 1, leaf: count) */
+} ilm_bvh_info;
+
+typedef struct ilm_ctx ilm_ctx;
+typedef struct ilm_scene ilm_scene;
+typedef struct ilm_req ilm_req;
+typedef struct ilm_desc ilm_desc;
+typedef struct ilm_bvh ilm_bvh;
+
+int ilm_version(void);
+const char* ilm_global_error(void);
+
+/* ---- context ---- */
+int ilm_open(int device, const ilm_opts* opts, ilm_ctx** out);
+void ilm_close(ilm_ctx* ctx);
+const char* ilm_last_error(ilm_ctx* ctx);
+int ilm_stats_get(ilm_ctx* ctx, ilm_stats* out);
+
+/* ---- scenes ---- */
+int ilm_scene_upload(ilm_ctx* ctx, const ilm_scene_desc* desc, ilm_scene** out);
+void ilm_scene_free(ilm_scene* scene);
+int64_t ilm_scene_device_bytes(const ilm_scene* scene);
+
+/* Reference-schema scene JSON (+ OBJ meshes) to a host descriptor; host only. */
+int ilm_desc_load_json(const char* path, ilm_desc** out);
+const ilm_scene_desc* ilm_desc_scene(const ilm_desc* d);
+int ilm_desc_settings(const ilm_desc* d, ilm_render_params* out);
+void ilm_desc_free(ilm_desc* d);
+const char* ilm_loader_error(void);
+
+/* ---- plain render (one request, synchronous) ----
+ * accum_rgb: W*H*3 floats, Kahan-compensated sum of the spp sample radiances (nullable).
+ * out: W*H pixels after tonemapping, as src/renderer.cu:426-441 (nullable). */
+int ilm_render(ilm_ctx* ctx, ilm_scene* scene, const ilm_render_params* params,
+               float* accum_rgb, ilm_pixel* out, ilm_render_info* info);
+
+/* ---- batched render: n requests' paths share every launch of one pipeline ---- */
+int ilm_render_batch(ilm_ctx* ctx, int32_t n, ilm_scene* const* scenes, const ilm_render_params* params,
+                     float* const* accum_rgb, ilm_pixel* const* out, ilm_render_info* info);
+
+/* ---- served form: continuous batching ---- */
+int ilm_submit(ilm_ctx* ctx, ilm_scene* scene, const ilm_render_params* params, ilm_req** out);
+int ilm_poll(ilm_req* req, ilm_req_status* out);
+int ilm_wait(ilm_req* req, int32_t timeout_ms);
+int ilm_result(ilm_req* req, float* accum_rgb, ilm_pixel* out);
+int ilm_cancel(ilm_req* req);
+void ilm_req_free(ilm_req* req);
+
+/* ---- BVH build (host only; the builder used by ilm_scene_upload) ---- */
+int ilm_bvh_build(const ilm_mesh_desc* mesh, ilm_bvh** out);
+int ilm_bvh_get_info(const ilm_bvh* b, ilm_bvh_info* out);
+/* boxes: num_nodes*6 (lo.xyz, hi.xyz); children: num_nodes*2 (inner: child node ids;
+ * leaf: -1, -1); leaf_first/leaf_count: num_nodes (first slot into tri_order, count; 0 for inner);
+ * tri_order: num_triangles */
+int ilm_bvh_export(const ilm_bvh* b, float* boxes, int32_t* children, int32_t* leaf_first,
+                   int32_t* leaf_count, int32_t* tri_order);
+/* 8-bit quantised BVH4 layout built from the same tree: node count, root child code
+ * (>= 0 node, < 0 leaf code) and depth; per node and child k < 4: the dequantised box as the
+ * kernel computes it (qboxes, 24 floats per node), the exact FP32 box it encodes (exact) and
+ * the child code (child: >= 0 node, leaf code with bit 31 set, 0x7fffffff unused). */
+int ilm_bvh_quantized_size(const ilm_bvh* b, int32_t* num_qnodes, int32_t* root, int32_t* max_depth);
+int ilm_bvh_export_quantized(const ilm_bvh* b, float* qboxes, float* exact, int32_t* child);
+void ilm_bvh_free(ilm_bvh* b);
+
+/* ---- parity hooks ---- */
+/* rays: n*6 floats (o.xyz, d.xyz) in world space. inst: instance index, or -2-s for sphere s,
+ * or -1 for a miss; prim: triangle index within the instance's mesh (-1 otherwise); t: hit distance. */
+int ilm_debug_closest_hit(ilm_ctx* ctx, ilm_scene* scene, const float* rays, int32_t n,
+                          int32_t* inst, int32_t* prim, float* t);
+/* Radiance of single paths (pixel, pass) with the render's RNG stream; rgb: n*3;
+ * segments: n (nullable) - number of closest-hit queries each path made. */
+int ilm_debug_path_radiance(ilm_ctx* ctx, ilm_scene* scene, const ilm_render_params* params,
+                            const uint32_t* pixel_pass, int32_t n, float* rgb, int32_t* segments);
+/* Instance i's matrices as the device uses them: out[0..15] object-to-world, [16..31] inverse,
+ * [32..47] inverse-transpose (row-major). */
+int ilm_debug_instance_matrices(const ilm_scene* scene, int32_t i, float* out48);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ILLUME_C_H */

@@ -1,0 +1,148 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so).  TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use this module, and only
+as the checker or the timed CPU baseline.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from illume_amd._abi import Pixel, RenderParams, SceneDesc, TransformDesc, CameraDesc, TextureDesc, scene_desc
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError("oracle/liboracle.so not built; run `python -m illume_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    P = C.POINTER
+    vp = C.c_void_p
+    fp = P(C.c_float)
+    ip = P(C.c_int32)
+    sig = {
+        "orc_threads_available": (C.c_int, []),
+        "orc_scene_create": (vp, [P(SceneDesc), C.c_int, C.c_int]),
+        "orc_scene_free": (None, [vp]),
+        "orc_ref_bvh_size": (C.c_int, [vp, C.c_int, ip]),
+        "orc_ref_bvh_export": (C.c_int, [vp, C.c_int, fp, P(C.c_uint32), ip]),
+        "orc_closest_hit": (C.c_int, [vp, fp, C.c_int, ip, ip, fp, C.c_int, ip, C.c_int]),
+        "orc_path_radiance": (C.c_int, [vp, P(RenderParams), P(C.c_uint32), C.c_int, vp, ip, C.c_int, C.c_int]),
+        "orc_render": (C.c_double, [vp, P(RenderParams), C.c_int64, C.c_int64, C.c_int, C.c_int, vp, P(Pixel),
+                                    P(C.c_uint64), C.c_int]),
+        "orc_tonemap_image": (C.c_int, [fp, C.c_int64, C.c_int, C.c_int, P(Pixel)]),
+        "orc_fresnel_dielectric": (C.c_float, [C.c_float, C.c_float, fp]),
+        "orc_fresnel_dielectric64": (C.c_double, [C.c_double, C.c_double]),
+        "orc_fresnel_conductor": (C.c_float, [C.c_float, C.c_float, C.c_float]),
+        "orc_diffuse_fresnel": (C.c_float, [C.c_float, C.c_int]),
+        "orc_triangle_intersect": (C.c_float, [fp, fp, fp, fp, fp, fp, fp]),
+        "orc_aabb_intersect": (C.c_float, [fp, fp, fp, fp]),
+        "orc_sphere_intersect": (C.c_int, [fp, C.c_float, fp, fp, fp, fp, fp]),
+        "orc_tonemap_eval": (None, [C.c_int, fp, fp]),
+        "orc_wang_hash": (C.c_uint32, [C.c_uint32]),
+        "orc_sample": (None, [C.c_int, C.c_float, fp, C.c_int, fp]),
+        "orc_g1_beckmann": (C.c_float, [fp, fp, C.c_float]),
+        "orc_g1_ggx": (C.c_float, [fp, fp, fp, C.c_float]),
+        "orc_to_basis": (None, [fp, fp, fp]),
+        "orc_transform_matrices": (None, [P(TransformDesc), fp, P(C.c_double)]),
+        "orc_render_info": (None, [P(CameraDesc), C.c_int, C.c_int, fp]),
+        "orc_texture_eval": (C.c_float, [P(TextureDesc), C.c_float, C.c_float, fp]),
+        "orc_pmath": (None, [C.c_int, fp, fp, C.c_int, fp]),
+        "orc_pcg32": (None, [C.c_uint64, C.c_uint64, C.c_int, P(C.c_uint32)]),
+        "orc_path_uniforms": (None, [C.c_uint32, C.c_uint32, C.c_int, fp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def f32p(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def i32p(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+def vec(x, dtype=np.float32):
+    return np.ascontiguousarray(np.asarray(x, dtype=dtype).reshape(-1))
+
+
+class OracleScene:
+    """The oracle's copy of a scene spec (FP32 faithful, optionally with the FP64 twin)."""
+
+    def __init__(self, spec, fp64=False, threads=0):
+        self.desc, self._keep = scene_desc(spec)
+        self.h = lib().orc_scene_create(C.byref(self.desc), int(fp64), int(threads))
+        self.fp64 = fp64
+
+    def free(self):
+        if self.h:
+            lib().orc_scene_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def closest_hit(self, rays, brute=False, counts=False, threads=0):
+        rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 6)
+        n = rays.shape[0]
+        inst = np.empty(n, np.int32)
+        prim = np.empty(n, np.int32)
+        t = np.empty(n, np.float32)
+        cnt = np.empty((n, 2), np.int32) if counts else None
+        lib().orc_closest_hit(self.h, f32p(rays), n, i32p(inst), i32p(prim), f32p(t), int(brute),
+                              i32p(cnt) if counts else None, int(threads))
+        return (inst, prim, t, cnt) if counts else (inst, prim, t)
+
+    def path_radiance(self, params, pixel_pass, fp64=False, threads=0):
+        pp = np.ascontiguousarray(pixel_pass, dtype=np.uint32).reshape(-1, 2)
+        n = pp.shape[0]
+        rgb = np.empty((n, 3), np.float64 if fp64 else np.float32)
+        seg = np.empty(n, np.int32)
+        rc = lib().orc_path_radiance(self.h, C.byref(params), pp.ctypes.data_as(C.POINTER(C.c_uint32)), n,
+                                     rgb.ctypes.data_as(C.c_void_p), i32p(seg), int(fp64), int(threads))
+        assert rc == 0
+        return rgb, seg
+
+    def render(self, params, pix_begin=0, pix_end=None, kahan=True, fp64=False, threads=0, pixels=True):
+        """Returns (accum (n,3), pixels (n,3) uint64 or None, segments, seconds)."""
+        npx_all = params.width * params.height
+        pix_end = npx_all if pix_end is None else pix_end
+        n = pix_end - pix_begin
+        acc = np.empty((n, 3), np.float64 if fp64 else np.float32)
+        px = (Pixel * n)() if pixels else None
+        segs = C.c_uint64()
+        secs = lib().orc_render(self.h, C.byref(params), int(pix_begin), int(pix_end), int(kahan), int(fp64),
+                                acc.ctypes.data_as(C.c_void_p), px, C.byref(segs), int(threads))
+        pxa = np.frombuffer(px, dtype=np.uint64).reshape(n, 3) if pixels else None
+        return acc, pxa, segs.value, secs
+
+    def ref_bvh(self, mesh):
+        nt = C.c_int32()
+        n = lib().orc_ref_bvh_size(self.h, int(mesh), C.byref(nt))
+        boxes = np.empty((n, 6), np.float32)
+        ab = np.empty((n, 2), np.uint32)
+        ti = np.empty(nt.value, np.int32)
+        lib().orc_ref_bvh_export(self.h, int(mesh), f32p(boxes), ab.ctypes.data_as(C.POINTER(C.c_uint32)), i32p(ti))
+        return boxes, ab, ti
+
+
+def tonemap_image(accum, spp, op):
+    a = np.ascontiguousarray(accum, dtype=np.float32).reshape(-1, 3)
+    n = a.shape[0]
+    px = (Pixel * n)()
+    lib().orc_tonemap_image(f32p(a), n, int(spp), int(op), px)
+    return np.frombuffer(px, dtype=np.uint64).reshape(n, 3)

@@ -1,0 +1,145 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle on the same inputs.
+
+Bars (from the hot-path contract):
+  closest hit    primitive ids bit-exact against the reference traversal, except distance ties
+                 within 1e-6 (relative); hit distances bit-exact where the primitive agrees
+  path radiance  same PCG stream as the oracle: within 1e-4 relative per path (most are
+                 bit-identical; the fraction is printed)
+"""
+import numpy as np
+import pytest
+
+import illume_amd.scenes as S
+from illume_amd._abi import render_params
+from oracle.pyoracle import OracleScene
+from conftest import camera_rays, compare_hits, random_rays
+
+pytestmark = pytest.mark.gpu
+
+
+def _params(st, **kw):
+    d = dict(st)
+    d.update(kw)
+    return render_params(**d)
+
+
+@pytest.fixture(scope="module")
+def cornell(ctx):
+    spec, st = S.cornell()
+    return spec, st, ctx.scene(spec), OracleScene(spec)
+
+
+@pytest.fixture(scope="module")
+def tiny(ctx):
+    spec, st = S.tiny_mesh_scene()
+    return spec, st, ctx.scene(spec), OracleScene(spec)
+
+
+@pytest.fixture(scope="module")
+def gold(ctx):
+    spec, st = S.gold_spheres()
+    return spec, st, ctx.scene(spec), OracleScene(spec)
+
+
+def _hit_parity(ctx, sc, orc, rays, label):
+    g = ctx.closest_hit(sc, rays)
+    o = orc.closest_hit(rays)
+    exact, tie, bad = compare_hits(g, o)
+    same = (g[0] == o[0]) & (g[1] == o[1]) & (g[0] != -1)
+    t_exact = np.array_equal(g[2][same], o[2][same])
+    print("%s: %d rays, %d same prim, %d ties, %d mismatches, hits %d, t bit-exact %s"
+          % (label, len(rays), exact, tie, len(bad), int((o[0] != -1).sum()), t_exact))
+    for i in bad[:5]:
+        print("  ray", i, rays[i], "gpu", g[0][i], g[1][i], g[2][i], "oracle", o[0][i], o[1][i], o[2][i])
+    return g, o, bad, t_exact
+
+
+def test_closest_hit_cornell(ctx, cornell):
+    spec, st, sc, orc = cornell
+    rays = np.concatenate([camera_rays(spec, st, 30000, seed=3),
+                           random_rays([-0.45, 0.05, -0.45], [0.45, 0.95, 0.45], 30000, seed=4)])
+    g, o, bad, t_exact = _hit_parity(ctx, sc, orc, rays, "cornell")
+    assert len(bad) == 0
+    assert t_exact
+
+
+def test_closest_hit_tiny_all_materials(ctx, tiny):
+    spec, st, sc, orc = tiny
+    rays = np.concatenate([camera_rays(spec, st, 20000, seed=5), random_rays([-3, 0.1, -1], [3, 3, 3], 20000, seed=6)])
+    g, o, bad, t_exact = _hit_parity(ctx, sc, orc, rays, "tiny")
+    assert len(bad) == 0
+    assert t_exact
+
+
+def test_closest_hit_gold_100k(ctx, gold):
+    spec, st, sc, orc = gold
+    rays = np.concatenate([camera_rays(spec, st, 20000, seed=7), random_rays([-2.5, 0.2, -1.5], [2.5, 2.2, 1.5], 20000, seed=8)])
+    g, o, bad, t_exact = _hit_parity(ctx, sc, orc, rays, "gold 100k")
+    assert len(bad) == 0
+    assert t_exact
+
+
+def test_closest_hit_quantized_layout(ctx_quant, gold):
+    spec, st, _, orc = gold
+    sc = ctx_quant.scene(spec)
+    rays = np.concatenate([camera_rays(spec, st, 20000, seed=9), random_rays([-2.5, 0.2, -1.5], [2.5, 2.2, 1.5], 20000, seed=10)])
+    g, o, bad, t_exact = _hit_parity(ctx_quant, sc, orc, rays, "gold 100k quant8")
+    assert len(bad) == 0
+    assert t_exact
+
+
+def _radiance_parity(ctx, sc, orc, p, pairs, label):
+    gr, gs = ctx.path_radiance(sc, p, pairs)
+    orr, osg = orc.path_radiance(p, pairs)
+    exact = np.all(gr == orr, axis=1)
+    denom = np.maximum(np.abs(orr), 1e-6)
+    rel = np.max(np.abs(gr - orr) / denom, axis=1)
+    ok = rel <= 1e-4
+    print("%s: %d paths, bit-identical %.5f, within 1e-4 %.5f, segments equal %.5f, mean radiance gpu %s oracle %s"
+          % (label, len(pairs), exact.mean(), ok.mean(), (gs == osg).mean(), gr.mean(0), orr.mean(0)))
+    for i in np.nonzero(~ok)[0][:5]:
+        print("  path", pairs[i], "gpu", gr[i], gs[i], "oracle", orr[i], osg[i])
+    return ok, exact
+
+
+def _pairs(W, H, n, passes, seed):
+    rng = np.random.default_rng(seed)
+    px = rng.integers(0, W * H, n)
+    return np.array([[p, s] for p in px for s in range(passes)], np.uint32)
+
+
+def test_path_radiance_cornell(ctx, cornell):
+    spec, st, sc, orc = cornell
+    p = _params(st)
+    ok, exact = _radiance_parity(ctx, sc, orc, p, _pairs(st["width"], st["height"], 4000, 4, 11), "cornell paths")
+    assert ok.mean() >= 0.999
+
+
+def test_path_radiance_tiny_all_materials(ctx, tiny):
+    spec, st, sc, orc = tiny
+    p = _params(st)
+    ok, exact = _radiance_parity(ctx, sc, orc, p, _pairs(st["width"], st["height"], 3000, 4, 12), "tiny paths")
+    assert ok.mean() >= 0.999
+
+
+def test_path_radiance_gold(ctx, gold):
+    spec, st, sc, orc = gold
+    p = _params(st)
+    ok, exact = _radiance_parity(ctx, sc, orc, p, _pairs(st["width"], st["height"], 1500, 2, 13), "gold paths")
+    assert ok.mean() >= 0.999
+
+
+def test_render_cornell_small_matches_oracle(ctx, cornell):
+    """Whole pipeline (raygen, extend, shade, compaction, Kahan accumulate, tonemap) at 64x64."""
+    spec, st, sc, orc = cornell
+    p = _params(st, width=64, height=64, spp=8)
+    acc, px, info = ctx.render(sc, p)
+    oacc, opx, osegs, _ = orc.render(p, kahan=True)
+    oacc = oacc.reshape(64, 64, 3)
+    opx = opx.reshape(64, 64, 3)
+    same = np.all(acc == oacc, axis=2)
+    rel = np.abs(acc - oacc) / np.maximum(np.abs(oacc), 1e-6)
+    print("render 64x64x8: pixels bit-identical %.4f, max rel %.3g, segments gpu %d oracle %d, tonemap equal %.4f"
+          % (same.mean(), rel.max(), info["segments"], osegs, np.all(px == opx, axis=2).mean()))
+    assert info["segments"] == osegs
+    assert np.mean(rel <= 1e-4) >= 0.999
