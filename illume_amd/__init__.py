@@ -306,7 +306,9 @@ class JsonScene:
 
     @property
     def desc(self):
-        return lib().ilm_desc_scene(self.h).contents
+        d = lib().ilm_desc_scene(self.h).contents
+        d._owner = self   # the struct points into this loader's memory: keep it alive
+        return d
 
     def settings(self):
         p = RenderParams()

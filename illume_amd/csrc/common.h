@@ -34,13 +34,15 @@ struct __attribute__((aligned(16))) BNode {
  * with a per-axis power-of-two scale; lo rounds down and hi rounds up, so every dequantised
  * box contains the FP32 box it encodes. */
 struct __attribute__((aligned(16))) QNode {
-    float ox, oy, oz;       /* origin */
-    uint8_t ex, ey, ez;     /* scale = 2^(e-127) */
-    uint8_t nchild;
-    uint8_t qlo[3][4];      /* [axis][child] */
-    uint8_t qhi[3][4];
-    int32_t child[4];       /* same encoding as BNode children; unused slots = ILM_NO_CHILD */
+    float ox, oy, oz;       /* origin                                  bytes  0..11 */
+    uint8_t ex, ey, ez;     /* scale = 2^(e-127)                       bytes 12..14 */
+    uint8_t nchild;         /*                                         byte  15     */
+    uint8_t qlo[3][4];      /* [axis][child]                           bytes 16..27 */
+    uint8_t qhi[3][4];      /*                                         bytes 28..39 */
+    uint32_t pad[2];        /*                                         bytes 40..47 */
+    int32_t child[4];       /* BNode child encoding; unused slots = ILM_NO_CHILD, bytes 48..63 */
 };
+static_assert(sizeof(QNode) == 64, "QNode is one 64-byte line");
 
 #define ILM_NO_CHILD ((int32_t)0x7fffffff)
 #define ILM_LEAF_MAX 16
@@ -86,8 +88,11 @@ struct DInst {
     float tinv[12];     /* rows 0..2 of the inverse-transpose (normals object -> world) */
     int mesh;
     int material;
-    int pad[2];
+    int flags;          /* ILM_INST_* copied from the mesh */
+    int pad;
 };
+#define ILM_INST_FACE_NORMALS 1
+#define ILM_INST_TEXCOORDS 2
 
 struct DMesh {
     const BNode* nodes;
@@ -190,7 +195,8 @@ struct DPaths {
 
 struct DHits {
     float4* h;          /* t, u, v, bits of prim */
-    int* inst;          /* instance, -2-s for sphere s, -1 miss */
+    float4* n;          /* object-space face normal cross(v1-v0, v2-v0) (triangle hits), bits of
+                           the instance (-2-s for sphere s, -1 for a miss) */
 };
 
 /* Everything a kernel launch needs. Passed by value. */
@@ -206,6 +212,8 @@ struct KParams {
     unsigned long long* trav_stats;   /* [0] child-box tests, [1] triangle tests, or null */
     int bvh_layout;
     int tri_test;
+    int extend_mode;
+    int use_ovf;            /* 1 when a scene's stack bound exceeds the LDS stack depth */
 };
 
 /* ---------------- vector helpers (reference operation order) ---------------- */

@@ -217,6 +217,8 @@ int ilm_scene_upload(ilm_ctx* ctx, const ilm_scene_desc* d, ilm_scene** out) {
         m4_rows(tinv, di.tinv);
         di.mesh = d->instances[i].mesh;
         di.material = d->instances[i].material;
+        di.flags = (d->meshes[di.mesh].face_normals ? ILM_INST_FACE_NORMALS : 0) |
+                   (d->meshes[di.mesh].has_texcoords ? ILM_INST_TEXCOORDS : 0);
         for (int k = 0; k < 16; k++) {
             s->inst_mats[48 * i + k] = mat.m[k / 4][k % 4];
             s->inst_mats[48 * i + 16 + k] = inv.m[k / 4][k % 4];
@@ -270,7 +272,7 @@ int ilm_scene_upload(ilm_ctx* ctx, const ilm_scene_desc* d, ilm_scene** out) {
     const int cap = ilm_k_lds_stack() + ilm_k_ovf_stack();
     s->stack_bound_fp32 = tlas_depth + 1 + blas_depth;
     s->stack_bound_q = 3 * (tlas_qdepth + 1) + 1 + 3 * (blas_qdepth + 1);
-    if (s->stack_bound_fp32 > cap)
+    if (s->stack_bound_fp32 > cap || (ctx->opts.bvh_layout == ILM_BVH_QUANT8 && s->stack_bound_q > cap))
         return fail(ctx, ILM_E_UNSUPPORTED, "BVH too deep for the traversal stack");
 
     /* ---- materials, mediums, spheres, environment, camera ---- */
@@ -384,7 +386,7 @@ static int ctx_alloc_paths(ilm_ctx* ctx, int64_t cap) {
         HIPTRY(ctx, A((void**)&P->rng, cap * sizeof(uint2)));
     }
     HIPTRY(ctx, A((void**)&ctx->hits.h, cap * sizeof(float4)));
-    HIPTRY(ctx, A((void**)&ctx->hits.inst, cap * sizeof(int)));
+    HIPTRY(ctx, A((void**)&ctx->hits.n, cap * sizeof(float4)));
     HIPTRY(ctx, A((void**)&ctx->contrib, cap * sizeof(float4)));
     ctx->cap = cap;
     return ILM_OK;
@@ -409,11 +411,13 @@ int ilm_open(int device, const ilm_opts* opts, ilm_ctx** out) {
     HIPTRY(ctx, hipGetDeviceProperties(&prop, device));
     ctx->n_cu = prop.multiProcessorCount;
     HIPTRY(ctx, hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
-    int occ = ilm_k_extend_occupancy(ctx->opts.tri_test, ctx->opts.bvh_layout);
+    int occ = ilm_k_extend_occupancy(ctx->opts.tri_test, ctx->opts.bvh_layout, ctx->opts.extend_mode);
     ctx->ext_grid = ctx->n_cu * std::max(1, occ);
     ctx->gen_grid = ctx->n_cu * 8;
     ctx->shade_grid = ctx->n_cu * 8;
-    int64_t cap = ctx->opts.paths_in_flight > 0 ? ctx->opts.paths_in_flight : (int64_t)1 << 22;
+    /* 2^26 paths (~9.5 GB of path state) hold a whole 1024x1024x64 render in one chunk: every
+     * depth is one launch over all live paths, so launch tails are amortised */
+    int64_t cap = ctx->opts.paths_in_flight > 0 ? ctx->opts.paths_in_flight : (int64_t)1 << 26;
     cap = std::max<int64_t>(cap, 4096);
     cap = std::min<int64_t>(cap, (int64_t)1 << 30);
     int rc = ctx_alloc_paths(ctx, cap);
@@ -471,6 +475,17 @@ const char* ilm_last_error(ilm_ctx* ctx) {
 
 /* ======================= chunk pipeline ======================= */
 
+/* whether a scene needs the spilling traversal stack under this context's layout */
+static bool scene_needs_ovf(const ilm_ctx* ctx, const ilm_scene* s) {
+    const int bound = ctx->opts.bvh_layout == ILM_BVH_QUANT8 ? s->stack_bound_q : s->stack_bound_fp32;
+    return bound > ilm_k_lds_stack();
+}
+static bool jobs_need_ovf(const ilm_ctx* ctx, const std::vector<Job*>& jobs) {
+    for (const Job* j : jobs)
+        if (scene_needs_ovf(ctx, j->scene)) return true;
+    return false;
+}
+
 static KParams base_params(ilm_ctx* ctx) {
     KParams P;
     std::memset(&P, 0, sizeof(P));
@@ -482,6 +497,7 @@ static KParams base_params(ilm_ctx* ctx) {
     P.contrib = ctx->contrib;
     P.bvh_layout = ctx->opts.bvh_layout;
     P.tri_test = ctx->opts.tri_test;
+    P.extend_mode = ctx->opts.extend_mode;
     P.trav_stats = ctx->opts.count_tests ? ctx->d_trav : nullptr;
     return P;
 }
@@ -524,14 +540,16 @@ static int enqueue_pipeline(ilm_ctx* ctx, const KParams& base, int depth_steps, 
     return ILM_OK;
 }
 
-static int launch_chunk(ilm_ctx* ctx, const DChunk& hc, ProfileSink* prof) {
+static int launch_chunk(ilm_ctx* ctx, const DChunk& hc, bool ovf, ProfileSink* prof) {
     int rc = stage_chunk(ctx, hc);
     if (rc) return rc;
     KParams P = base_params(ctx);
+    P.use_ovf = ovf ? 1 : 0;
     const int D = hc.depth_steps;
     const bool graphs = ctx->opts.use_graphs && !prof && !ctx->opts.count_tests;
     if (graphs) {
-        auto it = ctx->graphs.find(D);
+        const int key = 2 * D + (ovf ? 1 : 0);
+        auto it = ctx->graphs.find(key);
         if (it == ctx->graphs.end()) {
             hipGraph_t g = nullptr;
             hipGraphExec_t ge = nullptr;
@@ -542,7 +560,7 @@ static int launch_chunk(ilm_ctx* ctx, const DChunk& hc, ProfileSink* prof) {
             if (ec != hipSuccess) return fail(ctx, ILM_E_DEVICE, std::string("graph capture: ") + hipGetErrorString(ec));
             HIPTRY(ctx, hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
             hipGraphDestroy(g);
-            it = ctx->graphs.emplace(D, ge).first;
+            it = ctx->graphs.emplace(key, ge).first;
         }
         HIPTRY(ctx, hipGraphLaunch(it->second, ctx->stream));
         return ILM_OK;
@@ -672,7 +690,7 @@ static int run_jobs_sync(ilm_ctx* ctx, std::vector<Job*>& jobs, ilm_render_info*
         if (in_chunk.empty()) break;
         /* pass offsets: first_sample shifts every pass of the request */
         for (int w = 0; w < hc->n_work; w++) hc->work[w].pass0 += in_chunk[hc->work[w].req]->p.first_sample;
-        int rc = launch_chunk(ctx, *hc, ctx->opts.profile ? &prof : nullptr);
+        int rc = launch_chunk(ctx, *hc, jobs_need_ovf(ctx, in_chunk), ctx->opts.profile ? &prof : nullptr);
         if (rc) return rc;
         chunks++;
         max_paths = std::max<int64_t>(max_paths, hc->total_paths);
@@ -792,7 +810,7 @@ int ilm_debug_closest_hit(ilm_ctx* ctx, ilm_scene* scene, const float* rays, int
     std::unique_ptr<DChunk> hc(new DChunk());
     std::memset(hc.get(), 0, sizeof(DChunk));
     hc->reqs[0].scene = scene->d_scene;
-    std::vector<float4> o4, d4, h4;
+    std::vector<float4> o4, d4, h4, n4;
     std::vector<int> hi;
     for (int64_t base = 0; base < n; base += ctx->cap) {
         int64_t m = std::min<int64_t>(ctx->cap, n - base);
@@ -811,16 +829,20 @@ int ilm_debug_closest_hit(ilm_ctx* ctx, ilm_scene* scene, const float* rays, int
         HIPTRY(ctx, hipMemcpyAsync(ctx->A.d, d4.data(), m * sizeof(float4), hipMemcpyHostToDevice, ctx->stream));
         KParams P = base_params(ctx);
         P.trav_stats = nullptr;
+        P.use_ovf = scene_needs_ovf(ctx, scene) ? 1 : 0;
         HIPTRY(ctx, ilm_k_ctl_init(P, ctx->stream));
         HIPTRY(ctx, ilm_k_extend(P, 0, ctx->ovf, ctx->ext_grid, ctx->stream));
         h4.resize(m);
+        n4.resize(m);
         hi.resize(m);
         HIPTRY(ctx, hipMemcpyAsync(h4.data(), ctx->hits.h, m * sizeof(float4), hipMemcpyDeviceToHost, ctx->stream));
-        HIPTRY(ctx, hipMemcpyAsync(hi.data(), ctx->hits.inst, m * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        HIPTRY(ctx, hipMemcpyAsync(n4.data(), ctx->hits.n, m * sizeof(float4), hipMemcpyDeviceToHost, ctx->stream));
         HIPTRY(ctx, hipStreamSynchronize(ctx->stream));
         for (int64_t i = 0; i < m; i++) {
-            int32_t pr;
+            int32_t pr, in;
             std::memcpy(&pr, &h4[i].w, 4);
+            std::memcpy(&in, &n4[i].w, 4);
+            hi[i] = in;
             inst[base + i] = hi[i];
             prim[base + i] = hi[i] >= 0 ? pr : -1;
             t[base + i] = hi[i] == -1 ? 3.402823466e+38f : h4[i].x;
@@ -864,6 +886,7 @@ int ilm_debug_path_radiance(ilm_ctx* ctx, ilm_scene* scene, const ilm_render_par
         }
         KParams P = base_params(ctx);
         P.trav_stats = nullptr;
+        P.use_ovf = scene_needs_ovf(ctx, scene) ? 1 : 0;
         P.list = d_list;
         P.seg_count = ctx->seg_count;
         if (ilm_k_ctl_init(P, ctx->stream) != hipSuccess || ilm_k_raygen(P, ctx->gen_grid, ctx->stream) != hipSuccess) {
@@ -930,7 +953,7 @@ static void server_loop(ilm_ctx* ctx) {
             if (!issuable.empty() && inflight.size() < 2) {
                 build_chunk(ctx, issuable, *hc, in_chunk);
                 for (int w = 0; w < hc->n_work; w++) hc->work[w].pass0 += in_chunk[hc->work[w].req]->p.first_sample;
-                int rc = launch_chunk(ctx, *hc, nullptr);
+                int rc = launch_chunk(ctx, *hc, jobs_need_ovf(ctx, in_chunk), nullptr);
                 Inflight f;
                 hipEventCreateWithFlags(&f.ev, hipEventDisableTiming);
                 for (Job* j : in_chunk) {

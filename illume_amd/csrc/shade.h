@@ -9,6 +9,19 @@
 #pragma once
 #include "common.h"
 
+/* Explicit address spaces: pointers read out of scene records are generic to the compiler,
+ * which would turn every access into a flat_* instruction; these casts make them global (1)
+ * or LDS (3) accesses. */
+#if defined(__HIP_DEVICE_COMPILE__)
+#define GLOBAL __attribute__((address_space(1)))
+#define LDSAS __attribute__((address_space(3)))
+#else   /* host pass: these functions are never called there */
+#define GLOBAL
+#define LDSAS
+#endif
+template <class T> __device__ inline const GLOBAL T* gp(const T* p) { return (const GLOBAL T*)p; }
+template <class T> __device__ inline GLOBAL T* gpw(T* p) { return (GLOBAL T*)p; }
+
 #define ILM_PI_D 3.14159265358979323846
 #define ILM_INV_PI_D 0.318309886183790671538
 
@@ -124,14 +137,16 @@ __device__ inline V3 sample_hg(float g, float u1, float u2) {
 
 /* src/texture.cu:129-162. Bitmap filtering: wrap, normalised coordinates, texel centres at
  * (i + 0.5) / w, bilinear for HDR and point for LDR, computed in software. */
-__device__ inline V3 tex_eval(const DTex& t, float u, float v) {
-    if (t.type == ILM_TEX_CONSTANT) return mk3(t.a[0], t.a[1], t.a[2]);
-    if (t.type == ILM_TEX_CHECKERBOARD) {
-        int x = (int)((float)(int)t.scale[0] * (u + 0.25f));
-        int y = (int)((float)(int)t.scale[1] * v);
+__device__ inline V3 tex_eval(const GLOBAL DTex* tp, float u, float v) {
+    const int type = tp->type;
+    if (type == ILM_TEX_CONSTANT) return mk3(tp->a[0], tp->a[1], tp->a[2]);
+    if (type == ILM_TEX_CHECKERBOARD) {
+        int x = (int)((float)(int)tp->scale[0] * (u + 0.25f));
+        int y = (int)((float)(int)tp->scale[1] * v);
         bool on = (x + y) % 2 == 0;
-        return on ? mk3(t.a[0], t.a[1], t.a[2]) : mk3(t.b[0], t.b[1], t.b[2]);
+        return on ? mk3(tp->a[0], tp->a[1], tp->a[2]) : mk3(tp->b[0], tp->b[1], tp->b[2]);
     }
+    const DTex t = *tp;
     u = u * t.scale[0];
     v = v * t.scale[1];
     float uw = u - __builtin_floorf(u), vw = v - __builtin_floorf(v);
@@ -143,10 +158,11 @@ __device__ inline V3 tex_eval(const DTex& t, float u, float v) {
         x0 = ((x0 % t.w) + t.w) % t.w;
         y0 = ((y0 % t.h) + t.h) % t.h;
         int x1 = (x0 + 1) % t.w, y1 = (y0 + 1) % t.h;
-        float4 p00 = t.f[(size_t)y0 * t.w + x0];
-        float4 p10 = t.f[(size_t)y0 * t.w + x1];
-        float4 p01 = t.f[(size_t)y1 * t.w + x0];
-        float4 p11 = t.f[(size_t)y1 * t.w + x1];
+        const GLOBAL float4* tf = gp(t.f);
+        float4 p00 = tf[(size_t)y0 * t.w + x0];
+        float4 p10 = tf[(size_t)y0 * t.w + x1];
+        float4 p01 = tf[(size_t)y1 * t.w + x0];
+        float4 p11 = tf[(size_t)y1 * t.w + x1];
         float wx = 1.0f - ax, wy = 1.0f - ay;
         float r0 = (p00.x * wx + p10.x * ax) * wy + (p01.x * wx + p11.x * ax) * ay;
         float r1 = (p00.y * wx + p10.y * ax) * wy + (p01.y * wx + p11.y * ax) * ay;
@@ -156,7 +172,7 @@ __device__ inline V3 tex_eval(const DTex& t, float u, float v) {
     int x = (int)(uw * (float)t.w), y = (int)(vw * (float)t.h);
     if (x >= t.w) x = t.w - 1;
     if (y >= t.h) y = t.h - 1;
-    uchar4 p = t.u8[(size_t)y * t.w + x];
+    uchar4 p = gp(t.u8)[(size_t)y * t.w + x];
     return mk3((float)p.x / 255.0f, (float)p.y / 255.0f, (float)p.z / 255.0f);
 }
 

@@ -1,0 +1,481 @@
+/*
+ * traverse.h - closest-hit traversal and the extend kernels.
+ *
+ * Two-level walk: spheres in index order (src/renderer.cu:117-125), then a TLAS over instance
+ * world boxes whose leaves enter a mesh BLAS in object space (ray transformed by the
+ * instance's inverse, direction not renormalised, src/primitives/mesh_instance.cu:96-101).
+ * Triangles use Moller-Trumbore exactly as src/primitives/mesh.cu:446-476 (or the watertight
+ * test). Unlike the reference's exhaustive walk (src/primitives/mesh.cu:492-530) the walk is
+ * ordered (near child first) and t-pruned; box tests are conservative by a relative margin, so
+ * the closest hit equals the reference's except for distance ties.
+ *
+ * The stack lives in LDS, 64 lanes interleaved (entry s of lane l at s*64 + l, one bank per
+ * lane); scenes whose proven stack bound exceeds the LDS depth use the spilling variant
+ * (OVF = 1) whose extra entries go to a per-lane global area.
+ */
+#pragma once
+#include "shade.h"
+
+#define EXT_BLOCK 64            /* one wave per workgroup: the LDS stack belongs to one wave */
+#ifndef LDS_STACK
+#define LDS_STACK 32            /* stack entries per lane kept in LDS */
+#endif
+#define OVF_STACK (128 - LDS_STACK)  /* spill entries per lane (spilling variant) */
+#define STACK_SENTINEL ((int)0x7ffffffe)
+
+/* conservative slab-test margins (relative); cover rcp and fma rounding of the box test */
+#define BOX_LO 0.99999f
+#define BOX_HI 1.00001f
+
+__device__ inline uint32_t f2u(float f) { return __float_as_uint(f); }
+__device__ inline float u2f(uint32_t u) { return __uint_as_float(u); }
+
+struct Best {
+    float t, u, v;
+    int inst, prim;
+    V3 n;               /* cross(v1 - v0, v2 - v0) of the closest triangle */
+};
+
+__device__ inline void best_init(Best& B) {
+    B.t = ILM_FLT_MAX; B.u = 0.0f; B.v = 0.0f; B.inst = -1; B.prim = -1; B.n = mk3(0.0f, 0.0f, 0.0f);
+}
+/* record a closer triangle hit; the face normal is formed exactly as
+ * src/primitives/mesh.cu:541 does, from the same vertex differences */
+__device__ inline void best_tri(Best& B, float t, float u, float v, int inst, float4 A, float4 Bq, float4 C) {
+    B.t = t; B.u = u; B.v = v; B.inst = inst; B.prim = __float_as_int(A.w);
+    V3 v0 = mk3(A.x, A.y, A.z);
+    B.n = cross3(sub3(mk3(Bq.x, Bq.y, Bq.z), v0), sub3(mk3(C.x, C.y, C.z), v0));
+}
+__device__ inline void write_hit(const KParams& P, uint32_t i, const Best& B) {
+    gpw(P.hits.h)[i] = make_float4(B.t, B.u, B.v, __int_as_float(B.prim));
+    gpw(P.hits.n)[i] = make_float4(B.n.x, B.n.y, B.n.z, __int_as_float(B.inst));
+}
+
+__device__ inline V3 rcp3(V3 d) {
+    return mk3(__builtin_amdgcn_rcpf(d.x), __builtin_amdgcn_rcpf(d.y), __builtin_amdgcn_rcpf(d.z));
+}
+__device__ inline V3 neg_o_over_d(V3 o, V3 id) { return mk3(-o.x * id.x, -o.y * id.y, -o.z * id.z); }
+
+/* Two child boxes of an FP32 BVH2 node against the ray (o, 1/d); noid = -o/d. */
+__device__ inline void box2(const float4 a, const float4 b, const float4 c, V3 id, V3 noid, float tmax,
+                            float& t0, float& t1, bool& h0, bool& h1) {
+    float x0 = __builtin_fmaf(a.x, id.x, noid.x), x1 = __builtin_fmaf(a.y, id.x, noid.x);
+    float y0 = __builtin_fmaf(a.z, id.y, noid.y), y1 = __builtin_fmaf(a.w, id.y, noid.y);
+    float z0 = __builtin_fmaf(b.x, id.z, noid.z), z1 = __builtin_fmaf(b.y, id.z, noid.z);
+    float tn = __builtin_fmaxf(__builtin_fmaxf(__builtin_fminf(x0, x1), __builtin_fminf(y0, y1)),
+                               __builtin_fmaxf(__builtin_fminf(z0, z1), 0.0f));
+    float tf = __builtin_fminf(__builtin_fminf(__builtin_fmaxf(x0, x1), __builtin_fmaxf(y0, y1)),
+                               __builtin_fmaxf(z0, z1));
+    t0 = tn * BOX_LO;
+    h0 = (t0 <= tf * BOX_HI) && (t0 <= tmax);
+
+    float X0 = __builtin_fmaf(b.z, id.x, noid.x), X1 = __builtin_fmaf(b.w, id.x, noid.x);
+    float Y0 = __builtin_fmaf(c.x, id.y, noid.y), Y1 = __builtin_fmaf(c.y, id.y, noid.y);
+    float Z0 = __builtin_fmaf(c.z, id.z, noid.z), Z1 = __builtin_fmaf(c.w, id.z, noid.z);
+    float Tn = __builtin_fmaxf(__builtin_fmaxf(__builtin_fminf(X0, X1), __builtin_fminf(Y0, Y1)),
+                               __builtin_fmaxf(__builtin_fminf(Z0, Z1), 0.0f));
+    float Tf = __builtin_fminf(__builtin_fminf(__builtin_fmaxf(X0, X1), __builtin_fmaxf(Y0, Y1)),
+                               __builtin_fmaxf(Z0, Z1));
+    t1 = Tn * BOX_LO;
+    h1 = (t1 <= Tf * BOX_HI) && (t1 <= tmax);
+}
+
+/* Up to four dequantised child boxes of a QNode (dequantisation as bvh.cpp's qnode_child_box). */
+__device__ inline void box4q(const GLOBAL QNode* np, V3 id, V3 noid, float tmax, float* tn_out, uint32_t& hitmask,
+                             int& nchild, int4& ch) {
+    const float4 w0 = ((const GLOBAL float4*)np)[0];   /* ox oy oz | ex ey ez nchild */
+    const float4 w1 = ((const GLOBAL float4*)np)[1];   /* qlo[3][4] qhi[0][4] ... */
+    const float4 w2 = ((const GLOBAL float4*)np)[2];
+    ch = ((const GLOBAL int4*)np)[3];
+    const uint32_t e = f2u(w0.w);
+    const float sx = u2f((e & 0xffu) << 23), sy = u2f(((e >> 8) & 0xffu) << 23), sz = u2f(((e >> 16) & 0xffu) << 23);
+    nchild = (int)(e >> 24);
+    const uint32_t lx = f2u(w1.x), ly = f2u(w1.y), lz = f2u(w1.z);
+    const uint32_t hx = f2u(w1.w), hy = f2u(w2.x), hz = f2u(w2.y);
+    hitmask = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const int sh = 8 * k;
+        float bx0 = w0.x + (float)((lx >> sh) & 0xffu) * sx, bx1 = w0.x + (float)((hx >> sh) & 0xffu) * sx;
+        float by0 = w0.y + (float)((ly >> sh) & 0xffu) * sy, by1 = w0.y + (float)((hy >> sh) & 0xffu) * sy;
+        float bz0 = w0.z + (float)((lz >> sh) & 0xffu) * sz, bz1 = w0.z + (float)((hz >> sh) & 0xffu) * sz;
+        float x0 = __builtin_fmaf(bx0, id.x, noid.x), x1 = __builtin_fmaf(bx1, id.x, noid.x);
+        float y0 = __builtin_fmaf(by0, id.y, noid.y), y1 = __builtin_fmaf(by1, id.y, noid.y);
+        float z0 = __builtin_fmaf(bz0, id.z, noid.z), z1 = __builtin_fmaf(bz1, id.z, noid.z);
+        float tn = __builtin_fmaxf(__builtin_fmaxf(__builtin_fminf(x0, x1), __builtin_fminf(y0, y1)),
+                                   __builtin_fmaxf(__builtin_fminf(z0, z1), 0.0f));
+        float tf = __builtin_fminf(__builtin_fminf(__builtin_fmaxf(x0, x1), __builtin_fmaxf(y0, y1)),
+                                   __builtin_fmaxf(z0, z1));
+        tn = tn * BOX_LO;
+        tn_out[k] = tn;
+        if (k < nchild && tn <= tf * BOX_HI && tn <= tmax) hitmask |= 1u << k;
+    }
+}
+
+template <int OVF>
+struct Stack {
+    LDSAS int* lds;     /* &stk[lane]; entry s at lds[s * EXT_BLOCK] */
+    GLOBAL int* ovf;    /* this lane's spill area (OVF = 1) */
+    int sp;
+    __device__ inline void push(int v) {
+        if (!OVF) {
+            if (sp < LDS_STACK) { lds[sp * EXT_BLOCK] = v; sp++; }
+        } else {
+            if (sp < LDS_STACK) lds[sp * EXT_BLOCK] = v;
+            else if (sp < LDS_STACK + OVF_STACK) ovf[sp - LDS_STACK] = v;
+            sp = sp < LDS_STACK + OVF_STACK ? sp + 1 : sp;
+        }
+    }
+    __device__ inline int pop() {
+        --sp;
+        if (!OVF || sp < LDS_STACK) return lds[sp * EXT_BLOCK];
+        return ovf[sp - LDS_STACK];
+    }
+};
+
+struct TravCount {
+    uint32_t boxes, tris;
+};
+
+__device__ inline void load_rows12(const GLOBAL float* src, float* dst) {
+    const GLOBAL float4* s4 = (const GLOBAL float4*)src;
+    float4 a = s4[0], b = s4[1], c = s4[2];
+    dst[0] = a.x; dst[1] = a.y; dst[2] = a.z; dst[3] = a.w;
+    dst[4] = b.x; dst[5] = b.y; dst[6] = b.z; dst[7] = b.w;
+    dst[8] = c.x; dst[9] = c.y; dst[10] = c.z; dst[11] = c.w;
+}
+
+__device__ inline void spheres_first(const GLOBAL DScene* S, V3 O, V3 D, Best& B) {
+    const int ns = S->n_spheres;
+    const GLOBAL DSphere* sph = gp(S->spheres);
+    for (int s = 0; s < ns; s++) {
+        DSphere sp = sph[s];
+        float t;
+        if (sphere_hit(sp, O, D, t) && t < B.t) { B.t = t; B.inst = -2 - s; }
+    }
+}
+
+/* Intersect the triangles of BLAS leaf `node`. */
+template <int TRI, int COUNT>
+__device__ inline void leaf_tris(const GLOBAL float4* tris, int node, V3 o, V3 d, const WTRay& wt, int cur, Best& B,
+                                 TravCount* tc) {
+    const uint32_t f = ilm_leaf_first(node), cnt = ilm_leaf_count(node);
+    if (COUNT) tc->tris += cnt;
+    for (uint32_t q = 0; q < cnt; q++) {
+        const GLOBAL float4* tp = tris + 3 * (f + q);
+        float4 A = tp[0], Bq = tp[1], C = tp[2];
+        float u = 0.0f, v = 0.0f, t;
+        if (TRI) t = tri_wt(wt, mk3(A.x, A.y, A.z), mk3(Bq.x, Bq.y, Bq.z), mk3(C.x, C.y, C.z), o, u, v);
+        else t = tri_mt(mk3(A.x, A.y, A.z), mk3(Bq.x, Bq.y, Bq.z), mk3(C.x, C.y, C.z), o, d, u, v);
+        if (t > 0.0f && t < B.t) best_tri(B, t, u, v, cur, A, Bq, C);
+    }
+}
+
+/* Push the hit children of an inner node nearest-last and continue with the nearest; returns
+ * false when no child is hit. */
+template <int QUANT, int OVF, int COUNT>
+__device__ inline bool inner_step(const GLOBAL BNode* nodes, const GLOBAL QNode* qnodes, int& node, V3 id, V3 noid,
+                                  float tmax, Stack<OVF>& st, TravCount* tc) {
+    if (QUANT) {
+        float tn[4];
+        uint32_t hm;
+        int nc;
+        int4 ch;
+        box4q(qnodes + node, id, noid, tmax, tn, hm, nc, ch);
+        if (COUNT) tc->boxes += nc;
+        if (hm == 0) return false;
+        /* visit nearest first: push the others far-to-near */
+        int best = -1;
+        float bt = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            if ((hm >> k) & 1u) { if (best < 0 || tn[k] < bt) { best = k; bt = tn[k]; } }
+        uint32_t rest = hm & ~(1u << best);
+        while (rest) {
+            int far = -1;
+            float ft = 0.0f;
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                if ((rest >> k) & 1u) { if (far < 0 || tn[k] > ft) { far = k; ft = tn[k]; } }
+            st.push(far == 0 ? ch.x : far == 1 ? ch.y : far == 2 ? ch.z : ch.w);
+            rest &= ~(1u << far);
+        }
+        node = best == 0 ? ch.x : best == 1 ? ch.y : best == 2 ? ch.z : ch.w;
+        return true;
+    } else {
+        const GLOBAL BNode* n = nodes + node;
+        float4 a = n->a, b = n->b, c = n->c;
+        int4 ch = n->d;
+        float t0, t1;
+        bool h0, h1;
+        box2(a, b, c, id, noid, tmax, t0, t1, h0, h1);
+        if (COUNT) tc->boxes += 2;
+        if (h0 && h1) {
+            int nearc = ch.x, farc = ch.y;
+            if (t1 < t0) { nearc = ch.y; farc = ch.x; }
+            st.push(farc);
+            node = nearc;
+            return true;
+        }
+        if (h0) { node = ch.x; return true; }
+        if (h1) { node = ch.y; return true; }
+        return false;
+    }
+}
+
+template <int TRI, int QUANT, int COUNT, int OVF>
+__device__ inline void trace_ray(const GLOBAL DScene* S, V3 O, V3 D, Best& B, Stack<OVF>& st, TravCount* tc) {
+    best_init(B);
+    spheres_first(S, O, D, B);
+    int node = QUANT ? S->qtlas_root : S->tlas_root;
+    if (node == ILM_NO_CHILD) return;
+    const GLOBAL BNode* nodes = gp(S->tlas);
+    const GLOBAL QNode* qnodes = gp(S->qtlas);
+    const GLOBAL float4* tris = nullptr;
+    bool blas = false;
+    int cur = -1;
+    V3 o = O, d = D;
+    V3 id = rcp3(d);
+    V3 noid = neg_o_over_d(o, id);
+    WTRay wt;
+    st.sp = 0;
+    for (;;) {
+        if (node >= 0) {
+            if (inner_step<QUANT, OVF, COUNT>(nodes, qnodes, node, id, noid, B.t, st, tc)) continue;
+        } else if (!blas) {
+            /* TLAS leaf: one instance; enter its BLAS in object space */
+            const int k = gp(S->tlas_prims)[ilm_leaf_first(node)];
+            const GLOBAL DInst* I = gp(S->inst) + k;
+            const GLOBAL DMesh* M = gp(S->meshes) + I->mesh;
+            const int broot = QUANT ? M->qroot : M->root;
+            if (broot != ILM_NO_CHILD) {
+                float inv[12];
+                load_rows12(I->inv, inv);
+                o = xform3(inv, O, 1.0f);
+                d = xform3(inv, D, 0.0f);
+                id = rcp3(d);
+                noid = neg_o_over_d(o, id);
+                if (TRI) wt = wt_setup(d);
+                nodes = gp(M->nodes);
+                qnodes = gp(M->qnodes);
+                tris = gp(M->tris);
+                st.push(STACK_SENTINEL);
+                blas = true;
+                cur = k;
+                node = broot;
+                continue;
+            }
+        } else {
+            leaf_tris<TRI, COUNT>(tris, node, o, d, wt, cur, B, tc);
+        }
+        /* pop */
+        for (;;) {
+            if (st.sp == 0) return;
+            node = st.pop();
+            if (node != STACK_SENTINEL) break;
+            blas = false;
+            nodes = gp(S->tlas);
+            qnodes = gp(S->qtlas);
+            o = O;
+            d = D;
+            id = rcp3(d);
+            noid = neg_o_over_d(o, id);
+        }
+    }
+}
+
+/* One ray batch per wave: a wave takes 64 consecutive queue entries (spatially coherent at
+ * depth 0) with one atomic and walks them to completion. */
+template <int TRI, int QUANT, int COUNT, int OVF>
+__global__ __launch_bounds__(EXT_BLOCK) void k_extend(KParams P, int depth, int* ovf_base) {
+    __shared__ int stk[LDS_STACK * EXT_BLOCK];
+    const int lane = threadIdx.x;
+    Stack<OVF> st;
+    st.lds = (LDSAS int*)stk + lane;
+    st.ovf = gpw(ovf_base) + ((size_t)blockIdx.x * EXT_BLOCK + lane) * OVF_STACK;
+    st.sp = 0;
+    GLOBAL DCtl* ctl = gpw(P.ctl);
+    const uint32_t count = ctl->count[depth];
+    const GLOBAL float4* ino = gp(P.in.o);
+    const GLOBAL float4* ind = gp(P.in.d);
+    const GLOBAL DReq* reqs = gp(P.chunk)->reqs;
+    TravCount tc = {0u, 0u};
+    for (;;) {
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(&ctl->fetch[depth], (uint32_t)EXT_BLOCK);
+        base = __shfl(base, 0);
+        if (base >= count) break;
+        const uint32_t i = base + lane;
+        if (i < count) {
+            float4 o4 = ino[i];
+            float4 d4 = ind[i];
+            const GLOBAL DScene* S = gp(reqs[f2u(o4.w) >> 16].scene);
+            Best B;
+            trace_ray<TRI, QUANT, COUNT, OVF>(S, mk3(o4.x, o4.y, o4.z), mk3(d4.x, d4.y, d4.z), B, st, &tc);
+            write_hit(P, i, B);
+        }
+    }
+    if (COUNT) {
+        atomicAdd(&P.trav_stats[0], (unsigned long long)tc.boxes);
+        atomicAdd(&P.trav_stats[1], (unsigned long long)tc.tris);
+    }
+}
+
+/* ---------------- extend with lane refill ----------------
+ * Each lane keeps its ray's traversal state across an outer loop; after every BLAS leaf (and
+ * when its ray finishes) the wave counts its idle lanes and, once REFILL_MIN of them are idle,
+ * hands them the next rays of a wave-local pool taken from the queue 64 at a time (persistent
+ * traversal with dynamic fetch, Aila and Laine, HPG 2009). Trades ray coherence for lane
+ * occupancy; selected with ilm_opts.extend_mode = 1. */
+#ifndef REFILL_MIN
+#define REFILL_MIN 16
+#endif
+#define POP_MARK ((int)0x7ffffffd)
+#define NO_RAY 0xffffffffu
+
+struct TState {
+    V3 o, d, id, noid;      /* ray in the current (world or object) space */
+    const GLOBAL BNode* nodes;
+    const GLOBAL QNode* qnodes;
+    const GLOBAL float4* tris;
+    const GLOBAL DScene* S;
+    int node;
+    int cur;
+    int blas;
+    WTRay wt;
+};
+
+template <int QUANT, int OVF>
+__device__ inline void seg_init(TState& T, Best& B, Stack<OVF>& st, const GLOBAL DScene* S, V3 O, V3 D) {
+    best_init(B);
+    spheres_first(S, O, D, B);
+    T.S = S;
+    T.node = QUANT ? S->qtlas_root : S->tlas_root;
+    if (T.node == ILM_NO_CHILD) T.node = POP_MARK;   /* no instances: nothing to walk */
+    T.nodes = gp(S->tlas);
+    T.qnodes = gp(S->qtlas);
+    T.tris = nullptr;
+    T.blas = 0;
+    T.cur = -1;
+    T.o = O;
+    T.d = D;
+    T.id = rcp3(D);
+    T.noid = neg_o_over_d(O, T.id);
+    st.sp = 0;
+}
+
+/* Walk until one BLAS leaf has been intersected (returns false) or the ray is done (true). */
+template <int TRI, int QUANT, int COUNT, int OVF>
+__device__ inline bool seg_run(TState& T, Best& B, Stack<OVF>& st, const GLOBAL float4* wo, const GLOBAL float4* wd,
+                               uint32_t ridx, TravCount* tc) {
+    int node = T.node;
+    for (;;) {
+        if (node == POP_MARK) goto pop;
+        if (node >= 0) {
+            if (inner_step<QUANT, OVF, COUNT>(T.nodes, T.qnodes, node, T.id, T.noid, B.t, st, tc)) continue;
+            goto pop;
+        } else if (!T.blas) {
+            const int k = gp(T.S->tlas_prims)[ilm_leaf_first(node)];
+            const GLOBAL DInst* I = gp(T.S->inst) + k;
+            const GLOBAL DMesh* M = gp(T.S->meshes) + I->mesh;
+            const int broot = QUANT ? M->qroot : M->root;
+            if (broot == ILM_NO_CHILD) goto pop;
+            float inv[12];
+            load_rows12(I->inv, inv);
+            V3 O = T.o, D = T.d;
+            T.o = xform3(inv, O, 1.0f);
+            T.d = xform3(inv, D, 0.0f);
+            T.id = rcp3(T.d);
+            T.noid = neg_o_over_d(T.o, T.id);
+            if (TRI) T.wt = wt_setup(T.d);
+            T.nodes = gp(M->nodes);
+            T.qnodes = gp(M->qnodes);
+            T.tris = gp(M->tris);
+            st.push(STACK_SENTINEL);
+            T.blas = 1;
+            T.cur = k;
+            node = broot;
+            continue;
+        } else {
+            leaf_tris<TRI, COUNT>(T.tris, node, T.o, T.d, T.wt, T.cur, B, tc);
+            T.node = POP_MARK;          /* resume with a pop after the wave had a chance to refill */
+            return st.sp == 0;
+        }
+    pop:
+        for (;;) {
+            if (st.sp == 0) return true;
+            node = st.pop();
+            if (node != STACK_SENTINEL) break;
+            /* leaving the instance: the world ray comes back from the path state */
+            float4 a = wo[ridx], b = wd[ridx];
+            T.o = mk3(a.x, a.y, a.z);
+            T.d = mk3(b.x, b.y, b.z);
+            T.id = rcp3(T.d);
+            T.noid = neg_o_over_d(T.o, T.id);
+            T.blas = 0;
+            T.nodes = gp(T.S->tlas);
+            T.qnodes = gp(T.S->qtlas);
+        }
+    }
+}
+
+template <int TRI, int QUANT, int COUNT, int OVF>
+__global__ __launch_bounds__(EXT_BLOCK) void k_extend_refill(KParams P, int depth, int* ovf_base) {
+    __shared__ int stk[LDS_STACK * EXT_BLOCK];
+    const int lane = threadIdx.x;
+    Stack<OVF> st;
+    st.lds = (LDSAS int*)stk + lane;
+    st.ovf = gpw(ovf_base) + ((size_t)blockIdx.x * EXT_BLOCK + lane) * OVF_STACK;
+    st.sp = 0;
+    GLOBAL DCtl* ctl = gpw(P.ctl);
+    const uint32_t count = ctl->count[depth];
+    const GLOBAL float4* ino = gp(P.in.o);
+    const GLOBAL float4* ind = gp(P.in.d);
+    const GLOBAL DReq* reqs = gp(P.chunk)->reqs;
+    const unsigned long long lt_mask = (1ull << lane) - 1ull;
+    uint32_t ridx = NO_RAY;
+    bool drained = false;
+    /* wave-local pool [pool, pool_end) of ray indices taken from the queue 64 at a time */
+    uint32_t pool = 0, pool_end = 0;
+    TState T;
+    Best B;
+    TravCount tc = {0u, 0u};
+    for (;;) {
+        const bool idle = ridx == NO_RAY;
+        const unsigned long long idle_mask = __ballot(idle);
+        const uint32_t n_idle = (uint32_t)__popcll(idle_mask);
+        if (!drained && n_idle >= (uint32_t)REFILL_MIN) {
+            if (pool == pool_end) {
+                uint32_t base = 0;
+                if (lane == 0) base = atomicAdd(&ctl->fetch[depth], (uint32_t)EXT_BLOCK);
+                base = __shfl(base, 0);
+                pool = base < count ? base : count;
+                pool_end = base + EXT_BLOCK < count ? base + EXT_BLOCK : count;
+            }
+            const uint32_t take = n_idle < pool_end - pool ? n_idle : pool_end - pool;
+            if (idle) {
+                const uint32_t rank = (uint32_t)__popcll(idle_mask & lt_mask);
+                if (rank < take) {
+                    const uint32_t r = pool + rank;
+                    ridx = r;
+                    float4 o4 = ino[r], d4 = ind[r];
+                    const GLOBAL DScene* S = gp(reqs[f2u(o4.w) >> 16].scene);
+                    seg_init<QUANT, OVF>(T, B, st, S, mk3(o4.x, o4.y, o4.z), mk3(d4.x, d4.y, d4.z));
+                }
+            }
+            pool += take;
+            if (pool == pool_end && pool_end >= count) drained = true;
+        }
+        if (ridx != NO_RAY) {
+            if (seg_run<TRI, QUANT, COUNT, OVF>(T, B, st, ino, ind, ridx, &tc)) {
+                write_hit(P, ridx, B);
+                ridx = NO_RAY;
+            }
+        }
+        if (drained && __ballot(ridx != NO_RAY) == 0ull) break;
+    }
+    if (COUNT) {
+        atomicAdd(&P.trav_stats[0], (unsigned long long)tc.boxes);
+        atomicAdd(&P.trav_stats[1], (unsigned long long)tc.tris);
+    }
+}

@@ -182,6 +182,7 @@ typedef struct ilm_opts {
     int64_t hbm_budget_bytes;   /* serving admission budget; 0 = 90% of free device memory */
     int32_t use_graphs;         /* capture the per-chunk pipeline as a hipGraph */
     int32_t max_queued;         /* serving: admission queue bound (0 = 4096) */
+    int32_t extend_mode;        /* 0: one coherent ray batch per wave; 1: lanes refill as rays finish */
     int32_t profile;            /* time every extend launch with HIP events (disables graphs) */
     int32_t count_tests;        /* count box/triangle tests of the extend kernel (slower) */
     int32_t verbose;

@@ -1,0 +1,141 @@
+"""Batched and served forms on the GPU: each request's result is the same alone and in a mix,
+graph replays are identical to direct launches, and admission/cancellation behave."""
+import time
+
+import numpy as np
+import pytest
+
+import illume_amd
+import illume_amd.scenes as S
+from illume_amd._abi import render_params
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def scenes(ctx):
+    out = {}
+    for name, fn in [("cornell", S.cornell), ("tiny", S.tiny_mesh_scene), ("gold", lambda: S.gold_spheres(n=20))]:
+        spec, st = fn()
+        out[name] = (spec, st, ctx.scene(spec))
+    return out
+
+
+def _p(st, **kw):
+    d = dict(st)
+    d.update(kw)
+    return render_params(**d)
+
+
+MIX = [("cornell", dict(width=64, height=64, spp=5)), ("tiny", dict(width=80, height=60, spp=3)),
+       ("gold", dict(width=96, height=72, spp=4, max_depth=6)), ("cornell", dict(width=40, height=30, spp=7, first_sample=9)),
+       ("tiny", dict(width=33, height=17, spp=2, max_depth=20)), ("gold", dict(width=128, height=128, spp=2))]
+
+
+def test_graph_replays_match_direct_launches(scenes):
+    spec, st, _ = scenes["tiny"]
+    p = _p(st, width=64, height=48, spp=6)
+    with illume_amd.Context(0, use_graphs=0) as c0, illume_amd.Context(0, use_graphs=1, paths_in_flight=8192) as c1:
+        s0, s1 = c0.scene(spec), c1.scene(spec)
+        ref, _, i0 = c0.render(s0, p)
+        outs = []
+        for k in range(3):   # replays with eager work in between
+            a, _, i1 = c1.render(s1, p)
+            outs.append(a)
+            c0.render(s0, _p(st, width=16, height=16, spp=1))
+        for a in outs:
+            assert np.array_equal(a, ref)
+        assert i1["segments"] == i0["segments"] and i1["chunks"] > 1
+
+
+def test_batch_equals_individual_renders(ctx, scenes):
+    ss = [scenes[n][2] for n, _ in MIX]
+    ps = [_p(scenes[n][1], **kw) for n, kw in MIX]
+    batch, info = ctx.render_batch(ss, ps)
+    segs = 0
+    for (acc_b, px_b), s, p in zip(batch, ss, ps):
+        acc, px, i = ctx.render(s, p)
+        segs += i["segments"]
+        assert np.array_equal(acc_b, acc)
+        assert np.array_equal(px_b, px)
+    assert info["segments"] == segs
+
+
+def test_served_requests_equal_plain_renders(ctx, scenes):
+    reqs = []
+    for name, kw in MIX * 2:
+        reqs.append((ctx.submit(scenes[name][2], _p(scenes[name][1], **kw)), name, kw))
+    for r, name, kw in reqs:
+        r.wait(60000)
+        st = r.poll()
+        assert st["state"] == "done" and st["end_ms"] >= st["start_ms"] >= st["submit_ms"]
+        acc, px = r.result()
+        ref, refpx, _ = ctx.render(scenes[name][2], _p(scenes[name][1], **kw))
+        assert np.array_equal(acc, ref), (name, kw)
+        assert np.array_equal(px, refpx)
+        r.free()
+    s = ctx.stats()
+    assert s["requests_done"] >= len(reqs) and s["p99_ms"] >= s["p50_ms"] > 0
+
+
+def test_cancel_and_admission_limits(scenes):
+    spec, st, _ = scenes["gold"]
+    big = _p(st, width=512, height=512, spp=256)
+    # a 512x512 request holds 512*512*56 B = 14.7 MB of HBM while admitted; the budget fits one
+    with illume_amd.Context(0, max_queued=2, hbm_budget_bytes=20 * 2 ** 20) as c:
+        s = c.scene(spec)
+        # over the HBM budget on its own: rejected at submission
+        with pytest.raises(illume_amd.IlmError) as e:
+            c.submit(s, _p(st, width=2048, height=2048, spp=1))
+        assert e.value.code == -2
+
+        def wait_state(r, want):
+            for _ in range(2000):
+                if r.poll()["state"] == want:
+                    return
+                time.sleep(0.001)
+            raise AssertionError("request never reached " + want)
+        r1 = c.submit(s, big)          # admitted (runs)
+        wait_state(r1, "running")
+        r2 = c.submit(s, big)          # waits: the budget holds one
+        r3 = c.submit(s, big)
+        assert r2.poll()["state"] == "queued"
+        with pytest.raises(illume_amd.IlmError) as e:
+            c.submit(s, big)           # queue of 2 is full
+        assert e.value.code == -6
+        r2.cancel()
+        with pytest.raises(illume_amd.IlmError) as e:
+            r2.wait(60000)
+        assert e.value.code == -7
+        r1.cancel()
+        r3.cancel()
+        for r in (r1, r3):
+            try:
+                r.wait(60000)
+            except illume_amd.IlmError as ex:
+                assert ex.code == -7
+        st2 = c.stats()
+        assert st2["requests_rejected"] == 2 and st2["requests_cancelled"] >= 1
+        for r in (r1, r2, r3):
+            r.free()
+        s.free()
+
+
+def test_first_sample_offsets_compose(ctx, scenes):
+    spec, st, s = scenes["cornell"]
+    full, _, _ = ctx.render(s, _p(st, width=32, height=32, spp=6))
+    a, _, _ = ctx.render(s, _p(st, width=32, height=32, spp=2))
+    b, _, _ = ctx.render(s, _p(st, width=32, height=32, spp=4, first_sample=2))
+    assert np.allclose(full, a + b, rtol=1e-5, atol=1e-6)
+
+
+def test_quantized_and_watertight_renders_agree(ctx_quant, ctx_wt, ctx, scenes):
+    spec, st, s = scenes["tiny"]
+    p = _p(st, width=64, height=48, spp=4)
+    ref, _, i = ctx.render(s, p)
+    q, _, iq = ctx_quant.render(ctx_quant.scene(spec), p)
+    assert np.array_equal(q, ref) and iq["segments"] == i["segments"]
+    w, _, iw = ctx_wt.render(ctx_wt.scene(spec), p)
+    # watertight differs from Moller-Trumbore only on edge/grazing hits
+    rel = np.abs(w - ref) / np.maximum(np.abs(ref), 1e-3)
+    assert np.mean(rel < 1e-4) > 0.95
