@@ -3,7 +3,8 @@
     python -m illume_amd.build          # both
     python -m illume_amd.build --lib    # only the HIP library
 
-The HIP library is illume_amd/libillume_amd.so (C ABI of include/illume_c.h); the oracle is
+The HIP library is illume_amd/libillume_amd.so (C ABI of include/illume_c.h); the command line
+illume_amd/illume (`illume <save> <scene.json> <spp> <maxdepth>`) links it; the oracle is
 oracle/liboracle.so, built with g++ from oracle/oracle.cpp. Object files go to _build/.
 """
 import os
@@ -15,6 +16,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 CSRC = os.path.join(ROOT, "illume_amd", "csrc")
 OUT_LIB = os.path.join(ROOT, "illume_amd", "libillume_amd.so")
 OUT_ORACLE = os.path.join(ROOT, "oracle", "liboracle.so")
+OUT_CLI = os.path.join(ROOT, "illume_amd", "illume")
 BUILD_DIR = os.path.join(ROOT, "_build")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -54,6 +56,10 @@ def build_lib(verbose=False):
     tmp = OUT_LIB + ".tmp"
     _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-o", tmp] + objs + ["-lpthread"])
     os.replace(tmp, OUT_LIB)
+    tmp = OUT_CLI + ".tmp"
+    _run(["g++", "-std=c++17", "-O2", "-Wall", os.path.join(CSRC, "illume_cli.cpp"), "-o", tmp,
+          "-L" + os.path.dirname(OUT_LIB), "-lillume_amd", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, OUT_CLI)
     return OUT_LIB
 
 

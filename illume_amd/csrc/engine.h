@@ -16,6 +16,9 @@
 #define ILM_RING 8
 
 struct ilm_scene {
+    ~ilm_scene() {   /* device memory of an upload that failed part-way */
+        for (void* p : allocs) (void)hipFree(p);
+    }
     ilm_ctx* ctx = nullptr;
     std::atomic<int> refs{0};
     std::vector<void*> allocs;
@@ -64,6 +67,9 @@ struct ProfileSink {
 };
 
 struct ilm_ctx {
+    /* held by the caller's handle and by every live scene and request handle: the context's
+     * device resources are released when the last of them goes */
+    std::atomic<int> refs{1};
     int device = 0;
     hipStream_t stream = nullptr;
     ilm_opts opts;

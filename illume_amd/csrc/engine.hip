@@ -345,21 +345,32 @@ int ilm_scene_upload(ilm_ctx* ctx, const ilm_scene_desc* d, ilm_scene** out) {
         std::lock_guard<std::mutex> lk(ctx->sched_mu);
         ctx->in_use += s->bytes;
     }
+    ctx->refs.fetch_add(1);
     *out = s.release();
     return ILM_OK;
 }
 
-static void scene_release(ilm_scene* s) {
+void ctx_unref(ilm_ctx* ctx);
+
+/* Drops one reference; the last frees the scene's device memory. sched_locked: the caller holds
+ * the context's sched_mu (the scheduler's own release paths). */
+static void scene_release(ilm_scene* s, bool sched_locked = false) {
     if (!s) return;
     if (s->refs.fetch_sub(1) != 1) return;
     ilm_ctx* ctx = s->ctx;
-    hipSetDevice(ctx->device);
-    for (void* p : s->allocs) hipFree(p);
-    {
+    (void)hipSetDevice(ctx->device);
+    for (void* p : s->allocs) (void)hipFree(p);
+    s->allocs.clear();
+    (void)hipGetLastError();
+    if (sched_locked) {
+        ctx->in_use -= s->bytes;
+    } else {
         std::lock_guard<std::mutex> lk(ctx->sched_mu);
         ctx->in_use -= s->bytes;
     }
     delete s;
+    /* the scheduler thread never holds the last context reference (ilm_close joins it first) */
+    ctx_unref(ctx);
 }
 
 void ilm_scene_free(ilm_scene* s) { scene_release(s); }
@@ -449,21 +460,37 @@ int ilm_open(int device, const ilm_opts* opts, ilm_ctx** out) {
 
 static void server_stop(ilm_ctx* ctx);
 
+static void ctx_destroy(ilm_ctx* ctx) {
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto& kv : ctx->graphs) (void)hipGraphExecDestroy(kv.second);
+    for (void* p : ctx->allocs) (void)hipFree(p);
+    if (ctx->seg_count) (void)hipFree(ctx->seg_count);
+    for (int i = 0; i < ILM_RING; i++) {
+        if (ctx->h_chunk[i]) (void)hipHostFree(ctx->h_chunk[i]);
+        if (ctx->ring_ev[i]) (void)hipEventDestroy(ctx->ring_ev[i]);
+    }
+    if (ctx->h_u64) (void)hipHostFree(ctx->h_u64);
+    (void)hipStreamDestroy(ctx->stream);
+    (void)hipGetLastError();
+    delete ctx;
+}
+
+void ctx_unref(ilm_ctx* ctx) {
+    if (ctx->refs.fetch_sub(1) == 1) ctx_destroy(ctx);
+}
+
+/* Stops the scheduler (queued and running requests end cancelled) and drops the caller's
+ * reference; scenes and request handles still alive keep the device resources until freed. */
 void ilm_close(ilm_ctx* ctx) {
     if (!ctx) return;
     server_stop(ctx);
-    hipSetDevice(ctx->device);
-    hipStreamSynchronize(ctx->stream);
-    for (auto& kv : ctx->graphs) hipGraphExecDestroy(kv.second);
-    for (void* p : ctx->allocs) hipFree(p);
-    if (ctx->seg_count) hipFree(ctx->seg_count);
-    for (int i = 0; i < ILM_RING; i++) {
-        if (ctx->h_chunk[i]) hipHostFree(ctx->h_chunk[i]);
-        if (ctx->ring_ev[i]) hipEventDestroy(ctx->ring_ev[i]);
+    {
+        std::lock_guard<std::mutex> dev(ctx->dev_mu);
+        (void)hipSetDevice(ctx->device);
+        (void)hipStreamSynchronize(ctx->stream);
     }
-    if (ctx->h_u64) hipHostFree(ctx->h_u64);
-    hipStreamDestroy(ctx->stream);
-    delete ctx;
+    ctx_unref(ctx);
 }
 
 const char* ilm_last_error(ilm_ctx* ctx) {
@@ -559,7 +586,7 @@ static int launch_chunk(ilm_ctx* ctx, const DChunk& hc, bool ovf, ProfileSink* p
             if (rc2) return rc2;
             if (ec != hipSuccess) return fail(ctx, ILM_E_DEVICE, std::string("graph capture: ") + hipGetErrorString(ec));
             HIPTRY(ctx, hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
-            hipGraphDestroy(g);
+            (void)hipGraphDestroy(g);
             it = ctx->graphs.emplace(key, ge).first;
         }
         HIPTRY(ctx, hipGraphLaunch(it->second, ctx->stream));
@@ -593,8 +620,8 @@ static int job_prepare(ilm_ctx* ctx, Job* j) {
 }
 
 static void job_release_device(Job* j) {
-    if (j->d_accum) hipFree(j->d_accum);
-    if (j->d_px) hipFree(j->d_px);
+    if (j->d_accum) (void)hipFree(j->d_accum);
+    if (j->d_px) (void)hipFree(j->d_px);
     j->d_accum = nullptr;
     j->d_px = nullptr;
 }
@@ -705,15 +732,15 @@ static int run_jobs_sync(ilm_ctx* ctx, std::vector<Job*>& jobs, ilm_render_info*
     HIPTRY(ctx, hipStreamSynchronize(ctx->stream));
     float ms = 0.0f;
     HIPTRY(ctx, hipEventElapsedTime(&ms, e0, e1));
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     double ext_ms = 0.0;
     for (auto& pr : prof.ev) {
         float m = 0.0f;
-        hipEventElapsedTime(&m, pr.first, pr.second);
+        (void)hipEventElapsedTime(&m, pr.first, pr.second);
         ext_ms += m;
-        hipEventDestroy(pr.first);
-        hipEventDestroy(pr.second);
+        (void)hipEventDestroy(pr.first);
+        (void)hipEventDestroy(pr.second);
     }
     const unsigned long long segs = ctx->h_u64[0] - ctx->h_u64[3];
     if (info) {
@@ -918,7 +945,7 @@ int ilm_debug_path_radiance(ilm_ctx* ctx, ilm_scene* scene, const ilm_render_par
             if (segments) segments[base + i] = sc[i];
         }
     }
-    hipFree(d_list);
+    (void)hipFree(d_list);
     return rc;
 }
 
@@ -927,7 +954,7 @@ int ilm_debug_path_radiance(ilm_ctx* ctx, ilm_scene* scene, const ilm_render_par
 static int admit_locked(ilm_ctx* ctx);
 
 static void server_loop(ilm_ctx* ctx) {
-    hipSetDevice(ctx->device);
+    (void)hipSetDevice(ctx->device);
     std::unique_ptr<DChunk> hc(new DChunk());
     std::vector<Job*> in_chunk;
     struct Inflight { hipEvent_t ev; std::vector<Job*> finishing; };
@@ -955,17 +982,18 @@ static void server_loop(ilm_ctx* ctx) {
                 for (int w = 0; w < hc->n_work; w++) hc->work[w].pass0 += in_chunk[hc->work[w].req]->p.first_sample;
                 int rc = launch_chunk(ctx, *hc, jobs_need_ovf(ctx, in_chunk), nullptr);
                 Inflight f;
-                hipEventCreateWithFlags(&f.ev, hipEventDisableTiming);
+                (void)hipEventCreateWithFlags(&f.ev, hipEventDisableTiming);
                 for (Job* j : in_chunk) {
                     if (!job_has_work(j)) {
                         const int64_t wh = (int64_t)j->p.width * j->p.height;
-                        ilm_k_tonemap(j->d_accum, j->d_px, (int)wh, j->p.spp, j->p.tonemap, ctx->stream);
-                        hipMemcpyAsync(j->h_accum, j->d_accum, wh * sizeof(float4), hipMemcpyDeviceToHost, ctx->stream);
-                        hipMemcpyAsync(j->h_px, j->d_px, wh * sizeof(ilm_pixel), hipMemcpyDeviceToHost, ctx->stream);
+                        if (ilm_k_tonemap(j->d_accum, j->d_px, (int)wh, j->p.spp, j->p.tonemap, ctx->stream) != hipSuccess)
+                            j->failed = true;
+                        (void)hipMemcpyAsync(j->h_accum, j->d_accum, wh * sizeof(float4), hipMemcpyDeviceToHost, ctx->stream);
+                        (void)hipMemcpyAsync(j->h_px, j->d_px, wh * sizeof(ilm_pixel), hipMemcpyDeviceToHost, ctx->stream);
                         f.finishing.push_back(j);
                     }
                 }
-                hipEventRecord(f.ev, ctx->stream);
+                (void)hipEventRecord(f.ev, ctx->stream);
                 for (Job* j : in_chunk) j->last_ev_seq = ctx->chunk_seq;
                 ctx->chunk_seq++;
                 if (rc) {
@@ -982,7 +1010,7 @@ static void server_loop(ilm_ctx* ctx) {
             if (q == hipErrorNotReady) break;
             double t = now_ms(ctx);
             std::vector<Job*> done = f.finishing;
-            hipEventDestroy(f.ev);
+            (void)hipEventDestroy(f.ev);
             inflight.pop_front();
             ctx->chunks_retired++;
             std::lock_guard<std::mutex> lk(ctx->sched_mu);
@@ -996,7 +1024,7 @@ static void server_loop(ilm_ctx* ctx) {
                 j->state.store(j->failed ? ILM_REQ_FAILED : ILM_REQ_DONE);
                 j->samples_done = j->p.spp;
                 j->cv.notify_all();
-                if (--j->refs == 0) { scene_release(j->scene); delete j; }
+                if (--j->refs == 0) { scene_release(j->scene, true); delete j; }
             }
             launched = false;
         }
@@ -1007,7 +1035,7 @@ static void server_loop(ilm_ctx* ctx) {
                 Job* j = ctx->active[k];
                 bool busy = j->last_ev_seq >= ctx->chunk_seq - (int64_t)inflight.size();
                 if (j->cancel.load() && !busy) {
-                    hipStreamSynchronize(ctx->stream);
+                    (void)hipStreamSynchronize(ctx->stream);
                     job_release_device(j);
                     ctx->in_use -= j->bytes;
                     ctx->active.erase(ctx->active.begin() + k);
@@ -1015,7 +1043,7 @@ static void server_loop(ilm_ctx* ctx) {
                     ctx->stat_cancelled++;
                     j->state.store(ILM_REQ_CANCELLED);
                     j->cv.notify_all();
-                    if (--j->refs == 0) { scene_release(j->scene); delete j; }
+                    if (--j->refs == 0) { scene_release(j->scene, true); delete j; }
                     continue;
                 }
                 k++;
@@ -1034,7 +1062,7 @@ static int admit_locked(ilm_ctx* ctx) {
             j->end_ms = now_ms(ctx);
             ctx->stat_cancelled++;
             j->cv.notify_all();
-            if (--j->refs == 0) { scene_release(j->scene); delete j; }
+            if (--j->refs == 0) { scene_release(j->scene, true); delete j; }
             continue;
         }
         if (ctx->in_use + j->bytes > ctx->budget && !ctx->active.empty()) break;
@@ -1045,7 +1073,7 @@ static int admit_locked(ilm_ctx* ctx) {
                 job_release_device(j);
                 j->state.store(ILM_REQ_FAILED);
                 j->cv.notify_all();
-                if (--j->refs == 0) { scene_release(j->scene); delete j; }
+                if (--j->refs == 0) { scene_release(j->scene, true); delete j; }
                 continue;
             }
         }
@@ -1086,8 +1114,7 @@ int ilm_submit(ilm_ctx* ctx, ilm_scene* scene, const ilm_render_params* params, 
     j->bytes = wh * (2 * (int64_t)sizeof(float4) + (int64_t)sizeof(ilm_pixel));
     if (hipHostMalloc((void**)&j->h_accum, wh * sizeof(float4), hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc((void**)&j->h_px, wh * sizeof(ilm_pixel), hipHostMallocDefault) != hipSuccess) {
-        if (j->h_accum) hipHostFree(j->h_accum);
-        delete j;
+        delete j;   /* ~Job frees whichever buffer was allocated */
         return fail(ctx, ILM_E_NOMEM, "pinned result buffers");
     }
     {
@@ -1095,15 +1122,11 @@ int ilm_submit(ilm_ctx* ctx, ilm_scene* scene, const ilm_render_params* params, 
         int maxq = ctx->opts.max_queued > 0 ? ctx->opts.max_queued : 4096;
         if ((int)ctx->pending.size() >= maxq) {
             ctx->stat_rejected++;
-            hipHostFree(j->h_accum);
-            hipHostFree(j->h_px);
             delete j;
             return fail(ctx, ILM_E_BUSY, "admission queue full");
         }
         if (j->bytes > ctx->budget) {
             ctx->stat_rejected++;
-            hipHostFree(j->h_accum);
-            hipHostFree(j->h_px);
             delete j;
             return fail(ctx, ILM_E_NOMEM, "request exceeds the HBM budget");
         }
@@ -1118,6 +1141,7 @@ int ilm_submit(ilm_ctx* ctx, ilm_scene* scene, const ilm_render_params* params, 
     ilm_req* r = new ilm_req();
     r->job = j;
     r->ctx = ctx;
+    ctx->refs.fetch_add(1);
     *out = r;
     return ILM_OK;
 }
@@ -1186,19 +1210,20 @@ void ilm_req_free(ilm_req* r) {
         delete j;
     }
     delete r;
+    ctx_unref(ctx);
 }
 
 Job::~Job() {
-    if (h_accum) hipHostFree(h_accum);
-    if (h_px) hipHostFree(h_px);
+    if (h_accum) (void)hipHostFree(h_accum);
+    if (h_px) (void)hipHostFree(h_px);
 }
 
 int ilm_stats_get(ilm_ctx* ctx, ilm_stats* st) {
     if (!ctx || !st) return fail(ctx, ILM_E_INVALID, "null argument");
     unsigned long long segs = 0;
-    hipSetDevice(ctx->device);
-    hipStreamSynchronize(ctx->stream);
-    hipMemcpy(&segs, ctx->d_seg_total, sizeof(segs), hipMemcpyDeviceToHost);
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipMemcpy(&segs, ctx->d_seg_total, sizeof(segs), hipMemcpyDeviceToHost);
     std::lock_guard<std::mutex> lk(ctx->sched_mu);
     std::memset(st, 0, sizeof(*st));
     st->segments = segs;

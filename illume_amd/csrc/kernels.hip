@@ -488,21 +488,29 @@ __global__ void k_fill_zero4(float4* p, size_t n) {
 /* ---------------- host-side launchers ---------------- */
 #include "kernels.h"
 
+/* A launch reports its own error only: the thread's sticky error is cleared first, so a failed
+ * call elsewhere (whose status its caller already handled) is not blamed on the launch. */
+#define ILM_LAUNCH(...)                                                                            \
+    do {                                                                                           \
+        (void)hipGetLastError();                                                                   \
+        hipLaunchKernelGGL(__VA_ARGS__);                                                           \
+    } while (0)
+
 hipError_t ilm_k_ctl_init(const KParams& P, hipStream_t s) {
-    hipLaunchKernelGGL(k_ctl_init, dim3(1), dim3(256), 0, s, P.ctl, P.chunk);
+    ILM_LAUNCH(k_ctl_init, dim3(1), dim3(256), 0, s, P.ctl, P.chunk);
     return hipGetLastError();
 }
 hipError_t ilm_k_raygen(const KParams& P, int grid, hipStream_t s) {
-    hipLaunchKernelGGL(k_raygen, dim3(grid), dim3(256), 0, s, P);
+    ILM_LAUNCH(k_raygen, dim3(grid), dim3(256), 0, s, P);
     return hipGetLastError();
 }
 
 template <int T, int Q, int C, int O>
 static void launch_extend(const KParams& P, int depth, int* ovf, int grid, hipStream_t s) {
     if (P.extend_mode == 1)
-        hipLaunchKernelGGL((k_extend_refill<T, Q, C, O>), dim3(grid), dim3(EXT_BLOCK), 0, s, P, depth, ovf);
+        ILM_LAUNCH((k_extend_refill<T, Q, C, O>), dim3(grid), dim3(EXT_BLOCK), 0, s, P, depth, ovf);
     else
-        hipLaunchKernelGGL((k_extend<T, Q, C, O>), dim3(grid), dim3(EXT_BLOCK), 0, s, P, depth, ovf);
+        ILM_LAUNCH((k_extend<T, Q, C, O>), dim3(grid), dim3(EXT_BLOCK), 0, s, P, depth, ovf);
 }
 template <int T, int Q, int C>
 static void launch_extend_o(const KParams& P, int depth, int* ovf, int grid, hipStream_t s) {
@@ -526,27 +534,27 @@ hipError_t ilm_k_extend(const KParams& P, int depth, int* ovf, int grid, hipStre
     return hipGetLastError();
 }
 hipError_t ilm_k_shade(const KParams& P, int depth, int grid, hipStream_t s) {
-    hipLaunchKernelGGL(k_shade, dim3(grid), dim3(SHADE_BLOCK), 0, s, P, depth);
+    ILM_LAUNCH(k_shade, dim3(grid), dim3(SHADE_BLOCK), 0, s, P, depth);
     return hipGetLastError();
 }
 hipError_t ilm_k_accumulate(const KParams& P, int grid, hipStream_t s) {
-    hipLaunchKernelGGL(k_accumulate, dim3(grid), dim3(256), 0, s, P);
+    ILM_LAUNCH(k_accumulate, dim3(grid), dim3(256), 0, s, P);
     return hipGetLastError();
 }
 hipError_t ilm_k_chunk_end(const KParams& P, unsigned long long* seg_total, hipStream_t s) {
-    hipLaunchKernelGGL(k_chunk_end, dim3(1), dim3(64), 0, s, P.ctl, P.chunk, seg_total);
+    ILM_LAUNCH(k_chunk_end, dim3(1), dim3(64), 0, s, P.ctl, P.chunk, seg_total);
     return hipGetLastError();
 }
 hipError_t ilm_k_tonemap(const float4* accum, ilm_pixel* out, int n, int spp, int op, hipStream_t s) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_tonemap, dim3((n + 255) / 256), dim3(256), 0, s, accum, out, n, spp, op);
+    ILM_LAUNCH(k_tonemap, dim3((n + 255) / 256), dim3(256), 0, s, accum, out, n, spp, op);
     return hipGetLastError();
 }
 hipError_t ilm_k_fill_zero4(float4* p, size_t n, hipStream_t s) {
     if (n == 0) return hipSuccess;
     size_t g = (n + 255) / 256;
     if (g > 65535) g = 65535;
-    hipLaunchKernelGGL(k_fill_zero4, dim3((unsigned)g), dim3(256), 0, s, p, n);
+    ILM_LAUNCH(k_fill_zero4, dim3((unsigned)g), dim3(256), 0, s, p, n);
     return hipGetLastError();
 }
 template <class K>
