@@ -1,8 +1,20 @@
 """Benchmark of the path-tracer hot path on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 1] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 1] [--no-cpu] [--no-tune]
 
-A step is one full render of the workload (all passes and depths) through the C ABI
+Configurations (scene generators in illume_amd/scenes.py, fixed seeds):
+  0  Cornell box 256x256, 16 spp, depth 8 (8 instances, 36 triangles)
+  1  displaced icosphere (100,820 triangles) in rough gold + rough dielectric sphere + checker
+     plane, 1024x1024, 64 spp, depth 16                                       <- the default
+  2  displaced icosphere (524,880 triangles) holding a scattering medium inside a smooth
+     dielectric, 2048x1024 HDR sky + sun, 960x720, 32 spp, depth 45
+  3  batched form: 64 requests over configurations 0-2 (side 128..1024, 4..256 spp) in one
+     ilm_render_batch call, against the same requests rendered one at a time
+  4  served form: a prefix of the Poisson trace (20 req/s) over configurations 0-2 and a
+     12-instance 722k-triangle mesh, submitted at their arrival times to the scheduler, against
+     plain one-request-at-a-time renders of the same trace
+
+For 0-2 a step is one full render of the workload (all passes and depths) through the C ABI
 (ilm_render: ray generation, BVH extend, shade + compaction per depth, Kahan accumulation,
 tonemap and pixel readback), with the scene and its BVH already resident in HBM.
 Prints ONE JSON line on stdout; progress goes to stderr.
@@ -10,12 +22,13 @@ Prints ONE JSON line on stdout; progress goes to stderr.
 value       segments/s / 1e6 over the timed steps (wall clock around the synchronous calls);
             a segment is one closest-hit query (src/renderer.cu:554 counts the same)
 roofline    extend kernel: algorithmic bytes B_seg = 28*N_box + 36*N_tri + 128 per segment,
-            N_box / N_tri the mean child-box / triangle tests per query counted by the kernel's
-            own counters on the same workload; achieved = bytes / summed extend time (HIP
-            events around every extend launch on the render stream); peak 8 TB/s HBM3E
+            N_box / N_tri the mean child-box / triangle tests per query of the FP32 BVH counted
+            by the kernel's own counters on the same workload; achieved = B_seg * segments /
+            summed extend time (HIP events around every extend launch, on the render stream)
+            per launch; peak 8 TB/s HBM3E
 cpu_baseline  the oracle (CPU restatement of the reference per-path routine: exhaustive
             traversal of the reference's sweep-built BVH, linear instance loop) on a band of
-            the same image, OpenMP over pixels on the host's cores
+            the same image, OpenMP over pixels on the host's cores; a one-core figure beside it
 """
 import argparse
 import json
@@ -67,7 +80,9 @@ def traffic_from_profiles(cfg):
     return None
 
 
-def cpu_baseline(spec, st, target_s=12.0):
+def cpu_baseline(spec, st, target_s=15.0, target_1core_s=5.0):
+    """Oracle render (FP32, reference BVH and traversal) of a band of rows of the same image at
+    full spp, sized to about target_s of work on all cores; then one core on a smaller band."""
     from illume_amd._abi import render_params
     from oracle.pyoracle import OracleScene, lib as orc_lib
     threads = orc_lib().orc_threads_available()
@@ -75,39 +90,80 @@ def cpu_baseline(spec, st, target_s=12.0):
     orc = OracleScene(spec, threads=threads)
     build_s = time.time() - t0
     W, H = st["width"], st["height"]
-    # a band of rows through the middle of the image (where the geometry is)
-    rows = max(1, H // 8)
-    r0 = H // 2 - rows // 2
-    pb, pe = r0 * W, (r0 + rows) * W
-    p = render_params(**dict(st, spp=1))
-    _, _, segs1, secs1 = orc.render(p, pb, pe, kahan=True, threads=threads, pixels=False)
-    spp = int(max(1, min(st["spp"], target_s / max(secs1, 1e-3))))
-    p = render_params(**dict(st, spp=spp))
+    p = render_params(**st)
+
+    def band(rows):
+        rows = max(1, min(H, rows))
+        r0 = max(0, H // 2 - rows // 2)
+        return r0, rows, r0 * W, (r0 + rows) * W
+
+    # probe: two rows at full spp
+    r0, rows, pb, pe = band(2)
     _, _, segs, secs = orc.render(p, pb, pe, kahan=True, threads=threads, pixels=False)
+    per_row = max(secs, 1e-4) / rows
+    r0, rows, pb, pe = band(int(target_s / per_row))
+    _, _, segs, secs = orc.render(p, pb, pe, kahan=True, threads=threads, pixels=False)
+    out = {"value": segs / secs / 1e6, "unit": "Mrays/s", "cores": int(threads), "kind": "port",
+           "sample": "oracle FP32 render of rows %d..%d (%d px) x %d spp of the same workload: %d segments in %.2f s;"
+                     " reference sweep-BVH build %.1f s not timed" % (r0, r0 + rows, pe - pb, st["spp"], segs, secs, build_s)}
+    # one core: a run of pixels from the middle row on, at full spp
+    per_px_1core = per_row * threads / W
+    n1 = int(min(W * H // 2, max(64, target_1core_s / max(per_px_1core, 1e-9))))
+    pb = (H // 2) * W
+    _, _, segs1, secs1 = orc.render(p, pb, pb + n1, kahan=True, threads=1, pixels=False)
+    out["value_1core"] = segs1 / secs1 / 1e6
+    out["sample_1core"] = "%d px from row %d x %d spp: %d segments in %.2f s" % (n1, H // 2, st["spp"], segs1, secs1)
     orc.free()
-    return {"value": segs / secs / 1e6, "unit": "Mrays/s", "cores": int(threads), "kind": "port",
-            "sample": "oracle FP32 render of rows %d..%d (%d px) x %d spp of the same workload: %d segments in %.2f s;"
-                      " reference sweep-BVH build %.1f s not timed" % (r0, r0 + rows, pe - pb, spp, segs, secs, build_s)}
+    return out
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", type=int, default=1)
-    ap.add_argument("--layout", default="fp32", choices=["fp32", "quant8"])
-    ap.add_argument("--tri", default="mt", choices=["mt", "watertight"])
-    ap.add_argument("--paths", type=int, default=0, help="paths in flight per chunk (0: library default)")
-    ap.add_argument("--extend-mode", type=int, default=0, help="0: one ray batch per wave, 1: lane refill")
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-count", action="store_true")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print("bench.py: this build measures one GPU only (the path shards by request or sample range; "
-              "multi-GPU runs are not part of it). Use --gpus 1.", file=sys.stderr)
-        sys.exit(2)
+def rel_rmse(gpu_sum, ref_sum, spp):
+    g = np.asarray(gpu_sum, np.float64) / spp
+    o = np.asarray(ref_sum, np.float64) / spp
+    return float(np.sqrt(np.mean((g - o) ** 2)) / max(np.mean(np.abs(o)), 1e-12))
 
+
+class Fp64Oracle:
+    """FP64 oracle renders of pixel samples of GPU results (median-split BVH: closest hits do not
+    depend on the tree except for ties)."""
+
+    def __init__(self, specs, threads):
+        from oracle.pyoracle import OracleScene
+        self.threads = threads
+        t0 = time.time()
+        self.scenes = {k: OracleScene(s, fp64=True, threads=threads, median_bvh=True) for k, s in specs.items()}
+        log("fp64 oracle scenes built in %.1f s" % (time.time() - t0))
+
+    def rmse(self, key, params, gpu_acc, npx=128):
+        W, H = params.width, params.height
+        n = min(npx, W)
+        pb = (H // 2) * W + (W - n) // 2
+        ref, _, _, _ = self.scenes[key].render(params, pb, pb + n, kahan=True, fp64=True, threads=self.threads,
+                                               pixels=False)
+        g = np.asarray(gpu_acc).reshape(-1, 3)
+        return rel_rmse(g[pb:pb + n], ref, params.spp), "%d px of row %d" % (n, H // 2)
+
+    def free(self):
+        for s in self.scenes.values():
+            s.free()
+
+
+def base_line(args, metric, value, unit, ms_per_step, config, dtype="f32"):
+    return {"metric": metric, "value": round(value, 3), "unit": unit, "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": dtype, "data": "synthetic", "config": config}
+
+
+def ctx_opts(args):
+    o = dict(bvh_layout=args.layout, tri_test=args.tri, paths_in_flight=args.paths, extend_mode=args.extend_mode)
+    if args.sample_group is not None:
+        o["sample_group"] = args.sample_group
+    if args.tile_rows is not None:
+        o["tile_rows"] = args.tile_rows
+    return o
+
+
+def bench_plain(args):
     import illume_amd
     from illume_amd import scenes as S
     from illume_amd._abi import render_params
@@ -115,13 +171,21 @@ def main():
     t0 = time.time()
     spec, st = S.CONFIGS[args.config]()
     log("scene generated in %.2f s" % (time.time() - t0))
-    ctx = illume_amd.Context(0, bvh_layout=args.layout, tri_test=args.tri, profile=1, paths_in_flight=args.paths,
-                             extend_mode=args.extend_mode)
+    ctx = illume_amd.Context(0, profile=1, **ctx_opts(args))
     t0 = time.time()
     scene = ctx.scene(spec)
     upload_s = time.time() - t0
     log("scene upload + BVH build %.2f s, %.1f MB on device" % (upload_s, scene.device_bytes / 1e6))
     p = render_params(**st)
+
+    tuned = None
+    if not args.no_tune:
+        from illume_amd.autotune import autotune
+
+        def trial(c):
+            _, _, i = c.render(scene, p, want_accum=False, want_pixels=False)
+            return i["segments"], i["device_ms"]
+        tuned, _ = autotune(ctx, trial, repeats=1, log=log, budget_s=args.tune_budget)
 
     for i in range(args.warmup):
         _, _, info = ctx.render(scene, p, want_accum=False)
@@ -146,11 +210,14 @@ def main():
     wall = time.perf_counter() - t_start
     value = segs / wall / 1e6
 
-    # traversal test counts on the same workload (a separate, untimed render)
+    # traversal test counts on the same workload, FP32 nodes (a separate, untimed render)
     roof = None
     if not args.no_count:
-        cctx = illume_amd.Context(0, bvh_layout=args.layout, tri_test=args.tri, count_tests=1,
-                                  paths_in_flight=args.paths, extend_mode=args.extend_mode)
+        copts = ctx_opts(args)
+        copts["bvh_layout"] = "fp32"
+        cctx = illume_amd.Context(0, count_tests=1, **copts)
+        if tuned:
+            cctx.tune(**tuned)
         cscene = cctx.scene(spec)
         _, _, cinfo = cctx.render(cscene, p, want_accum=False, want_pixels=False)
         n_box = cinfo["box_tests"] / max(1, cinfo["segments"])
@@ -166,7 +233,8 @@ def main():
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": "k_extend", "b_seg": round(b_seg, 1), "n_box": round(n_box, 2), "n_tri": round(n_tri, 2),
                 "avg_launch_ms": round(avg_launch_s * 1e3, 4), "launches": ext_launches,
-                "algorithmic_bytes_per_launch": round(per_launch_bytes, 1)}
+                "algorithmic_bytes_per_launch": round(per_launch_bytes, 1),
+                "segments_per_s_at_roofline_M": round(HBM_PEAK_GBS * 1e9 / b_seg / 1e6, 1)}
         log("roofline: N_box %.2f N_tri %.2f B_seg %.1f B, extend %.1f ms over %d launches -> %.1f GB/s"
             % (n_box, n_tri, b_seg, ext_ms, ext_launches, achieved))
 
@@ -174,26 +242,236 @@ def main():
     if not args.no_cpu:
         try:
             cpu = cpu_baseline(spec, st)
-            log("cpu baseline: %.2f Mrays/s on %d cores" % (cpu["value"], cpu["cores"]))
+            log("cpu baseline: %.2f Mrays/s on %d cores, %.3f on one" % (cpu["value"], cpu["cores"], cpu["value_1core"]))
         except Exception as e:  # the GPU number stands without it
             log("cpu baseline failed: %r" % (e,))
 
-    out = {
-        "metric": METRIC, "value": round(value, 3), "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(wall * 1e3 / args.steps, 3), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": workload_name(args.config, spec, st), "width": st["width"], "height": st["height"],
-                   "spp": st["spp"], "max_depth": st["max_depth"], "bvh_layout": args.layout, "tri_test": args.tri,
-                   "segments_per_step": segs // max(1, args.steps), "parallelism": "1 GPU"},
-        "latency_ms": {"p50": round(percentile(lat, 0.5), 3), "p99": round(percentile(lat, 0.99), 3),
-                       "form": "plain (one request per ilm_render)"},
-        "value_device": round(segs / (dev_ms / 1e3) / 1e6, 3),
-        "roofline": roof,
-        "cpu_baseline": cpu,
-    }
+    config = {"workload": workload_name(args.config, spec, st), "width": st["width"], "height": st["height"],
+              "spp": st["spp"], "max_depth": st["max_depth"], "bvh_layout": args.layout, "tri_test": args.tri,
+              "segments_per_step": segs // max(1, args.steps), "parallelism": "1 GPU",
+              "launch_shape": ctx.settings(), "autotuned": tuned is not None}
+    out = base_line(args, METRIC, value, "Mrays/s", wall * 1e3 / args.steps, config)
+    out["latency_ms"] = {"p50": round(percentile(lat, 0.5), 3), "p99": round(percentile(lat, 0.99), 3),
+                         "form": "plain (one request per ilm_render)"}
+    out["value_device"] = round(segs / (dev_ms / 1e3) / 1e6, 3)
+    out["roofline"] = roof
+    out["cpu_baseline"] = cpu
     print(json.dumps(out), flush=True)
     scene.free()
     ctx.close()
+
+
+def _upload_mix(ctx, specs):
+    t0 = time.time()
+    out = {k: ctx.scene(s) for k, s in specs.items()}
+    log("uploaded %d scenes in %.1f s" % (len(out), time.time() - t0))
+    return out
+
+
+def bench_batch(args):
+    """Batched form: one ilm_render_batch over 64 requests vs the same requests one at a time."""
+    import illume_amd
+    from illume_amd import scenes as S
+    from illume_amd._abi import render_params
+
+    reqs = S.batch_requests()[:args.requests or None]
+    specs, sts = {}, {}
+    for k in sorted({r["scene"] for r in reqs}):
+        specs[k], sts[k] = S.MIX_SCENES[k]()
+    params = [render_params(**dict(sts[r["scene"]], width=r["width"], height=r["height"], spp=r["spp"])) for r in reqs]
+    results = {}
+    for layout in ("fp32", "quant8"):
+        opts = ctx_opts(args)
+        opts["bvh_layout"] = layout
+        with illume_amd.Context(0, **opts) as ctx:
+            sc = _upload_mix(ctx, specs)
+            ss = [sc[r["scene"]] for r in reqs]
+            ctx.render_batch(ss[:4], params[:4])   # warm-up (graph capture)
+            t0 = time.perf_counter()
+            batch, binfo = ctx.render_batch(ss, params)
+            t_batch = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            plain = []
+            psegs = 0
+            for s, p in zip(ss, params):
+                a, px, i = ctx.render(s, p)
+                plain.append((a, px))
+                psegs += i["segments"]
+            t_plain = time.perf_counter() - t0
+            same = sum(int(np.array_equal(b[0], q[0]) and np.array_equal(b[1], q[1])) for b, q in zip(batch, plain))
+            results[layout] = dict(batch_s=t_batch, plain_s=t_plain, segments=binfo["segments"], plain_segments=psegs,
+                                   identical=same, accs=[b[0] for b in batch])
+            log("%s: batch %.2f s (%.1f Mrays/s), plain %.2f s (%.1f Mrays/s), %d/%d identical"
+                % (layout, t_batch, binfo["segments"] / t_batch / 1e6, t_plain, psegs / t_plain / 1e6, same, len(reqs)))
+            for s in sc.values():
+                s.free()
+    rm = {}
+    if not args.no_rmse:
+        from oracle.pyoracle import lib as orc_lib
+        orc = Fp64Oracle(specs, orc_lib().orc_threads_available())
+        t0 = time.time()
+        for layout in results:
+            errs = [orc.rmse(r["scene"], p, acc)[0] for r, p, acc in zip(reqs, params, results[layout]["accs"])]
+            rm[layout] = {"max": round(max(errs), 5), "mean": round(float(np.mean(errs)), 5)}
+        orc.free()
+        log("rmse vs fp64 oracle %s (%.1f s)" % (rm, time.time() - t0))
+    r = results["quant8"]
+    config = {"workload": "config3: %d requests (seed 3) over configs 0-2, side 128..1024, 4..256 spp, "
+                          "one ilm_render_batch call, quantised 8-bit BVH child boxes" % len(reqs),
+              "requests": len(reqs), "parallelism": "1 GPU", "launch_shape": "library defaults"}
+    out = base_line(args, "batched path segments/s (Mrays/s)", r["segments"] / r["batch_s"] / 1e6, "Mrays/s",
+                    r["batch_s"] * 1e3, config)
+    out["forms"] = {k: {"batched_Mrays_s": round(v["segments"] / v["batch_s"] / 1e6, 2),
+                        "plain_Mrays_s": round(v["plain_segments"] / v["plain_s"] / 1e6, 2),
+                        "batched_s": round(v["batch_s"], 3), "plain_s": round(v["plain_s"], 3),
+                        "speedup": round(v["plain_s"] / v["batch_s"], 3),
+                        "identical_to_plain": "%d/%d" % (v["identical"], len(reqs))} for k, v in results.items()}
+    out["rmse_vs_fp64"] = rm or None
+    out["rmse_sample"] = "128 px of the middle row of every request, same spp, FP64 oracle with the same RNG stream"
+    out["roofline"] = None
+    out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+def _replay(ctx_or_none, trace, start, submit_fn):
+    for r in trace:
+        dt = start + r["t"] - time.perf_counter()
+        if dt > 0:
+            time.sleep(dt)
+        submit_fn(r)
+
+
+def bench_served(args):
+    """Served form vs plain one-request-at-a-time renders on a prefix of the Poisson trace."""
+    import illume_amd
+    from illume_amd import scenes as S
+    from illume_amd._abi import render_params
+
+    trace = S.serve_trace()[:args.requests or 24]
+    t0 = trace[0]["t"]
+    trace = [dict(r, t=r["t"] - t0) for r in trace]
+    specs, sts = {}, {}
+    for k in sorted({r["scene"] for r in trace}):
+        specs[k], sts[k] = S.MIX_SCENES[k]()
+    params = [render_params(**dict(sts[r["scene"]], width=r["width"], height=r["height"], spp=r["spp"],
+                                   max_depth=r["max_depth"])) for r in trace]
+    forms = {}
+    accs = {}
+    with illume_amd.Context(0, **ctx_opts(args)) as ctx:
+        sc = _upload_mix(ctx, specs)
+        # warm-up: capture the graphs of every depth in the trace
+        for d in sorted({p.max_depth for p in params}):
+            ctx.render(sc[0], render_params(**dict(sts[0], width=64, height=64, spp=1, max_depth=d)))
+
+        def plain_form(name, **tune):
+            saved = ctx.settings()
+            if tune:
+                ctx.tune(**tune)
+            lat, segs = [], 0
+            out = []
+            start = time.perf_counter()
+            for r, p in zip(trace, params):
+                dt = start + r["t"] - time.perf_counter()
+                if dt > 0:
+                    time.sleep(dt)
+                a, _, i = ctx.render(sc[r["scene"]], p, want_pixels=False)
+                lat.append((time.perf_counter() - start - r["t"]) * 1e3)
+                segs += i["segments"]
+                out.append(a)
+            span = time.perf_counter() - start
+            ctx.tune(**saved)
+            forms[name] = dict(Mrays_s=segs / span / 1e6, makespan_s=span, segments=segs,
+                               p50_ms=percentile(lat, 0.5), p99_ms=percentile(lat, 0.99))
+            log("%s: %.1f Mrays/s, makespan %.2f s, p50 %.0f ms, p99 %.0f ms"
+                % (name, segs / span / 1e6, span, forms[name]["p50_ms"], forms[name]["p99_ms"]))
+            return out
+
+        # served: submissions at arrival times, the scheduler batches whatever is admitted
+        s0 = ctx.stats()
+        handles = []
+        start = time.perf_counter()
+        _replay(None, trace, start, lambda r: handles.append(ctx.submit(sc[r["scene"]], params[len(handles)])))
+        served_acc = []
+        lat = []
+        for h in handles:
+            h.wait()
+        span = time.perf_counter() - start
+        for h in handles:
+            a, _ = h.result()
+            served_acc.append(a)
+            st = h.poll()
+            lat.append(st["end_ms"] - st["submit_ms"])
+        s1 = ctx.stats()
+        segs = s1["segments"] - s0["segments"]
+        forms["served"] = dict(Mrays_s=segs / span / 1e6, makespan_s=span, segments=segs,
+                               p50_ms=percentile(lat, 0.5), p99_ms=percentile(lat, 0.99))
+        log("served: %.1f Mrays/s, makespan %.2f s, p50 %.0f ms, p99 %.0f ms"
+            % (segs / span / 1e6, span, forms["served"]["p50_ms"], forms["served"]["p99_ms"]))
+        for h in handles:
+            h.free()
+        accs["served"] = served_acc
+        accs["plain"] = plain_form("plain")
+        plain_form("plain_per_pass", chunk_passes=1)
+        same = sum(int(np.array_equal(a, b)) for a, b in zip(accs["served"], accs["plain"]))
+        for s in sc.values():
+            s.free()
+    rm = None
+    if not args.no_rmse:
+        from oracle.pyoracle import lib as orc_lib
+        orc = Fp64Oracle(specs, orc_lib().orc_threads_available())
+        errs = [orc.rmse(r["scene"], p, a)[0] for r, p, a in zip(trace, params, accs["served"])]
+        orc.free()
+        rm = {"max": round(max(errs), 5), "mean": round(float(np.mean(errs)), 5)}
+        log("rmse vs fp64 oracle: %s" % rm)
+    sv = forms["served"]
+    config = {"workload": "config4: first %d requests of the Poisson trace (seed 4, 20 req/s) over configs 0-2 and "
+                          "the 12-instance 722k-tri mesh, 256^2..1920x1080, 8..512 spp, depth 8..32" % len(trace),
+              "requests": len(trace), "parallelism": "1 GPU"}
+    out = base_line(args, "served path segments/s (Mrays/s); p50/p99 request latency", sv["Mrays_s"], "Mrays/s",
+                    sv["makespan_s"] * 1e3, config)
+    out["latency_ms"] = {"p50": round(sv["p50_ms"], 1), "p99": round(sv["p99_ms"], 1),
+                         "form": "served (completion - arrival)"}
+    out["forms"] = {k: {kk: round(vv, 3) for kk, vv in v.items()} for k, v in forms.items()}
+    out["served_vs_plain"] = round(sv["Mrays_s"] / forms["plain"]["Mrays_s"], 3)
+    out["served_vs_plain_per_pass"] = round(sv["Mrays_s"] / forms["plain_per_pass"]["Mrays_s"], 3)
+    out["identical_to_plain"] = "%d/%d" % (same, len(trace))
+    out["rmse_vs_fp64"] = rm
+    out["roofline"] = None
+    out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", type=int, default=1)
+    ap.add_argument("--layout", default="fp32", choices=["fp32", "quant8"])
+    ap.add_argument("--tri", default="mt", choices=["mt", "watertight"])
+    ap.add_argument("--paths", type=int, default=0, help="paths in flight per chunk (0: library default)")
+    ap.add_argument("--extend-mode", type=int, default=0, help="0: one ray batch per wave, 1: lane refill")
+    ap.add_argument("--sample-group", type=int, default=None)
+    ap.add_argument("--tile-rows", type=int, default=None)
+    ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (library default launch shape)")
+    ap.add_argument("--tune-budget", type=float, default=30.0)
+    ap.add_argument("--requests", type=int, default=0, help="configs 3/4: number of requests")
+    ap.add_argument("--no-rmse", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-count", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU only (the path shards by request or sample range; "
+              "multi-GPU runs are not part of it). Use --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+    if args.config in (0, 1, 2):
+        bench_plain(args)
+    elif args.config == 3:
+        bench_batch(args)
+    elif args.config == 4:
+        bench_served(args)
+    else:
+        sys.exit("unknown config %d" % args.config)
 
 
 if __name__ == "__main__":

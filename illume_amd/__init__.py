@@ -42,6 +42,7 @@ def lib():
         "ilm_close": (None, [vp]),
         "ilm_last_error": (C.c_char_p, [vp]),
         "ilm_stats_get": (C.c_int, [vp, P(Stats)]),
+        "ilm_tune": (C.c_int, [vp, P(Opts)]),
         "ilm_scene_upload": (C.c_int, [vp, P(SceneDesc), P(vp)]),
         "ilm_scene_free": (None, [vp]),
         "ilm_scene_device_bytes": (C.c_int64, [vp]),
@@ -113,6 +114,22 @@ class Context:
         _check(L.ilm_open(int(device), C.byref(o), C.byref(h)))
         self.h = h
         self.opts = o
+
+    TUNABLE = ("extend_mode", "sample_group", "tile_rows", "chunk_passes", "paths_in_flight")
+
+    def tune(self, **kw):
+        """Change the tunable options (TUNABLE) of an idle context; others keep their values."""
+        o = Opts()
+        C.memmove(C.byref(o), C.byref(self.opts), C.sizeof(Opts))
+        for k, v in kw.items():
+            if k not in self.TUNABLE:
+                raise ValueError("not tunable: %s" % k)
+            setattr(o, k, int(v))
+        _check(lib().ilm_tune(self.h, C.byref(o)), self.h)
+        self.opts = o
+
+    def settings(self):
+        return {k: getattr(self.opts, k) for k in self.TUNABLE}
 
     def close(self):
         if self.h:

@@ -93,7 +93,8 @@ class Pixel(C.Structure):
 class Opts(C.Structure):
     _fields_ = [("bvh_layout", C.c_int32), ("tri_test", C.c_int32), ("paths_in_flight", C.c_int64),
                 ("hbm_budget_bytes", C.c_int64), ("use_graphs", C.c_int32), ("max_queued", C.c_int32),
-                ("extend_mode", C.c_int32), ("profile", C.c_int32), ("count_tests", C.c_int32), ("verbose", C.c_int32)]
+                ("extend_mode", C.c_int32), ("profile", C.c_int32), ("count_tests", C.c_int32), ("verbose", C.c_int32),
+                ("sample_group", C.c_int32), ("tile_rows", C.c_int32), ("chunk_passes", C.c_int32)]
 
 
 class RenderInfo(C.Structure):

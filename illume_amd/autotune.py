@@ -1,0 +1,68 @@
+"""Autotuner for the launch shape of the wavefront pipeline.
+
+The tunables change how paths are laid out and queued, never what is computed: every choice
+gives bit-identical images (samples are summed per pixel in pass order whatever the path order
+or chunking), so the tuner only measures time.
+
+    extend_mode      0: each wave traces one batch of 64 rays to completion
+                     1: lanes refill from a wave-local queue as their rays finish
+    sample_group     consecutive path ids per pixel (ray coherence of the first bounces)
+    tile_rows        pixel walk in bands of this many rows, column by column (wave footprint)
+    paths_in_flight  paths per chunk (queue size): larger chunks amortise launch tails,
+                     smaller ones bound the memory of the path state
+    chunk_passes     passes of one request per chunk (0: as many as fit)
+
+The search is coordinate descent from the context's current settings: each knob in turn is set
+to every candidate value with the others held, and the fastest is kept. The objective is device
+time per segment of a caller-supplied trial (normally the workload itself or a reduced-spp copy
+of it), the best of `repeats` runs.
+"""
+import time
+
+DEFAULT_SPACE = (
+    ("sample_group", (1, 4, 16, 64)),
+    ("tile_rows", (1, 4, 8)),
+    ("extend_mode", (0, 1)),
+    ("paths_in_flight", (1 << 24, 1 << 25, 1 << 26)),
+)
+
+
+def _measure(ctx, run, repeats):
+    best = None
+    for _ in range(repeats):
+        segs, ms = run(ctx)
+        rate = segs / max(ms, 1e-6)
+        best = rate if best is None else max(best, rate)
+    return best
+
+
+def autotune(ctx, run, space=DEFAULT_SPACE, repeats=2, log=None, budget_s=60.0):
+    """Tune ctx in place. run(ctx) renders the trial workload and returns (segments, device_ms).
+    Returns (best settings dict, list of trial records)."""
+    t0 = time.time()
+    cur = ctx.settings()
+    run(ctx)   # warm-up: graph capture, first-touch of buffers
+    best_rate = _measure(ctx, run, repeats)
+    trials = [dict(cur, rate=best_rate)]
+    if log:
+        log("autotune start %s: %.1f Mseg/s" % (cur, best_rate / 1e3))
+    for knob, values in space:
+        for v in values:
+            if time.time() - t0 > budget_s:
+                break
+            if v == cur[knob]:
+                continue
+            cand = dict(cur)
+            cand[knob] = v
+            ctx.tune(**cand)
+            run(ctx)
+            rate = _measure(ctx, run, repeats)
+            trials.append(dict(cand, rate=rate))
+            if log:
+                log("autotune %s=%s: %.1f Mseg/s" % (knob, v, rate / 1e3))
+            if rate > best_rate:
+                best_rate, cur = rate, cand
+        ctx.tune(**cur)
+    if log:
+        log("autotune pick %s: %.1f Mseg/s (%d trials, %.1f s)" % (cur, best_rate / 1e3, len(trials), time.time() - t0))
+    return cur, trials

@@ -159,7 +159,7 @@ struct DWork {
     uint32_t npx;
     uint32_t pid_base;      /* first path id of this slice */
     uint32_t acc_base;      /* first accumulate index of this slice (prefix of npx) */
-    uint32_t pad;
+    uint32_t sgroup;        /* consecutive path ids per pixel (power of two dividing npass) */
 };
 
 #define ILM_MAX_JOBS 256
@@ -214,7 +214,30 @@ struct KParams {
     int tri_test;
     int extend_mode;
     int use_ovf;            /* 1 when a scene's stack bound exceeds the LDS stack depth */
+    int tile_rows;          /* pixel order: bands of this many rows walked column-major (1 = rows) */
 };
+
+/* Path id order of a slice: local index l <-> (sample k of the slice, pixel slot q). Samples
+ * come in groups of g consecutive ids per pixel: l = (k / g) * npx * g + q * g + k % g. */
+__host__ __device__ inline uint32_t slice_local(uint32_t k, uint32_t q, uint32_t npx, uint32_t g) {
+    return (k / g) * npx * g + q * g + (k % g);
+}
+__host__ __device__ inline void slice_decode(uint32_t l, uint32_t npx, uint32_t g, uint32_t& k, uint32_t& q) {
+    const uint32_t span = npx * g;
+    const uint32_t kb = l / span, r = l - kb * span;
+    q = r / g;
+    k = kb * g + (r - q * g);
+}
+/* Pixel slot i (0 <= i < w*h) -> pixel index: bands of th rows, each walked column by column
+ * (th consecutive slots cover one column of the band); rows past the last full band in order. */
+__host__ __device__ inline uint32_t tile_pixel(uint32_t i, uint32_t w, uint32_t h, uint32_t th) {
+    if (th <= 1) return i;
+    const uint32_t band_px = th * w, full = (h / th) * band_px;
+    if (i >= full) return i;
+    const uint32_t band = i / band_px, r = i - band * band_px;
+    const uint32_t x = r / th, y = band * th + (r - x * th);
+    return y * w + x;
+}
 
 /* ---------------- vector helpers (reference operation order) ---------------- */
 struct V3 { float x, y, z; };

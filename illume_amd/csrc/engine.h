@@ -89,7 +89,8 @@ struct ilm_ctx {
     unsigned long long* d_seg_total = nullptr;
     unsigned long long* d_trav = nullptr;
     std::vector<void*> allocs;
-    int64_t own_bytes = 0;
+    std::vector<void*> path_allocs;  /* path state, hits, contributions (resized by ilm_tune) */
+    int64_t own_bytes = 0, path_bytes = 0;
 
     /* chunk descriptor staging */
     DChunk* h_chunk[ILM_RING] = {};

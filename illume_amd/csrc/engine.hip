@@ -384,10 +384,22 @@ int ilm_debug_instance_matrices(const ilm_scene* s, int32_t i, float* out48) {
 
 /* ======================= context ======================= */
 
+static void ctx_free_paths(ilm_ctx* ctx) {
+    for (void* p : ctx->path_allocs) (void)hipFree(p);
+    ctx->path_allocs.clear();
+    ctx->own_bytes -= ctx->path_bytes;
+    ctx->path_bytes = 0;
+    ctx->A = DPaths{};
+    ctx->B = DPaths{};
+    ctx->hits = DHits{};
+    ctx->contrib = nullptr;
+    ctx->cap = 0;
+}
+
 static int ctx_alloc_paths(ilm_ctx* ctx, int64_t cap) {
     auto A = [&](void** p, size_t b) -> hipError_t {
         hipError_t e = hipMalloc(p, b);
-        if (e == hipSuccess) { ctx->allocs.push_back(*p); ctx->own_bytes += (int64_t)b; }
+        if (e == hipSuccess) { ctx->path_allocs.push_back(*p); ctx->own_bytes += (int64_t)b; ctx->path_bytes += (int64_t)b; }
         return e;
     };
     for (DPaths* P : {&ctx->A, &ctx->B}) {
@@ -403,9 +415,29 @@ static int ctx_alloc_paths(ilm_ctx* ctx, int64_t cap) {
     return ILM_OK;
 }
 
+static bool opts_valid(const ilm_opts& o) {
+    const bool sg_ok = o.sample_group >= 0 && o.sample_group <= 4096 && (o.sample_group & (o.sample_group - 1)) == 0;
+    return !(o.bvh_layout < 0 || o.bvh_layout > ILM_BVH_QUANT8 || o.tri_test < 0 || o.tri_test > ILM_TRI_WATERTIGHT ||
+             o.extend_mode < 0 || o.extend_mode > 1 || !sg_ok || o.tile_rows < 0 || o.tile_rows > 256 ||
+             o.chunk_passes < 0 || o.paths_in_flight < 0 || o.hbm_budget_bytes < 0 || o.max_queued < 0);
+}
+
+static int64_t cap_of(const ilm_opts& o) {
+    /* 2^26 paths (~9.5 GB of path state) hold a whole 1024x1024x64 render in one chunk: every
+     * depth is one launch over all live paths, so launch tails are amortised */
+    int64_t cap = o.paths_in_flight > 0 ? o.paths_in_flight : (int64_t)1 << 26;
+    cap = std::max<int64_t>(cap, 4096);
+    return std::min<int64_t>(cap, (int64_t)1 << 30);
+}
+
+static size_t ovf_bytes(const ilm_ctx* ctx) {
+    return (size_t)ctx->ext_grid * ilm_k_ext_block() * ilm_k_ovf_stack() * sizeof(int);
+}
+
 int ilm_open(int device, const ilm_opts* opts, ilm_ctx** out) {
     if (!out) return fail(nullptr, ILM_E_INVALID, "null out");
     *out = nullptr;
+    if (opts && !opts_valid(*opts)) return fail(nullptr, ILM_E_INVALID, "bad ilm_opts field");
     int n = 0;
     hipError_t e = hipGetDeviceCount(&n);
     if (e != hipSuccess || n == 0) return fail(nullptr, ILM_E_DEVICE, "no HIP device available");
@@ -426,20 +458,15 @@ int ilm_open(int device, const ilm_opts* opts, ilm_ctx** out) {
     ctx->ext_grid = ctx->n_cu * std::max(1, occ);
     ctx->gen_grid = ctx->n_cu * 8;
     ctx->shade_grid = ctx->n_cu * 8;
-    /* 2^26 paths (~9.5 GB of path state) hold a whole 1024x1024x64 render in one chunk: every
-     * depth is one launch over all live paths, so launch tails are amortised */
-    int64_t cap = ctx->opts.paths_in_flight > 0 ? ctx->opts.paths_in_flight : (int64_t)1 << 26;
-    cap = std::max<int64_t>(cap, 4096);
-    cap = std::min<int64_t>(cap, (int64_t)1 << 30);
-    int rc = ctx_alloc_paths(ctx, cap);
+    int rc = ctx_alloc_paths(ctx, cap_of(ctx->opts));
     if (rc) return rc;
     auto A = [&](void** p, size_t b) -> hipError_t {
         hipError_t e2 = hipMalloc(p, b);
         if (e2 == hipSuccess) { ctx->allocs.push_back(*p); ctx->own_bytes += (int64_t)b; }
         return e2;
     };
-    size_t ovf = (size_t)ctx->ext_grid * ilm_k_ext_block() * ilm_k_ovf_stack() * sizeof(int);
-    HIPTRY(ctx, A((void**)&ctx->ovf, ovf));
+    HIPTRY(ctx, hipMalloc((void**)&ctx->ovf, ovf_bytes(ctx)));
+    ctx->own_bytes += (int64_t)ovf_bytes(ctx);
     HIPTRY(ctx, A((void**)&ctx->ctl, sizeof(DCtl)));
     HIPTRY(ctx, A((void**)&ctx->d_chunk, sizeof(DChunk)));
     HIPTRY(ctx, A((void**)&ctx->d_seg_total, 2 * sizeof(unsigned long long)));
@@ -464,6 +491,8 @@ static void ctx_destroy(ilm_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     for (auto& kv : ctx->graphs) (void)hipGraphExecDestroy(kv.second);
+    ctx_free_paths(ctx);
+    if (ctx->ovf) (void)hipFree(ctx->ovf);
     for (void* p : ctx->allocs) (void)hipFree(p);
     if (ctx->seg_count) (void)hipFree(ctx->seg_count);
     for (int i = 0; i < ILM_RING; i++) {
@@ -491,6 +520,51 @@ void ilm_close(ilm_ctx* ctx) {
         (void)hipStreamSynchronize(ctx->stream);
     }
     ctx_unref(ctx);
+}
+
+/* Retune an idle context: extend_mode, sample_group, tile_rows, chunk_passes and
+ * paths_in_flight take the values in *o (the other fields are ignored). Path buffers are
+ * reallocated when the chunk size changes; captured graphs are dropped. */
+int ilm_tune(ilm_ctx* ctx, const ilm_opts* o) {
+    if (!ctx || !o) return fail(ctx, ILM_E_INVALID, "null argument");
+    if (!opts_valid(*o)) return fail(ctx, ILM_E_INVALID, "bad ilm_opts field");
+    std::lock_guard<std::mutex> sched(ctx->sched_mu);
+    if (!ctx->pending.empty() || !ctx->active.empty()) return fail(ctx, ILM_E_BUSY, "requests in flight");
+    std::lock_guard<std::mutex> dev(ctx->dev_mu);
+    HIPTRY(ctx, hipSetDevice(ctx->device));
+    HIPTRY(ctx, hipStreamSynchronize(ctx->stream));
+    for (auto& kv : ctx->graphs) (void)hipGraphExecDestroy(kv.second);
+    ctx->graphs.clear();
+    if (o->extend_mode != ctx->opts.extend_mode) {
+        const int occ = ilm_k_extend_occupancy(ctx->opts.tri_test, ctx->opts.bvh_layout, o->extend_mode);
+        const int grid = ctx->n_cu * std::max(1, occ);
+        if (grid > ctx->ext_grid) {
+            const size_t old_b = ovf_bytes(ctx);
+            (void)hipFree(ctx->ovf);
+            ctx->ovf = nullptr;
+            ctx->own_bytes -= (int64_t)old_b;
+            ctx->ext_grid = grid;
+            HIPTRY(ctx, hipMalloc((void**)&ctx->ovf, ovf_bytes(ctx)));
+            ctx->own_bytes += (int64_t)ovf_bytes(ctx);
+        }
+        ctx->ext_grid = grid;
+        ctx->opts.extend_mode = o->extend_mode;
+    }
+    ctx->opts.sample_group = o->sample_group;
+    ctx->opts.tile_rows = o->tile_rows;
+    ctx->opts.chunk_passes = o->chunk_passes;
+    const int64_t cap = cap_of(*o);
+    if (cap != ctx->cap) {
+        ctx_free_paths(ctx);
+        int rc = ctx_alloc_paths(ctx, cap);
+        if (rc) {
+            ctx_free_paths(ctx);
+            if (ctx_alloc_paths(ctx, 4096) != ILM_OK) return fail(ctx, ILM_E_NOMEM, "path buffers lost after a failed resize");
+            return rc;
+        }
+    }
+    ctx->opts.paths_in_flight = o->paths_in_flight;
+    return ILM_OK;
 }
 
 const char* ilm_last_error(ilm_ctx* ctx) {
@@ -525,6 +599,7 @@ static KParams base_params(ilm_ctx* ctx) {
     P.bvh_layout = ctx->opts.bvh_layout;
     P.tri_test = ctx->opts.tri_test;
     P.extend_mode = ctx->opts.extend_mode;
+    P.tile_rows = std::max(1, ctx->opts.tile_rows);
     P.trav_stats = ctx->opts.count_tests ? ctx->d_trav : nullptr;
     return P;
 }
@@ -649,6 +724,7 @@ static void build_chunk(ilm_ctx* ctx, const std::vector<Job*>& jobs, DChunk& c, 
         w.pass0 = j->next_pass;
         if (j->next_px == 0 && wh <= left) {
             int64_t np = std::min<int64_t>(j->p.spp - j->next_pass, left / wh);
+            if (ctx->opts.chunk_passes > 0) np = std::min<int64_t>(np, ctx->opts.chunk_passes);
             w.npass = (int)np;
             w.px0 = 0;
             w.npx = (uint32_t)wh;
@@ -661,6 +737,9 @@ static void build_chunk(ilm_ctx* ctx, const std::vector<Job*>& jobs, DChunk& c, 
             j->next_px += npx;
             if (j->next_px >= wh) { j->next_px = 0; j->next_pass += 1; }
         }
+        w.sgroup = 1;
+        if (w.px0 == 0)
+            while (w.sgroup * 2 <= (uint32_t)std::max(1, ctx->opts.sample_group) && w.npass % (w.sgroup * 2) == 0) w.sgroup *= 2;
         w.pid_base = c.total_paths;
         w.acc_base = c.total_px;
         c.total_paths += (uint32_t)((int64_t)w.npass * w.npx);

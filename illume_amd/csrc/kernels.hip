@@ -59,13 +59,14 @@ __global__ __launch_bounds__(256) void k_raygen(KParams P) {
             pass = gp(P.list)[2 * i + 1];
         } else {
             const GLOBAL DWork* W = C->work + find_work_pid(C->work, C->n_work, i);
-            const uint32_t local = i - W->pid_base;
-            const uint32_t npx = W->npx;
+            uint32_t k, q;
+            slice_decode(i - W->pid_base, W->npx, W->sgroup, k, q);
             req = W->req;
-            pass = (uint32_t)W->pass0 + local / npx;
-            pixel = W->px0 + local % npx;
+            pass = (uint32_t)W->pass0 + k;
+            pixel = W->px0 + q;
         }
         const GLOBAL DReq* R = C->reqs + req;
+        if (!P.list) pixel = tile_pixel(pixel, (uint32_t)R->width, (uint32_t)R->height, P.tile_rows);
         const GLOBAL DScene* S = gp(R->scene);
         uint64_t rs = ilm_path_seed(pass, pixel);
         const uint64_t inc = ilm_pcg32_inc(pixel);
@@ -435,14 +436,14 @@ __global__ __launch_bounds__(256) void k_accumulate(KParams P) {
         const GLOBAL DWork* W = C->work + lo;
         const GLOBAL DReq* R = C->reqs + W->req;
         const uint32_t q = g - W->acc_base;
-        const uint32_t p = W->px0 + q;
+        const uint32_t p = tile_pixel(W->px0 + q, (uint32_t)R->width, (uint32_t)R->height, P.tile_rows);
         GLOBAL float4* acc = gpw(R->accum);
         GLOBAL float4* cmp = gpw(R->comp);
         float4 s = acc[p], c = cmp[p];
         const int npass = W->npass;
-        const uint32_t npx = W->npx, pid0 = W->pid_base;
+        const uint32_t npx = W->npx, pid0 = W->pid_base, sg = W->sgroup;
         for (int k = 0; k < npass; k++) {
-            float4 x = contrib[pid0 + (uint32_t)k * npx + q];
+            float4 x = contrib[pid0 + slice_local((uint32_t)k, q, npx, sg)];
             float y, t;
             y = x.x - c.x; t = s.x + y; c.x = (t - s.x) - y; s.x = t;
             y = x.y - c.y; t = s.y + y; c.y = (t - s.y) - y; s.y = t;

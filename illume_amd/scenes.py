@@ -254,3 +254,36 @@ def tiny_mesh_scene(n=4, seed=7):
 
 
 CONFIGS = {0: cornell, 1: gold_spheres, 2: subsurface}
+
+# Scenes of the request mixes: 0-2 are the configurations above, 3 the instanced mesh.
+MIX_SCENES = {0: cornell, 1: gold_spheres, 2: subsurface, 3: instanced}
+
+
+def batch_requests(seed=3, n=64):
+    """Batched workload: n requests, scene uniform over {0,1,2}, square side uniform over
+    {128,256,512,1024}, spp uniform in 4..256; depth and tonemap from the scene."""
+    rng = np.random.RandomState(seed)
+    out = []
+    for _ in range(n):
+        sc = int(rng.randint(3))
+        side = [128, 256, 512, 1024][rng.randint(4)]
+        out.append(dict(scene=sc, width=side, height=side, spp=int(rng.randint(4, 257))))
+    return out
+
+
+def serve_trace(seed=4, rate=20.0, duration=60.0):
+    """Served workload: Poisson arrivals (rate per second) over duration seconds; scene uniform
+    over MIX_SCENES, resolution uniform over {256^2, 512^2, 1280x720, 1920x1080}, spp uniform in
+    8..512, depth uniform in 8..32. Arrival times t are seconds from the start."""
+    rng = np.random.RandomState(seed)
+    res = [(256, 256), (512, 512), (1280, 720), (1920, 1080)]
+    out = []
+    t = 0.0
+    while True:
+        t += float(rng.exponential(1.0 / rate))
+        if t > duration:
+            break
+        w, h = res[rng.randint(4)]
+        out.append(dict(t=t, scene=int(rng.randint(4)), width=w, height=h, spp=int(rng.randint(8, 513)),
+                        max_depth=int(rng.randint(8, 33))))
+    return out

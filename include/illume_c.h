@@ -186,6 +186,15 @@ typedef struct ilm_opts {
     int32_t profile;            /* time every extend launch with HIP events (disables graphs) */
     int32_t count_tests;        /* count box/triangle tests of the extend kernel (slower) */
     int32_t verbose;
+    /* Path order inside a chunk (results are identical for every choice; only coherence
+     * changes): sample_group consecutive path ids take consecutive samples of one pixel (a
+     * power of two, 0 or 1 = pixel-major), tile_rows > 1 walks the image in bands of that many
+     * rows, column by column, so a wave covers a compact block of pixels. */
+    int32_t sample_group;
+    int32_t tile_rows;
+    /* at most this many passes of one request per chunk (0 = as many as paths_in_flight
+     * holds); 1 gives the reference's loop shape, one pass after another (src/renderer.cu:531) */
+    int32_t chunk_passes;
 } ilm_opts;
 
 typedef struct ilm_render_info {
@@ -241,6 +250,11 @@ int ilm_open(int device, const ilm_opts* opts, ilm_ctx** out);
 void ilm_close(ilm_ctx* ctx);
 const char* ilm_last_error(ilm_ctx* ctx);
 int ilm_stats_get(ilm_ctx* ctx, ilm_stats* out);
+/* Retune an idle context (no counterpart in the reference, whose launch shape is fixed):
+ * extend_mode, sample_group, tile_rows, chunk_passes and paths_in_flight take the values in
+ * *opts; other fields are ignored. ILM_E_BUSY while served requests are queued or running.
+ * The autotuner (illume_amd/autotune.py) drives this between short trial renders. */
+int ilm_tune(ilm_ctx* ctx, const ilm_opts* opts);
 
 /* ---- scenes ---- */
 int ilm_scene_upload(ilm_ctx* ctx, const ilm_scene_desc* desc, ilm_scene** out);

@@ -378,6 +378,42 @@ static void ref_flatten(const std::vector<BTree>& tree, int node, RefBVH& out, i
     }
 }
 
+/* Median-split builder (not the reference's): longest centroid axis, split at the median,
+ * leaves of at most tris_per_node triangles. O(N log N), depth about log2(N / leaf) so the
+ * 32-entry traversal stack never overflows. Closest hits do not depend on the tree except for
+ * ties, so the FP64 precision oracle may use it on large meshes where the reference sweep
+ * builder takes minutes; the CPU baseline and the FP32 restatement keep the reference builder. */
+static int median_build(std::vector<BTree>& tree, std::vector<BWork*>& work, size_t b, size_t e, int tris_per_node) {
+    int me = (int)tree.size();
+    tree.emplace_back();
+    AABB<float> box = aabb_empty<float>(), cbox = aabb_empty<float>();
+    for (size_t i = b; i < e; i++) {
+        aabb_update(&box, work[i]->bounds.lo);
+        aabb_update(&box, work[i]->bounds.hi);
+        aabb_update(&cbox, work[i]->center);
+    }
+    tree[me].bounds = box;
+    if (e - b <= (size_t)std::max(1, tris_per_node)) {
+        tree[me].leaf = true;
+        for (size_t i = b; i < e; i++) tree[me].tris.push_back(work[i]->tri);
+        return me;
+    }
+    V3<float> ext = vsub(cbox.hi, cbox.lo);
+    int axis = ext.x >= ext.y && ext.x >= ext.z ? 0 : (ext.y >= ext.z ? 1 : 2);
+    size_t mid = b + (e - b) / 2;
+    std::nth_element(work.begin() + b, work.begin() + mid, work.begin() + e, [axis](const BWork* x, const BWork* y) {
+        float cx = axis_of(x->center, axis), cy = axis_of(y->center, axis);
+        return cx < cy || (cx == cy && x->tri < y->tri);
+    });
+    int l = median_build(tree, work, b, mid, tris_per_node);
+    int r = median_build(tree, work, mid, e, tris_per_node);
+    tree[me].left = l;
+    tree[me].right = r;
+    return me;
+}
+
+static bool g_median_bvh = false;
+
 static RefBVH ref_bvh_build(const std::vector<AABB<float>>& aabbs, AABB<float> bounds, int tris_per_node) {
     std::vector<BWork> work(aabbs.size());
     std::vector<BWork*> list(aabbs.size());
@@ -389,7 +425,8 @@ static RefBVH ref_bvh_build(const std::vector<AABB<float>>& aabbs, AABB<float> b
     }
     std::vector<BTree> tree;
     tree.reserve(aabbs.size() / 2 + 16);
-    int root = ref_build(tree, list, 0, tris_per_node);
+    int root = g_median_bvh ? median_build(tree, list, 0, list.size(), tris_per_node)
+                            : ref_build(tree, list, 0, tris_per_node);
     tree[root].bounds = bounds;
     RefBVH out;
     out.nodes.resize(tree.size());
@@ -1101,17 +1138,24 @@ int orc_threads_available(void) {
 #endif
 }
 
-/* fp64: 0 = FP32 faithful only, 1 = also build the FP64 twin (same reference BVH) */
-orc_scene* orc_scene_create(const ilm_scene_desc* desc, int fp64, int threads) {
+/* fp64: 0 = FP32 faithful only, 1 = also build the FP64 twin (same BVH);
+ * median_bvh: 1 = the median-split builder instead of the reference sweep builder */
+orc_scene* orc_scene_create2(const ilm_scene_desc* desc, int fp64, int threads, int median_bvh) {
     set_threads(threads);
     orc_scene* s = new orc_scene();
+    g_median_bvh = median_bvh != 0;
     s->f = scene_build<float>(desc, nullptr);
+    g_median_bvh = false;
     if (fp64) {
         std::vector<RefBVH> bvhs;
         for (auto& m : s->f->meshes) bvhs.push_back(m.bvh);
         s->d = scene_build<double>(desc, &bvhs);
     }
     return s;
+}
+
+orc_scene* orc_scene_create(const ilm_scene_desc* desc, int fp64, int threads) {
+    return orc_scene_create2(desc, fp64, threads, 0);
 }
 
 void orc_scene_free(orc_scene* s) {

@@ -3,6 +3,8 @@ import ctypes
 import os
 import re
 
+import pytest
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
@@ -53,3 +55,17 @@ def test_open_fails_cleanly_without_device():
     rc = L.ilm_open(0, None, ctypes.byref(h))
     assert rc == -3   # ILM_E_DEVICE
     assert b"device" in L.ilm_global_error()
+
+
+@pytest.mark.parametrize("field,value", [("sample_group", 3), ("sample_group", -1), ("tile_rows", -2),
+                                         ("extend_mode", 2), ("bvh_layout", 7), ("chunk_passes", -1)])
+def test_open_rejects_bad_options(field, value):
+    """Option checks run before any device call, so they hold with or without a GPU."""
+    import illume_amd
+    from illume_amd._abi import Opts
+    L = illume_amd.lib()
+    o = Opts()
+    setattr(o, field, value)
+    h = ctypes.c_void_p()
+    assert L.ilm_open(0, ctypes.byref(o), ctypes.byref(h)) == -1   # ILM_E_INVALID
+    assert not h.value

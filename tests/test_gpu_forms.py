@@ -48,6 +48,25 @@ def test_graph_replays_match_direct_launches(scenes):
         assert i1["segments"] == i0["segments"] and i1["chunks"] > 1
 
 
+@pytest.mark.parametrize("order", [dict(sample_group=4, tile_rows=8), dict(sample_group=64, tile_rows=4),
+                                   dict(sample_group=2, tile_rows=3, paths_in_flight=3000)])
+def test_path_order_leaves_results_unchanged(ctx, scenes, order):
+    """Samples grouped per pixel and tiled pixel walks only reorder path ids: every pixel still
+    receives the same samples, summed in pass order, so the images are bit-identical."""
+    with illume_amd.Context(0, **order) as c:
+        for name, kw in [("tiny", dict(width=33, height=17, spp=6)), ("cornell", dict(width=64, height=48, spp=8)),
+                         ("gold", dict(width=40, height=40, spp=3, max_depth=5))]:
+            spec, st, s_ref = scenes[name]
+            p = _p(st, **kw)
+            ref, px_ref, i_ref = ctx.render(s_ref, p)
+            s = c.scene(spec)
+            got, px, i = c.render(s, p)
+            s.free()
+            assert np.array_equal(got, ref), (name, order)
+            assert np.array_equal(px, px_ref)
+            assert i["segments"] == i_ref["segments"]
+
+
 def test_batch_equals_individual_renders(ctx, scenes):
     ss = [scenes[n][2] for n, _ in MIX]
     ps = [_p(scenes[n][1], **kw) for n, kw in MIX]
