@@ -97,7 +97,7 @@ struct DInst {
 struct DMesh {
     const BNode* nodes;
     const QNode* qnodes;
-    const float4* tris;     /* 3 per leaf slot: (v0, orig idx bits), (e1, 0), (e2, 0) */
+    const float4* tris;     /* 3 per leaf slot in leaf order: (v0, original index bits), (v1, 0), (v2, 0) */
     const int* idx;         /* original triangle vertex indices, 3 per triangle */
     const float* pos;       /* 3 per vertex */
     const float* nrm;       /* 3 per vertex, or null */
@@ -149,8 +149,9 @@ struct DReq {
     float4* comp;           /* width*height Kahan compensations */
 };
 
-/* One slice of work in a chunk: passes [pass0, pass0+npass) of pixels [px0, px0+npx) of
- * request req. Paths are laid out pass-major: pid = pid_base + s*npx + (pixel - px0). */
+/* One slice of work in a chunk: passes [pass0, pass0+npass) of pixel slots [px0, px0+npx) of
+ * request req. Sample s of slot px0+q has pid = pid_base + slice_local(s, q, npx, sgroup); the
+ * slot's pixel is tile_pixel(px0+q, width, height, tile_rows). */
 struct DWork {
     int req;
     int pass0;
