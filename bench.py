@@ -441,6 +441,20 @@ def bench_served(args):
     print(json.dumps(out), flush=True)
 
 
+def calibrate_copy():
+    """A plain device copy of known size (1 GiB read + 1 GiB written, three times) for scaling
+    the FETCH_SIZE / WRITE_SIZE counters of rocprofv3 --pmc to bytes."""
+    import torch
+    n = 1 << 28
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read_per_launch": 4 * n, "copy_bytes_written_per_launch": 4 * n, "launches": 3}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -459,7 +473,10 @@ def main():
     ap.add_argument("--no-rmse", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-count", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true", help="only run the 1 GiB copy used to scale HBM counters")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        return calibrate_copy()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (the path shards by request or sample range; "
               "multi-GPU runs are not part of it). Use --gpus 1.", file=sys.stderr)

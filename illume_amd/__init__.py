@@ -105,6 +105,9 @@ class Context:
         o = Opts()
         o.use_graphs = 1
         for k, v in opts.items():
+            if k == "bvh_cache_dir":
+                setattr(o, k, None if v is None else os.fsencode(v))
+                continue
             if k == "bvh_layout" and isinstance(v, str):
                 v = BVH[v]
             if k == "tri_test" and isinstance(v, str):
@@ -274,6 +277,34 @@ class Request:
             self.free()
         except Exception:
             pass
+
+
+def bvh_device_layouts(positions, indices, cache_dir=None):
+    """The device BVH layouts ilm_scene_upload uses for a mesh, through the on-disk cache in
+    cache_dir. Returns (nodes bytes, qnodes bytes, leaf order, (root, qroot, depth, qdepth),
+    from_cache)."""
+    L = lib()
+    f = L.ilm_bvh_device_layouts
+    f.restype = C.c_int
+    f.argtypes = [C.POINTER(MeshDesc), C.c_char_p, C.POINTER(C.c_int32), C.c_void_p, C.POINTER(C.c_int64),
+                  C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
+    pos = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1, 3)
+    idx = np.ascontiguousarray(indices, dtype=np.int32).reshape(-1, 3)
+    m = MeshDesc()
+    m.num_vertices, m.positions = pos.shape[0], _fp(pos)
+    m.num_triangles, m.indices = idx.shape[0], _ip(idx)
+    m.face_normals = 1
+    d = os.fsencode(cache_dir) if cache_dir else None
+    nb, qb = C.c_int64(0), C.c_int64(0)
+    hit = C.c_int32(0)
+    _check(f(C.byref(m), None, None, None, C.byref(nb), None, C.byref(qb), None, None))   # sizes
+    nodes = np.zeros(nb.value, np.uint8)
+    qnodes = np.zeros(qb.value, np.uint8)
+    order = np.zeros(idx.shape[0], np.int32)
+    info = np.zeros(4, np.int32)
+    _check(f(C.byref(m), d, C.byref(hit), nodes.ctypes.data, C.byref(nb), qnodes.ctypes.data, C.byref(qb),
+             _ip(order), _ip(info)))
+    return nodes, qnodes, order, tuple(int(x) for x in info), bool(hit.value)
 
 
 def bvh_build(positions, indices):

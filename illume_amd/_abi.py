@@ -94,7 +94,8 @@ class Opts(C.Structure):
     _fields_ = [("bvh_layout", C.c_int32), ("tri_test", C.c_int32), ("paths_in_flight", C.c_int64),
                 ("hbm_budget_bytes", C.c_int64), ("use_graphs", C.c_int32), ("max_queued", C.c_int32),
                 ("extend_mode", C.c_int32), ("profile", C.c_int32), ("count_tests", C.c_int32), ("verbose", C.c_int32),
-                ("sample_group", C.c_int32), ("tile_rows", C.c_int32), ("chunk_passes", C.c_int32)]
+                ("sample_group", C.c_int32), ("tile_rows", C.c_int32), ("chunk_passes", C.c_int32),
+                ("bvh_cache_dir", C.c_char_p)]
 
 
 class RenderInfo(C.Structure):
@@ -106,7 +107,8 @@ class RenderInfo(C.Structure):
 class Stats(C.Structure):
     _fields_ = [("segments", C.c_uint64), ("requests_done", C.c_uint64), ("requests_rejected", C.c_uint64),
                 ("requests_cancelled", C.c_uint64), ("busy_ms", C.c_double), ("p50_ms", C.c_double),
-                ("p99_ms", C.c_double), ("hbm_budget_bytes", C.c_int64), ("hbm_in_use_bytes", C.c_int64)]
+                ("p99_ms", C.c_double), ("hbm_budget_bytes", C.c_int64), ("hbm_in_use_bytes", C.c_int64),
+                ("bvh_cache_hits", C.c_uint64), ("bvh_cache_misses", C.c_uint64)]
 
 
 class ReqStatus(C.Structure):

@@ -28,7 +28,7 @@ HIP_FLAGS = [
     "-fno-gpu-flush-denormals-to-zero", "-fhip-fp32-correctly-rounded-divide-sqrt",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall", "-Wno-unused-function",
 ]
-SOURCES = ["kernels.hip", "engine.hip", "bvh.cpp", "bvh_capi.cpp", "loader.cpp"]
+SOURCES = ["kernels.hip", "engine.hip", "bvh.cpp", "bvh_cache.cpp", "bvh_capi.cpp", "loader.cpp"]
 ORACLE_FLAGS = ["-std=c++17", "-O3", "-march=x86-64-v3", "-ffp-contract=off", "-fno-fast-math",
                 "-fopenmp", "-fPIC", "-shared", "-Wall", "-Wno-unused-function"]
 

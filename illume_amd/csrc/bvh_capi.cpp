@@ -7,6 +7,7 @@
 #include <string>
 
 #include "bvh.h"
+#include "bvh_cache.h"
 
 using namespace ilm;
 
@@ -88,3 +89,40 @@ extern "C" int ilm_bvh_export_quantized(const ilm_bvh* b, float* qboxes, float* 
 }
 
 extern "C" void ilm_bvh_free(ilm_bvh* b) { delete b; }
+
+/* The device layouts ilm_scene_upload would use for this mesh, built or loaded from cache_dir
+ * (NULL: no cache). *nodes_bytes / *qnodes_bytes hold the buffers' capacities on entry and the
+ * layout sizes on return (ILM_E_INVALID when a buffer is too small; NULL buffers only query).
+ * order: num_triangles ints. info4: root, qroot, depth, qdepth. */
+extern "C" int ilm_bvh_device_layouts(const ilm_mesh_desc* m, const char* cache_dir, int32_t* from_cache, void* nodes,
+                                      int64_t* nodes_bytes, void* qnodes, int64_t* qnodes_bytes, int32_t* order,
+                                      int32_t* info4) {
+    if (!m || !nodes_bytes || !qnodes_bytes || m->num_triangles < 0 ||
+        (m->num_triangles > 0 && (!m->positions || !m->indices)))
+        return ILM_E_INVALID;
+    for (int64_t k = 0; k < 3LL * m->num_triangles; k++)
+        if (m->indices[k] < 0 || m->indices[k] >= m->num_vertices) return ILM_E_INVALID;
+    BuildOpts o;
+    MeshBVH b;
+    bool hit = false;
+    uint64_t key = 0;
+    if (cache_dir) {
+        key = mesh_key(m->positions, m->num_vertices, m->indices, m->num_triangles, o);
+        hit = cache_load(cache_dir, key, m->num_triangles, b);
+    }
+    if (!hit) {
+        b = build_mesh_bvh(m->positions, m->indices, m->num_triangles, o);
+        if (cache_dir) (void)cache_store(cache_dir, key, b);
+    }
+    if (from_cache) *from_cache = hit ? 1 : 0;
+    const int64_t nb = (int64_t)(b.nodes.size() * sizeof(BNode)), qb = (int64_t)(b.qnodes.size() * sizeof(QNode));
+    const bool fits = (!nodes || *nodes_bytes >= nb) && (!qnodes || *qnodes_bytes >= qb);
+    *nodes_bytes = nb;
+    *qnodes_bytes = qb;
+    if (!fits) return ILM_E_INVALID;
+    if (nodes) std::memcpy(nodes, b.nodes.data(), nb);
+    if (qnodes) std::memcpy(qnodes, b.qnodes.data(), qb);
+    if (order) std::memcpy(order, b.order.data(), b.order.size() * sizeof(int32_t));
+    if (info4) { info4[0] = b.root; info4[1] = b.qroot; info4[2] = b.depth; info4[3] = b.qdepth; }
+    return ILM_OK;
+}

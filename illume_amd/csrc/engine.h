@@ -114,8 +114,13 @@ struct ilm_ctx {
     int64_t chunks_retired = 0;
     std::vector<double> latencies;
     uint64_t stat_segments = 0, stat_done = 0, stat_rejected = 0, stat_cancelled = 0;
+    uint64_t stat_bvh_hits = 0, stat_bvh_misses = 0;
+    std::string bvh_cache_dir;      /* empty: no BVH cache */
     double stat_busy_ms = 0.0;
 
+    /* Lock order: sched_mu before dev_mu; stat_mu is a leaf (no other lock is taken while it
+     * is held). Nothing takes sched_mu while holding dev_mu. */
+    std::mutex stat_mu;             /* stat_busy_ms, stat_bvh_* */
     std::mutex err_mu;
     std::string err, err_copy;
 };

@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "bvh.h"
+#include "bvh_cache.h"
 #include "engine.h"
 #include "hostmath.h"
 #include "kernels.h"
@@ -157,23 +158,30 @@ int ilm_scene_upload(ilm_ctx* ctx, const ilm_scene_desc* d, ilm_scene** out) {
     BuildOpts bo;
     for (int i = 0; i < d->num_meshes; i++) {
         const ilm_mesh_desc& m = d->meshes[i];
-        std::vector<Box> pb(m.num_triangles);
         Box mb;
         mb.reset();
-        for (int t = 0; t < m.num_triangles; t++) {
-            pb[t].reset();
-            for (int j = 0; j < 3; j++) pb[t].grow(&m.positions[3 * m.indices[3 * t + j]]);
-            mb.grow(pb[t]);
-        }
+        for (int64_t k = 0; k < 3LL * m.num_triangles; k++) mb.grow(&m.positions[3 * m.indices[k]]);
         mesh_bounds[i] = mb;
-        Tree tree = build_tree(pb, bo);
-        Layout2 L2 = make_layout2(tree);
-        LayoutQ LQ = make_layout_q(tree);
-        blas_depth = std::max(blas_depth, tree.max_depth);
-        blas_qdepth = std::max(blas_qdepth, LQ.max_depth);
+        MeshBVH bvh;
+        bool cached = false;
+        uint64_t key = 0;
+        if (!ctx->bvh_cache_dir.empty()) {
+            key = mesh_key(m.positions, m.num_vertices, m.indices, m.num_triangles, bo);
+            cached = cache_load(ctx->bvh_cache_dir, key, m.num_triangles, bvh);
+        }
+        if (!cached) {
+            bvh = build_mesh_bvh(m.positions, m.indices, m.num_triangles, bo);
+            if (!ctx->bvh_cache_dir.empty()) (void)cache_store(ctx->bvh_cache_dir, key, bvh);
+        }
+        if (!ctx->bvh_cache_dir.empty()) {
+            std::lock_guard<std::mutex> lk(ctx->stat_mu);
+            (cached ? ctx->stat_bvh_hits : ctx->stat_bvh_misses)++;
+        }
+        blas_depth = std::max(blas_depth, bvh.depth);
+        blas_qdepth = std::max(blas_qdepth, bvh.qdepth);
         std::vector<float4> tris(3 * (size_t)m.num_triangles);
         for (int k = 0; k < m.num_triangles; k++) {
-            int t = tree.order[k];
+            int t = bvh.order[k];
             const float* p0 = &m.positions[3 * m.indices[3 * t]];
             const float* p1 = &m.positions[3 * m.indices[3 * t + 1]];
             const float* p2 = &m.positions[3 * m.indices[3 * t + 2]];
@@ -187,8 +195,8 @@ int ilm_scene_upload(ilm_ctx* ctx, const ilm_scene_desc* d, ilm_scene** out) {
         DMesh& dm = dmeshes[i];
         std::memset(&dm, 0, sizeof(dm));
         BNode* bn; QNode* qn; float4* tp; int* ip; float* pp; float* np; float* up;
-        HIPTRY(ctx, dupload(s.get(), &bn, L2.nodes.data(), L2.nodes.size()));
-        HIPTRY(ctx, dupload(s.get(), &qn, LQ.nodes.data(), LQ.nodes.size()));
+        HIPTRY(ctx, dupload(s.get(), &bn, bvh.nodes.data(), bvh.nodes.size()));
+        HIPTRY(ctx, dupload(s.get(), &qn, bvh.qnodes.data(), bvh.qnodes.size()));
         HIPTRY(ctx, dupload(s.get(), &tp, tris.data(), tris.size()));
         HIPTRY(ctx, dupload(s.get(), &ip, m.indices, 3 * (size_t)m.num_triangles));
         HIPTRY(ctx, dupload(s.get(), &pp, m.positions, 3 * (size_t)m.num_vertices));
@@ -197,8 +205,8 @@ int ilm_scene_upload(ilm_ctx* ctx, const ilm_scene_desc* d, ilm_scene** out) {
         if (!m.face_normals) HIPTRY(ctx, dupload(s.get(), &np, m.normals, 3 * (size_t)m.num_vertices));
         if (m.has_texcoords) HIPTRY(ctx, dupload(s.get(), &up, m.texcoords, 2 * (size_t)m.num_vertices));
         dm.nodes = bn; dm.qnodes = qn; dm.tris = tp; dm.idx = ip; dm.pos = pp; dm.nrm = np; dm.uv = up;
-        dm.root = L2.root;
-        dm.qroot = LQ.root;
+        dm.root = bvh.root;
+        dm.qroot = bvh.qroot;
         dm.face_normals = m.face_normals != 0;
         dm.has_texcoords = m.has_texcoords != 0;
         s->mesh_tris += m.num_triangles;
@@ -449,6 +457,8 @@ int ilm_open(int device, const ilm_opts* opts, ilm_ctx** out) {
     std::memset(&ctx->opts, 0, sizeof(ctx->opts));
     ctx->opts.use_graphs = 1;
     if (opts) ctx->opts = *opts;
+    if (ctx->opts.bvh_cache_dir) ctx->bvh_cache_dir = ctx->opts.bvh_cache_dir;
+    ctx->opts.bvh_cache_dir = nullptr;   /* the caller's string is not kept */
     HIPTRY(ctx, hipSetDevice(device));
     hipDeviceProp_t prop;
     HIPTRY(ctx, hipGetDeviceProperties(&prop, device));
@@ -833,7 +843,7 @@ static int run_jobs_sync(ilm_ctx* ctx, std::vector<Job*>& jobs, ilm_render_info*
         info->tri_tests = ctx->h_u64[2];
     }
     {
-        std::lock_guard<std::mutex> lk(ctx->sched_mu);
+        std::lock_guard<std::mutex> lk(ctx->stat_mu);   /* dev_mu is held: never sched_mu here */
         ctx->stat_busy_ms += ms;
     }
     return ILM_OK;
@@ -1030,7 +1040,8 @@ int ilm_debug_path_radiance(ilm_ctx* ctx, ilm_scene* scene, const ilm_render_par
 
 /* ======================= served form ======================= */
 
-static int admit_locked(ilm_ctx* ctx);
+static void admit_select_locked(ilm_ctx* ctx, std::vector<Job*>& picked);
+static void admit_prepare(ilm_ctx* ctx, std::vector<Job*>& picked);
 
 static void server_loop(ilm_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
@@ -1039,14 +1050,19 @@ static void server_loop(ilm_ctx* ctx) {
     struct Inflight { hipEvent_t ev; std::vector<Job*> finishing; };
     std::deque<Inflight> inflight;
     for (;;) {
-        std::vector<Job*> active;
+        std::vector<Job*> active, picked;
         {
             std::unique_lock<std::mutex> lk(ctx->sched_mu);
             ctx->sched_cv.wait(lk, [&] {
                 return ctx->stop || !ctx->pending.empty() || !ctx->active.empty() || !inflight.empty();
             });
             if (ctx->stop && ctx->pending.empty() && ctx->active.empty() && inflight.empty()) break;
-            admit_locked(ctx);
+            admit_select_locked(ctx, picked);
+        }
+        /* device buffers of newly admitted jobs: dev_mu only, never inside sched_mu */
+        admit_prepare(ctx, picked);
+        {
+            std::lock_guard<std::mutex> lk(ctx->sched_mu);
             active = ctx->active;
         }
         /* retire cancelled jobs that have nothing in flight */
@@ -1131,9 +1147,10 @@ static void server_loop(ilm_ctx* ctx) {
     }
 }
 
-/* Admit queued jobs while the HBM budget allows. Caller holds sched_mu. */
-static int admit_locked(ilm_ctx* ctx) {
-    while (!ctx->pending.empty() && (int)ctx->active.size() < ILM_MAX_JOBS) {
+/* Take queued jobs out of the queue while the HBM budget allows, reserving their bytes; the
+ * caller holds sched_mu. Cancelled queued jobs end here. */
+static void admit_select_locked(ilm_ctx* ctx, std::vector<Job*>& picked) {
+    while (!ctx->pending.empty() && (int)(ctx->active.size() + picked.size()) < ILM_MAX_JOBS) {
         Job* j = ctx->pending.front();
         if (j->cancel.load()) {
             ctx->pending.pop_front();
@@ -1144,24 +1161,38 @@ static int admit_locked(ilm_ctx* ctx) {
             if (--j->refs == 0) { scene_release(j->scene, true); delete j; }
             continue;
         }
-        if (ctx->in_use + j->bytes > ctx->budget && !ctx->active.empty()) break;
+        if (ctx->in_use + j->bytes > ctx->budget && !(ctx->active.empty() && picked.empty())) break;
         ctx->pending.pop_front();
+        ctx->in_use += j->bytes;
+        picked.push_back(j);
+    }
+}
+
+/* Allocate and clear the picked jobs' device buffers (under dev_mu), then make them active
+ * (under sched_mu); the two locks are never held together. */
+static void admit_prepare(ilm_ctx* ctx, std::vector<Job*>& picked) {
+    for (Job* j : picked) {
+        int rc;
         {
             std::lock_guard<std::mutex> dev(ctx->dev_mu);
-            if (job_prepare(ctx, j) != ILM_OK) {
-                job_release_device(j);
-                j->state.store(ILM_REQ_FAILED);
-                j->cv.notify_all();
-                if (--j->refs == 0) { scene_release(j->scene, true); delete j; }
-                continue;
-            }
+            rc = job_prepare(ctx, j);
+            if (rc != ILM_OK) job_release_device(j);
         }
-        ctx->in_use += j->bytes;
+        std::lock_guard<std::mutex> lk(ctx->sched_mu);
+        if (rc != ILM_OK) {
+            ctx->in_use -= j->bytes;
+            j->end_ms = now_ms(ctx);
+            j->state.store(ILM_REQ_FAILED);
+            j->cv.notify_all();
+            if (--j->refs == 0) { scene_release(j->scene, true); delete j; }
+            continue;
+        }
+        if (ctx->stop) j->cancel.store(true);
         j->start_ms = now_ms(ctx);
         j->state.store(ILM_REQ_RUNNING);
         ctx->active.push_back(j);
     }
-    return ILM_OK;
+    picked.clear();
 }
 
 static void server_start(ilm_ctx* ctx) {
@@ -1309,7 +1340,12 @@ int ilm_stats_get(ilm_ctx* ctx, ilm_stats* st) {
     st->requests_done = ctx->stat_done;
     st->requests_rejected = ctx->stat_rejected;
     st->requests_cancelled = ctx->stat_cancelled;
-    st->busy_ms = ctx->stat_busy_ms;
+    {
+        std::lock_guard<std::mutex> sl(ctx->stat_mu);
+        st->busy_ms = ctx->stat_busy_ms;
+        st->bvh_cache_hits = ctx->stat_bvh_hits;
+        st->bvh_cache_misses = ctx->stat_bvh_misses;
+    }
     std::vector<double> l = ctx->latencies;
     if (!l.empty()) {
         std::sort(l.begin(), l.end());

@@ -195,6 +195,10 @@ typedef struct ilm_opts {
     /* at most this many passes of one request per chunk (0 = as many as paths_in_flight
      * holds); 1 gives the reference's loop shape, one pass after another (src/renderer.cu:531) */
     int32_t chunk_passes;
+    /* directory of the per-mesh BVH cache (versioned, hashed files "<key>.ilmbvh" holding both
+     * device layouts; replaces the reference's unversioned .bvh files, src/accel/bvh.cpp:399-439),
+     * or NULL for none. The string is copied at ilm_open. */
+    const char* bvh_cache_dir;
 } ilm_opts;
 
 typedef struct ilm_render_info {
@@ -217,6 +221,8 @@ typedef struct ilm_stats {
     double p50_ms, p99_ms;      /* request latency: completion - submission */
     int64_t hbm_budget_bytes;
     int64_t hbm_in_use_bytes;
+    uint64_t bvh_cache_hits;    /* meshes whose BVH came from the cache */
+    uint64_t bvh_cache_misses;  /* meshes built (and stored) with a cache directory set */
 } ilm_stats;
 
 enum { ILM_REQ_QUEUED = 0, ILM_REQ_RUNNING = 1, ILM_REQ_DONE = 2, ILM_REQ_CANCELLED = 3, ILM_REQ_FAILED = 4 };
@@ -301,6 +307,14 @@ int ilm_bvh_export(const ilm_bvh* b, float* boxes, int32_t* children, int32_t* l
 int ilm_bvh_quantized_size(const ilm_bvh* b, int32_t* num_qnodes, int32_t* root, int32_t* max_depth);
 int ilm_bvh_export_quantized(const ilm_bvh* b, float* qboxes, float* exact, int32_t* child);
 void ilm_bvh_free(ilm_bvh* b);
+/* Device layouts (FP32 BVH2 nodes, 8-bit BVH4 nodes, leaf triangle order) exactly as
+ * ilm_scene_upload builds them, through the BVH cache in cache_dir (NULL: none). Replaces the
+ * reference's .bvh cache read/write (src/accel/bvh.cpp:399-439, src/primitives/mesh.cu:407-414).
+ * *nodes_bytes / *qnodes_bytes: capacity in, size out; NULL buffers only query.
+ * info4: root, qroot, depth, qdepth. *from_cache = 1 when the cache supplied them. */
+int ilm_bvh_device_layouts(const ilm_mesh_desc* m, const char* cache_dir, int32_t* from_cache, void* nodes,
+                           int64_t* nodes_bytes, void* qnodes, int64_t* qnodes_bytes, int32_t* order,
+                           int32_t* info4);
 
 /* ---- parity hooks ---- */
 /* rays: n*6 floats (o.xyz, d.xyz) in world space. inst: instance index, or -2-s for sphere s,

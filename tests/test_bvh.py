@@ -100,3 +100,37 @@ def test_reference_traversal_equals_brute_force():
     exact, tie, bad = compare_hits(bvh, brute)
     print("reference traversal vs brute force: %d same, %d ties, %d other" % (exact, tie, len(bad)))
     assert len(bad) == 0
+
+
+def test_bvh_cache_round_trip_and_rejection(tmp_path):
+    """A cached entry gives the same device layouts as a fresh build; a corrupted or foreign
+    entry is rejected and rebuilt; a different mesh gets its own entry."""
+    from illume_amd import bvh_device_layouts
+    m = S.displaced_icosphere(12, 3)
+    fresh = bvh_device_layouts(m["positions"], m["indices"])
+    assert not fresh[4]
+    first = bvh_device_layouts(m["positions"], m["indices"], cache_dir=str(tmp_path))
+    assert not first[4]                     # miss: built and stored
+    files = list(tmp_path.glob("*.ilmbvh"))
+    assert len(files) == 1
+    second = bvh_device_layouts(m["positions"], m["indices"], cache_dir=str(tmp_path))
+    assert second[4]                        # hit
+    for a, b in zip(fresh[:3], second[:3]):
+        assert np.array_equal(a, b)
+    assert fresh[3] == second[3]
+    # flip one payload byte: the payload hash rejects the entry, which is rebuilt and rewritten
+    raw = bytearray(files[0].read_bytes())
+    raw[len(raw) // 2] ^= 0x40
+    files[0].write_bytes(bytes(raw))
+    third = bvh_device_layouts(m["positions"], m["indices"], cache_dir=str(tmp_path))
+    assert not third[4] and np.array_equal(third[0], fresh[0])
+    assert bvh_device_layouts(m["positions"], m["indices"], cache_dir=str(tmp_path))[4]
+    # truncated file
+    files[0].write_bytes(files[0].read_bytes()[:100])
+    assert not bvh_device_layouts(m["positions"], m["indices"], cache_dir=str(tmp_path))[4]
+    # a moved vertex is a different key
+    p2 = np.array(m["positions"], copy=True)
+    p2[0, 0] += 1e-3
+    other = bvh_device_layouts(p2, m["indices"], cache_dir=str(tmp_path))
+    assert not other[4]
+    assert len(list(tmp_path.glob("*.ilmbvh"))) == 2
