@@ -51,8 +51,15 @@ __device__ inline void write_hit(const KParams& P, uint32_t i, const Best& B) {
     gpw(P.hits.n)[i] = make_float4(B.n.x, B.n.y, B.n.z, __int_as_float(B.inst));
 }
 
+/* Reciprocal direction for the slab test. Components below 1e-30 in magnitude (axis-parallel
+ * rays) become +-1e-30 first: with a true 1/0 the slab bounds lo/d - o/d turn into inf - inf =
+ * NaN and a box the ray crosses could be rejected; with 1e-30 they are +-huge (or exactly 0),
+ * which keeps parallel slabs exact and the fma products finite for |o| < 3e8. Only the box test
+ * uses this; triangles and spheres see the true direction. */
+__device__ inline float safe_dir(float x) { return __builtin_fabsf(x) < 1e-30f ? __builtin_copysignf(1e-30f, x) : x; }
 __device__ inline V3 rcp3(V3 d) {
-    return mk3(__builtin_amdgcn_rcpf(d.x), __builtin_amdgcn_rcpf(d.y), __builtin_amdgcn_rcpf(d.z));
+    return mk3(__builtin_amdgcn_rcpf(safe_dir(d.x)), __builtin_amdgcn_rcpf(safe_dir(d.y)),
+               __builtin_amdgcn_rcpf(safe_dir(d.z)));
 }
 __device__ inline V3 neg_o_over_d(V3 o, V3 id) { return mk3(-o.x * id.x, -o.y * id.y, -o.z * id.z); }
 
