@@ -5,7 +5,9 @@ gives bit-identical images (samples are summed per pixel in pass order whatever 
 or chunking), so the tuner only measures time.
 
     extend_mode      0: each wave traces one batch of 64 rays to completion
-                     1: lanes refill from a wave-local queue as their rays finish
+                     1: idle lanes refill from a wave-local queue at BLAS leaves
+                     2: idle lanes refill after every traversal step
+    refill_min       idle lanes that trigger a refill (modes 1 and 2)
     sample_group     consecutive path ids per pixel (ray coherence of the first bounces)
     tile_rows        pixel walk in bands of this many rows, column by column (wave footprint)
     paths_in_flight  paths per chunk (queue size): larger chunks amortise launch tails,
@@ -22,7 +24,8 @@ import time
 DEFAULT_SPACE = (
     ("sample_group", (1, 4, 16, 64)),
     ("tile_rows", (1, 4, 8)),
-    ("extend_mode", (0, 1)),
+    ("extend_mode", (0, 1, 2)),
+    ("refill_min", (4, 8, 16, 32)),
     ("paths_in_flight", (1 << 24, 1 << 25, 1 << 26)),
 )
 

@@ -426,7 +426,8 @@ static int ctx_alloc_paths(ilm_ctx* ctx, int64_t cap) {
 static bool opts_valid(const ilm_opts& o) {
     const bool sg_ok = o.sample_group >= 0 && o.sample_group <= 4096 && (o.sample_group & (o.sample_group - 1)) == 0;
     return !(o.bvh_layout < 0 || o.bvh_layout > ILM_BVH_QUANT8 || o.tri_test < 0 || o.tri_test > ILM_TRI_WATERTIGHT ||
-             o.extend_mode < 0 || o.extend_mode > 1 || !sg_ok || o.tile_rows < 0 || o.tile_rows > 256 ||
+             o.extend_mode < 0 || o.extend_mode > 2 || !sg_ok || o.tile_rows < 0 || o.tile_rows > 256 ||
+             o.refill_min < 0 || o.refill_min > 64 ||
              o.chunk_passes < 0 || o.paths_in_flight < 0 || o.hbm_budget_bytes < 0 || o.max_queued < 0);
 }
 
@@ -467,7 +468,7 @@ int ilm_open(int device, const ilm_opts* opts, ilm_ctx** out) {
     int occ = ilm_k_extend_occupancy(ctx->opts.tri_test, ctx->opts.bvh_layout, ctx->opts.extend_mode);
     ctx->ext_grid = ctx->n_cu * std::max(1, occ);
     ctx->gen_grid = ctx->n_cu * 8;
-    ctx->shade_grid = ctx->n_cu * 8;
+    ctx->shade_grid = ctx->n_cu * 4;   /* 512-thread blocks, block-stride loop */
     int rc = ctx_alloc_paths(ctx, cap_of(ctx->opts));
     if (rc) return rc;
     auto A = [&](void** p, size_t b) -> hipError_t {
@@ -562,6 +563,7 @@ int ilm_tune(ilm_ctx* ctx, const ilm_opts* o) {
     }
     ctx->opts.sample_group = o->sample_group;
     ctx->opts.tile_rows = o->tile_rows;
+    ctx->opts.refill_min = o->refill_min;
     ctx->opts.chunk_passes = o->chunk_passes;
     const int64_t cap = cap_of(*o);
     if (cap != ctx->cap) {
@@ -610,6 +612,7 @@ static KParams base_params(ilm_ctx* ctx) {
     P.tri_test = ctx->opts.tri_test;
     P.extend_mode = ctx->opts.extend_mode;
     P.tile_rows = std::max(1, ctx->opts.tile_rows);
+    P.refill_min = ctx->opts.refill_min > 0 ? ctx->opts.refill_min : 8;
     P.trav_stats = ctx->opts.count_tests ? ctx->d_trav : nullptr;
     return P;
 }

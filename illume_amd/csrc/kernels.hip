@@ -20,7 +20,8 @@
  */
 #include "traverse.h"
 
-#define SHADE_BLOCK 256
+#define SHADE_BLOCK 512                       /* 8 waves share one compaction atomic */
+#define SHADE_WAVES (SHADE_BLOCK / 64)
 
 __global__ void k_ctl_init(DCtl* ctl_, const DChunk* chunk) {
     GLOBAL DCtl* ctl = gpw(ctl_);
@@ -360,11 +361,14 @@ __device__ inline int shade_path(const GLOBAL DScene* S, const float ray_bias, c
 }
 
 __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(KParams P, int depth) {
+    /* survivors of the block's 512 paths get their queue slots from one atomic: per-wave counts
+     * meet in LDS, wave 0 lane 0 reserves the block's range, each wave takes its offset */
+    __shared__ uint32_t wave_n[SHADE_WAVES];
+    __shared__ uint32_t block_base;
     GLOBAL DCtl* ctl = gpw(P.ctl);
     const uint32_t count = ctl->count[depth];
     const int lane = threadIdx.x & 63;
-    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+    const int wid = threadIdx.x >> 6;
     const GLOBAL DReq* reqs = gp(P.chunk)->reqs;
     const GLOBAL float4* ino = gp(P.in.o);
     const GLOBAL float4* ind = gp(P.in.d);
@@ -376,8 +380,10 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(KParams P, int depth) {
     GLOBAL float4* outd = gpw(P.out.d);
     GLOBAL float4* outc = gpw(P.out.c);
     GLOBAL uint2* outr = gpw(P.out.rng);
-    for (uint32_t wbase = wave * 64u; wbase < count; wbase += nwaves * 64u) {
-        uint32_t i = wbase + lane;
+    /* the loop bound depends only on blockIdx and count, so every thread of the block runs the
+     * same trips and reaches every barrier */
+    for (uint32_t tbase = blockIdx.x * (uint32_t)SHADE_BLOCK; tbase < count; tbase += gridDim.x * (uint32_t)SHADE_BLOCK) {
+        uint32_t i = tbase + threadIdx.x;
         int status = ST_DEAD;
         float4 oo = make_float4(0, 0, 0, 0), od = oo, oc = oo;
         uint2 org = make_uint2(0, 0);
@@ -405,13 +411,22 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(KParams P, int depth) {
             oc = make_float4(color.x, color.y, color.z, u2f(pid));
             org = make_uint2((uint32_t)rs, (uint32_t)(rs >> 32));
         }
-        /* compaction: ballot + prefix within the wave, one atomic per wave */
+        /* compaction: ballot + prefix within the wave, wave offsets within the block, one
+         * atomic per block */
         const bool alive = status == ST_ALIVE;
         const unsigned long long mask = __ballot(alive);
         const uint32_t n = (uint32_t)__popcll(mask);
-        uint32_t base = 0;
-        if (lane == 0 && n) base = atomicAdd(&ctl->count[depth + 1], n);
-        base = __shfl(base, 0);
+        if (lane == 0) wave_n[wid] = n;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t total = 0;
+            for (int w = 0; w < SHADE_WAVES; w++) total += wave_n[w];
+            block_base = total ? atomicAdd(&ctl->count[depth + 1], total) : 0u;
+        }
+        __syncthreads();
+        uint32_t base = block_base;
+        for (int w = 0; w < wid; w++) base += wave_n[w];
+        __syncthreads();   /* wave_n and block_base are rewritten by the next trip */
         if (alive) {
             const uint32_t slot = base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
             outo[slot] = oo;
@@ -508,7 +523,9 @@ hipError_t ilm_k_raygen(const KParams& P, int grid, hipStream_t s) {
 
 template <int T, int Q, int C, int O>
 static void launch_extend(const KParams& P, int depth, int* ovf, int grid, hipStream_t s) {
-    if (P.extend_mode == 1)
+    if (P.extend_mode == 2)
+        ILM_LAUNCH((k_extend_pers<T, Q, C, O>), dim3(grid), dim3(EXT_BLOCK), 0, s, P, depth, ovf);
+    else if (P.extend_mode == 1)
         ILM_LAUNCH((k_extend_refill<T, Q, C, O>), dim3(grid), dim3(EXT_BLOCK), 0, s, P, depth, ovf);
     else
         ILM_LAUNCH((k_extend<T, Q, C, O>), dim3(grid), dim3(EXT_BLOCK), 0, s, P, depth, ovf);
@@ -567,7 +584,12 @@ static int occ_of(K kernel) {
 /* Blocks per CU of the extend kernel (the smallest over the variants a context may launch). */
 int ilm_k_extend_occupancy(int tri_test, int bvh_layout, int extend_mode) {
     int a, b;
-    if (extend_mode == 1) {
+    if (extend_mode == 2) {
+        if (!tri_test && !bvh_layout) { a = occ_of(k_extend_pers<0, 0, 0, 0>); b = occ_of(k_extend_pers<0, 0, 0, 1>); }
+        else if (tri_test && !bvh_layout) { a = occ_of(k_extend_pers<1, 0, 0, 0>); b = occ_of(k_extend_pers<1, 0, 0, 1>); }
+        else if (!tri_test) { a = occ_of(k_extend_pers<0, 1, 0, 0>); b = occ_of(k_extend_pers<0, 1, 0, 1>); }
+        else { a = occ_of(k_extend_pers<1, 1, 0, 0>); b = occ_of(k_extend_pers<1, 1, 0, 1>); }
+    } else if (extend_mode == 1) {
         if (!tri_test && !bvh_layout) { a = occ_of(k_extend_refill<0, 0, 0, 0>); b = occ_of(k_extend_refill<0, 0, 0, 1>); }
         else if (tri_test && !bvh_layout) { a = occ_of(k_extend_refill<1, 0, 0, 0>); b = occ_of(k_extend_refill<1, 0, 0, 1>); }
         else if (!tri_test) { a = occ_of(k_extend_refill<0, 1, 0, 0>); b = occ_of(k_extend_refill<0, 1, 0, 1>); }

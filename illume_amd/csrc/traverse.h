@@ -291,8 +291,12 @@ __device__ inline void trace_ray(const GLOBAL DScene* S, V3 O, V3 D, Best& B, St
     }
 }
 
-/* One ray batch per wave: a wave takes 64 consecutive queue entries (spatially coherent at
- * depth 0) with one atomic and walks them to completion. */
+/* One ray batch per wave at a time: a wave reserves EXT_FETCH consecutive queue entries with
+ * one atomic (fewer same-address atomics than one per batch) and walks them as consecutive
+ * 64-ray batches, each to completion. */
+#ifndef EXT_FETCH
+#define EXT_FETCH 256
+#endif
 template <int TRI, int QUANT, int COUNT, int OVF>
 __global__ __launch_bounds__(EXT_BLOCK) void k_extend(KParams P, int depth, int* ovf_base) {
     __shared__ int stk[LDS_STACK * EXT_BLOCK];
@@ -309,17 +313,20 @@ __global__ __launch_bounds__(EXT_BLOCK) void k_extend(KParams P, int depth, int*
     TravCount tc = {0u, 0u};
     for (;;) {
         uint32_t base = 0;
-        if (lane == 0) base = atomicAdd(&ctl->fetch[depth], (uint32_t)EXT_BLOCK);
+        if (lane == 0) base = atomicAdd(&ctl->fetch[depth], (uint32_t)EXT_FETCH);
         base = __shfl(base, 0);
         if (base >= count) break;
-        const uint32_t i = base + lane;
-        if (i < count) {
-            float4 o4 = ino[i];
-            float4 d4 = ind[i];
-            const GLOBAL DScene* S = gp(reqs[f2u(o4.w) >> 16].scene);
-            Best B;
-            trace_ray<TRI, QUANT, COUNT, OVF>(S, mk3(o4.x, o4.y, o4.z), mk3(d4.x, d4.y, d4.z), B, st, &tc);
-            write_hit(P, i, B);
+        const uint32_t end = base + EXT_FETCH < count ? base + EXT_FETCH : count;
+        for (uint32_t b = base; b < end; b += EXT_BLOCK) {
+            const uint32_t i = b + lane;
+            if (i < end) {
+                float4 o4 = ino[i];
+                float4 d4 = ind[i];
+                const GLOBAL DScene* S = gp(reqs[f2u(o4.w) >> 16].scene);
+                Best B;
+                trace_ray<TRI, QUANT, COUNT, OVF>(S, mk3(o4.x, o4.y, o4.z), mk3(d4.x, d4.y, d4.z), B, st, &tc);
+                write_hit(P, i, B);
+            }
         }
     }
     if (COUNT) {
@@ -330,13 +337,10 @@ __global__ __launch_bounds__(EXT_BLOCK) void k_extend(KParams P, int depth, int*
 
 /* ---------------- extend with lane refill ----------------
  * Each lane keeps its ray's traversal state across an outer loop; after every BLAS leaf (and
- * when its ray finishes) the wave counts its idle lanes and, once REFILL_MIN of them are idle,
+ * when its ray finishes) the wave counts its idle lanes and, once P.refill_min of them are idle,
  * hands them the next rays of a wave-local pool taken from the queue 64 at a time (persistent
  * traversal with dynamic fetch, Aila and Laine, HPG 2009). Trades ray coherence for lane
  * occupancy; selected with ilm_opts.extend_mode = 1. */
-#ifndef REFILL_MIN
-#define REFILL_MIN 16
-#endif
 #define POP_MARK ((int)0x7ffffffd)
 #define NO_RAY 0xffffffffu
 
@@ -451,13 +455,13 @@ __global__ __launch_bounds__(EXT_BLOCK) void k_extend_refill(KParams P, int dept
         const bool idle = ridx == NO_RAY;
         const unsigned long long idle_mask = __ballot(idle);
         const uint32_t n_idle = (uint32_t)__popcll(idle_mask);
-        if (!drained && n_idle >= (uint32_t)REFILL_MIN) {
+        if (!drained && n_idle >= (uint32_t)P.refill_min) {
             if (pool == pool_end) {
                 uint32_t base = 0;
-                if (lane == 0) base = atomicAdd(&ctl->fetch[depth], (uint32_t)EXT_BLOCK);
+                if (lane == 0) base = atomicAdd(&ctl->fetch[depth], (uint32_t)EXT_FETCH);
                 base = __shfl(base, 0);
                 pool = base < count ? base : count;
-                pool_end = base + EXT_BLOCK < count ? base + EXT_BLOCK : count;
+                pool_end = base + EXT_FETCH < count ? base + EXT_FETCH : count;
             }
             const uint32_t take = n_idle < pool_end - pool ? n_idle : pool_end - pool;
             if (idle) {
@@ -480,6 +484,160 @@ __global__ __launch_bounds__(EXT_BLOCK) void k_extend_refill(KParams P, int dept
             }
         }
         if (drained && __ballot(ridx != NO_RAY) == 0ull) break;
+    }
+    if (COUNT) {
+        atomicAdd(&P.trav_stats[0], (unsigned long long)tc.boxes);
+        atomicAdd(&P.trav_stats[1], (unsigned long long)tc.tris);
+    }
+}
+
+/* ---------------- extend with per-step refill ----------------
+ * Like the refill kernel, but a lane advances its ray by exactly one action per loop trip (one
+ * inner-node visit, one instance entry or one BLAS leaf, plus the pops that follow), and the
+ * wave re-checks for idle lanes on every trip, so a lane whose ray finished takes a new one
+ * after at most one trip instead of waiting for the rest of the wave to reach a leaf. The world
+ * ray stays in registers for the return from an instance. Selected with extend_mode = 2. */
+struct PState {
+    V3 O, D;                /* world ray */
+    V3 o, d, id, noid;      /* ray in the current (world or object) space */
+    const GLOBAL BNode* nodes;
+    const GLOBAL QNode* qnodes;
+    const GLOBAL float4* tris;
+    const GLOBAL DScene* S;
+    int node;
+    int cur;
+    int blas;
+    WTRay wt;
+};
+
+template <int QUANT, int OVF>
+__device__ inline void pstep_init(PState& T, Best& B, Stack<OVF>& st, const GLOBAL DScene* S, V3 O, V3 D) {
+    best_init(B);
+    spheres_first(S, O, D, B);
+    T.S = S;
+    T.node = QUANT ? S->qtlas_root : S->tlas_root;
+    if (T.node == ILM_NO_CHILD) T.node = POP_MARK;
+    T.nodes = gp(S->tlas);
+    T.qnodes = gp(S->qtlas);
+    T.tris = nullptr;
+    T.blas = 0;
+    T.cur = -1;
+    T.O = O;
+    T.D = D;
+    T.o = O;
+    T.d = D;
+    T.id = rcp3(D);
+    T.noid = neg_o_over_d(O, T.id);
+    st.sp = 0;
+}
+
+/* One action of the walk; true when the ray is finished. */
+template <int TRI, int QUANT, int COUNT, int OVF>
+__device__ inline bool pstep(PState& T, Best& B, Stack<OVF>& st, TravCount* tc) {
+    int node = T.node;
+    bool pop = true;
+    if (node == POP_MARK) {
+        pop = true;
+    } else if (node >= 0) {
+        pop = !inner_step<QUANT, OVF, COUNT>(T.nodes, T.qnodes, node, T.id, T.noid, B.t, st, tc);
+    } else if (!T.blas) {
+        const int k = gp(T.S->tlas_prims)[ilm_leaf_first(node)];
+        const GLOBAL DInst* I = gp(T.S->inst) + k;
+        const GLOBAL DMesh* M = gp(T.S->meshes) + I->mesh;
+        const int broot = QUANT ? M->qroot : M->root;
+        if (broot != ILM_NO_CHILD) {
+            float inv[12];
+            load_rows12(I->inv, inv);
+            T.o = xform3(inv, T.O, 1.0f);
+            T.d = xform3(inv, T.D, 0.0f);
+            T.id = rcp3(T.d);
+            T.noid = neg_o_over_d(T.o, T.id);
+            if (TRI) T.wt = wt_setup(T.d);
+            T.nodes = gp(M->nodes);
+            T.qnodes = gp(M->qnodes);
+            T.tris = gp(M->tris);
+            st.push(STACK_SENTINEL);
+            T.blas = 1;
+            T.cur = k;
+            node = broot;
+            pop = false;
+        }
+    } else {
+        leaf_tris<TRI, COUNT>(T.tris, node, T.o, T.d, T.wt, T.cur, B, tc);
+    }
+    if (pop) {
+        for (;;) {
+            if (st.sp == 0) return true;
+            node = st.pop();
+            if (node != STACK_SENTINEL) break;
+            T.o = T.O;
+            T.d = T.D;
+            T.id = rcp3(T.D);
+            T.noid = neg_o_over_d(T.O, T.id);
+            T.blas = 0;
+            T.nodes = gp(T.S->tlas);
+            T.qnodes = gp(T.S->qtlas);
+        }
+    }
+    T.node = node;
+    return false;
+}
+
+template <int TRI, int QUANT, int COUNT, int OVF>
+__global__ __launch_bounds__(EXT_BLOCK) void k_extend_pers(KParams P, int depth, int* ovf_base) {
+    __shared__ int stk[LDS_STACK * EXT_BLOCK];
+    const int lane = threadIdx.x;
+    Stack<OVF> st;
+    st.lds = (LDSAS int*)stk + lane;
+    st.ovf = gpw(ovf_base) + ((size_t)blockIdx.x * EXT_BLOCK + lane) * OVF_STACK;
+    st.sp = 0;
+    GLOBAL DCtl* ctl = gpw(P.ctl);
+    const uint32_t count = ctl->count[depth];
+    const GLOBAL float4* ino = gp(P.in.o);
+    const GLOBAL float4* ind = gp(P.in.d);
+    const GLOBAL DReq* reqs = gp(P.chunk)->reqs;
+    const unsigned long long lt_mask = (1ull << lane) - 1ull;
+    const uint32_t refill_min = (uint32_t)P.refill_min;
+    uint32_t ridx = NO_RAY;
+    bool drained = false;
+    uint32_t pool = 0, pool_end = 0;
+    PState T;
+    Best B;
+    TravCount tc = {0u, 0u};
+    for (;;) {
+        const bool idle = ridx == NO_RAY;
+        const unsigned long long idle_mask = __ballot(idle);
+        const uint32_t n_idle = (uint32_t)__popcll(idle_mask);
+        if (drained) {
+            if (n_idle == EXT_BLOCK) break;
+        } else if (n_idle >= refill_min) {
+            if (pool == pool_end) {
+                uint32_t base = 0;
+                if (lane == 0) base = atomicAdd(&ctl->fetch[depth], (uint32_t)EXT_FETCH);
+                base = __shfl(base, 0);
+                pool = base < count ? base : count;
+                pool_end = base + EXT_FETCH < count ? base + EXT_FETCH : count;
+            }
+            const uint32_t take = n_idle < pool_end - pool ? n_idle : pool_end - pool;
+            if (idle) {
+                const uint32_t rank = (uint32_t)__popcll(idle_mask & lt_mask);
+                if (rank < take) {
+                    const uint32_t r = pool + rank;
+                    ridx = r;
+                    float4 o4 = ino[r], d4 = ind[r];
+                    const GLOBAL DScene* S = gp(reqs[f2u(o4.w) >> 16].scene);
+                    pstep_init<QUANT, OVF>(T, B, st, S, mk3(o4.x, o4.y, o4.z), mk3(d4.x, d4.y, d4.z));
+                }
+            }
+            pool += take;
+            if (pool == pool_end && pool_end >= count) drained = true;
+        }
+        if (ridx != NO_RAY) {
+            if (pstep<TRI, QUANT, COUNT, OVF>(T, B, st, &tc)) {
+                write_hit(P, ridx, B);
+                ridx = NO_RAY;
+            }
+        }
     }
     if (COUNT) {
         atomicAdd(&P.trav_stats[0], (unsigned long long)tc.boxes);

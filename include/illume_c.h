@@ -182,7 +182,8 @@ typedef struct ilm_opts {
     int64_t hbm_budget_bytes;   /* serving admission budget; 0 = 90% of free device memory */
     int32_t use_graphs;         /* capture the per-chunk pipeline as a hipGraph */
     int32_t max_queued;         /* serving: admission queue bound (0 = 4096) */
-    int32_t extend_mode;        /* 0: one coherent ray batch per wave; 1: lanes refill as rays finish */
+    int32_t extend_mode;        /* 0: one coherent ray batch per wave; 1: idle lanes refill at BLAS
+                                   leaves; 2: idle lanes refill after every traversal step */
     int32_t profile;            /* time every extend launch with HIP events (disables graphs) */
     int32_t count_tests;        /* count box/triangle tests of the extend kernel (slower) */
     int32_t verbose;
@@ -199,6 +200,8 @@ typedef struct ilm_opts {
      * device layouts; replaces the reference's unversioned .bvh files, src/accel/bvh.cpp:399-439),
      * or NULL for none. The string is copied at ilm_open. */
     const char* bvh_cache_dir;
+    /* extend modes 1 and 2: number of idle lanes of a wave that triggers a refill (0 = 8) */
+    int32_t refill_min;
 } ilm_opts;
 
 typedef struct ilm_render_info {

@@ -79,6 +79,18 @@ def test_closest_hit_gold_100k(ctx, gold):
     assert t_exact
 
 
+@pytest.mark.parametrize("mode", [1, 2])
+def test_closest_hit_refill_extend_modes(gold, mode):
+    import illume_amd
+    spec, st, _, orc = gold
+    with illume_amd.Context(0, extend_mode=mode, refill_min=4) as c:
+        sc = c.scene(spec)
+        rays = np.concatenate([camera_rays(spec, st, 20000, seed=31), random_rays(-1.5, 1.5, 20000, seed=32)])
+        g, o, bad, t_exact = _hit_parity(c, sc, orc, rays, "gold mode %d" % mode)
+        sc.free()
+    assert len(bad) == 0 and t_exact
+
+
 def test_closest_hit_quantized_layout(ctx_quant, gold):
     spec, st, _, orc = gold
     sc = ctx_quant.scene(spec)
