@@ -85,6 +85,11 @@ def _ip(a):
     return a.ctypes.data_as(C.POINTER(C.c_int32))
 
 
+def _pxp(a):
+    """(n, 3) uint64 array as ilm_pixel* (same layout: three size_t per pixel), or None."""
+    return None if a is None else a.ctypes.data_as(C.POINTER(Pixel))
+
+
 def _check(rc, ctx=None):
     if rc != 0:
         L = lib()
@@ -118,7 +123,8 @@ class Context:
         self.h = h
         self.opts = o
 
-    TUNABLE = ("extend_mode", "refill_min", "sample_group", "tile_rows", "chunk_passes", "paths_in_flight")
+    TUNABLE = ("extend_mode", "refill_min", "ext_fetch", "sample_group", "tile_rows", "chunk_passes",
+               "paths_in_flight")
 
     def tune(self, **kw):
         """Change the tunable options (TUNABLE) of an idle context; others keep their values."""
@@ -157,28 +163,27 @@ class Context:
     def render(self, scene, params, want_accum=True, want_pixels=True):
         """Plain synchronous render. Returns (accum (H,W,3) f32 sum over spp, pixels (H,W,3) u64, info)."""
         W, H = params.width, params.height
-        acc = np.zeros((H * W, 3), dtype=np.float32) if want_accum else None
-        px = (Pixel * (W * H))() if want_pixels else None
+        acc = np.empty((H * W, 3), dtype=np.float32) if want_accum else None
+        pix = np.empty((H * W, 3), dtype=np.uint64) if want_pixels else None
         info = RenderInfo()
         _check(lib().ilm_render(self.h, scene.h, C.byref(params), _fp(acc) if acc is not None else None,
-                                px, C.byref(info)), self.h)
-        pix = np.frombuffer(px, dtype=np.uint64).reshape(H, W, 3) if px is not None else None
-        return (acc.reshape(H, W, 3) if acc is not None else None), pix, struct_to_dict(info)
+                                _pxp(pix), C.byref(info)), self.h)
+        return (acc.reshape(H, W, 3) if acc is not None else None), \
+            (pix.reshape(H, W, 3) if pix is not None else None), struct_to_dict(info)
 
     def render_batch(self, scenes, params, want_accum=True):
         n = len(scenes)
         hs = (C.c_void_p * n)(*[s.h.value for s in scenes])
         ps = (RenderParams * n)(*params)
-        accs = [np.zeros((p.height * p.width, 3), dtype=np.float32) for p in params]
-        pxs = [(Pixel * (p.width * p.height))() for p in params]
+        accs = [np.empty((p.height * p.width, 3), dtype=np.float32) for p in params]
+        pxs = [np.empty((p.height * p.width, 3), dtype=np.uint64) for p in params]
         acc_ptrs = (C.POINTER(C.c_float) * n)(*[_fp(a) for a in accs])
-        px_ptrs = (C.POINTER(Pixel) * n)(*[C.cast(p, C.POINTER(Pixel)) for p in pxs])
+        px_ptrs = (C.POINTER(Pixel) * n)(*[_pxp(p) for p in pxs])
         info = RenderInfo()
         _check(lib().ilm_render_batch(self.h, n, hs, ps, acc_ptrs if want_accum else None, px_ptrs, C.byref(info)), self.h)
         out = []
         for p, a, px in zip(params, accs, pxs):
-            out.append((a.reshape(p.height, p.width, 3),
-                        np.frombuffer(px, dtype=np.uint64).reshape(p.height, p.width, 3)))
+            out.append((a.reshape(p.height, p.width, 3), px.reshape(p.height, p.width, 3)))
         return out, struct_to_dict(info)
 
     def submit(self, scene, params):
@@ -259,10 +264,10 @@ class Request:
 
     def result(self):
         p = self.params
-        acc = np.zeros((p.height * p.width, 3), dtype=np.float32)
-        px = (Pixel * (p.width * p.height))()
-        _check(lib().ilm_result(self.h, _fp(acc), px), self.ctx.h)
-        return acc.reshape(p.height, p.width, 3), np.frombuffer(px, dtype=np.uint64).reshape(p.height, p.width, 3)
+        acc = np.empty((p.height * p.width, 3), dtype=np.float32)
+        px = np.empty((p.height * p.width, 3), dtype=np.uint64)
+        _check(lib().ilm_result(self.h, _fp(acc), _pxp(px)), self.ctx.h)
+        return acc.reshape(p.height, p.width, 3), px.reshape(p.height, p.width, 3)
 
     def cancel(self):
         _check(lib().ilm_cancel(self.h))

@@ -8,6 +8,7 @@ or chunking), so the tuner only measures time.
                      1: idle lanes refill from a wave-local queue at BLAS leaves
                      2: idle lanes refill after every traversal step
     refill_min       idle lanes that trigger a refill (modes 1 and 2)
+    ext_fetch        rays an extend wave reserves per queue atomic (work-queue granularity)
     sample_group     consecutive path ids per pixel (ray coherence of the first bounces)
     tile_rows        pixel walk in bands of this many rows, column by column (wave footprint)
     paths_in_flight  paths per chunk (queue size): larger chunks amortise launch tails,
@@ -26,6 +27,7 @@ DEFAULT_SPACE = (
     ("tile_rows", (1, 4, 8)),
     ("extend_mode", (0, 1, 2)),
     ("refill_min", (4, 8, 16, 32)),
+    ("ext_fetch", (128, 256, 512, 1024, 2048)),
     ("paths_in_flight", (1 << 24, 1 << 25, 1 << 26)),
 )
 

@@ -34,12 +34,40 @@ struct ilm_scene {
 };
 
 /* One render request. */
+/* Request buffers (Kahan sums, pixels) are recycled instead of freed: hipFree waits for the
+ * whole device, which would stall the scheduler behind the chunk in flight. Holds at most
+ * `cap` bytes; a buffer is reused for a request needing between half and all of it. */
+struct BufPool {
+    std::mutex mu;                          /* leaf lock */
+    std::multimap<size_t, void*> free;
+    size_t bytes = 0, cap = (size_t)2 << 30;
+    void* take(size_t n, size_t* got) {
+        std::lock_guard<std::mutex> lk(mu);
+        auto it = free.lower_bound(n);
+        if (it == free.end() || it->first > 2 * n) return nullptr;
+        void* p = it->second;
+        *got = it->first;
+        bytes -= it->first;
+        free.erase(it);
+        return p;
+    }
+    bool give(void* p, size_t n) {          /* false: caller frees */
+        std::lock_guard<std::mutex> lk(mu);
+        if (bytes + n > cap) return false;
+        free.emplace(n, p);
+        bytes += n;
+        return true;
+    }
+};
+
 struct Job {
     ilm_scene* scene = nullptr;
     ilm_render_params p;
     DReq dreq;
     float4* d_accum = nullptr;      /* 2*W*H: Kahan sums then compensations */
     ilm_pixel* d_px = nullptr;
+    size_t accum_cap = 0, px_cap = 0;   /* allocation sizes (pool buffers may be larger) */
+    BufPool* pool = nullptr;
     int64_t bytes = 0;              /* device bytes while admitted */
     int next_pass = 0;              /* issue cursor: pass, pixel */
     int64_t next_px = 0;
@@ -89,6 +117,7 @@ struct ilm_ctx {
     unsigned long long* d_seg_total = nullptr;
     unsigned long long* d_trav = nullptr;
     std::vector<void*> allocs;
+    BufPool bufs;
     std::vector<void*> path_allocs;  /* path state, hits, contributions (resized by ilm_tune) */
     int64_t own_bytes = 0, path_bytes = 0;
 

@@ -427,7 +427,7 @@ static bool opts_valid(const ilm_opts& o) {
     const bool sg_ok = o.sample_group >= 0 && o.sample_group <= 4096 && (o.sample_group & (o.sample_group - 1)) == 0;
     return !(o.bvh_layout < 0 || o.bvh_layout > ILM_BVH_QUANT8 || o.tri_test < 0 || o.tri_test > ILM_TRI_WATERTIGHT ||
              o.extend_mode < 0 || o.extend_mode > 2 || !sg_ok || o.tile_rows < 0 || o.tile_rows > 256 ||
-             o.refill_min < 0 || o.refill_min > 64 ||
+             o.refill_min < 0 || o.refill_min > 64 || o.ext_fetch < 0 || o.ext_fetch > 65536 || (o.ext_fetch & 63) != 0 ||
              o.chunk_passes < 0 || o.paths_in_flight < 0 || o.hbm_budget_bytes < 0 || o.max_queued < 0);
 }
 
@@ -504,6 +504,8 @@ static void ctx_destroy(ilm_ctx* ctx) {
     for (auto& kv : ctx->graphs) (void)hipGraphExecDestroy(kv.second);
     ctx_free_paths(ctx);
     if (ctx->ovf) (void)hipFree(ctx->ovf);
+    for (auto& kv : ctx->bufs.free) (void)hipFree(kv.second);
+    ctx->bufs.free.clear();
     for (void* p : ctx->allocs) (void)hipFree(p);
     if (ctx->seg_count) (void)hipFree(ctx->seg_count);
     for (int i = 0; i < ILM_RING; i++) {
@@ -533,8 +535,8 @@ void ilm_close(ilm_ctx* ctx) {
     ctx_unref(ctx);
 }
 
-/* Retune an idle context: extend_mode, sample_group, tile_rows, chunk_passes and
- * paths_in_flight take the values in *o (the other fields are ignored). Path buffers are
+/* Retune an idle context: extend_mode, refill_min, ext_fetch, sample_group, tile_rows,
+ * chunk_passes and paths_in_flight take the values in *o (the other fields are ignored). Path buffers are
  * reallocated when the chunk size changes; captured graphs are dropped. */
 int ilm_tune(ilm_ctx* ctx, const ilm_opts* o) {
     if (!ctx || !o) return fail(ctx, ILM_E_INVALID, "null argument");
@@ -564,6 +566,7 @@ int ilm_tune(ilm_ctx* ctx, const ilm_opts* o) {
     ctx->opts.sample_group = o->sample_group;
     ctx->opts.tile_rows = o->tile_rows;
     ctx->opts.refill_min = o->refill_min;
+    ctx->opts.ext_fetch = o->ext_fetch;
     ctx->opts.chunk_passes = o->chunk_passes;
     const int64_t cap = cap_of(*o);
     if (cap != ctx->cap) {
@@ -611,8 +614,9 @@ static KParams base_params(ilm_ctx* ctx) {
     P.bvh_layout = ctx->opts.bvh_layout;
     P.tri_test = ctx->opts.tri_test;
     P.extend_mode = ctx->opts.extend_mode;
-    P.tile_rows = std::max(1, ctx->opts.tile_rows);
+    P.tile_rows = ctx->opts.tile_rows > 0 ? ctx->opts.tile_rows : 8;
     P.refill_min = ctx->opts.refill_min > 0 ? ctx->opts.refill_min : 8;
+    P.ext_fetch = ctx->opts.ext_fetch > 0 ? ctx->opts.ext_fetch : 256;
     P.trav_stats = ctx->opts.count_tests ? ctx->d_trav : nullptr;
     return P;
 }
@@ -688,8 +692,18 @@ static int launch_chunk(ilm_ctx* ctx, const DChunk& hc, bool ovf, ProfileSink* p
 static int job_prepare(ilm_ctx* ctx, Job* j) {
     const ilm_render_params& p = j->p;
     const int64_t wh = (int64_t)p.width * p.height;
-    HIPTRY(ctx, hipMalloc((void**)&j->d_accum, 2 * wh * sizeof(float4)));
-    HIPTRY(ctx, hipMalloc((void**)&j->d_px, wh * sizeof(ilm_pixel)));
+    const size_t ab = 2 * wh * sizeof(float4), pb = wh * sizeof(ilm_pixel);
+    j->pool = &ctx->bufs;
+    j->d_accum = (float4*)ctx->bufs.take(ab, &j->accum_cap);
+    if (!j->d_accum) {
+        HIPTRY(ctx, hipMalloc((void**)&j->d_accum, ab));
+        j->accum_cap = ab;
+    }
+    j->d_px = (ilm_pixel*)ctx->bufs.take(pb, &j->px_cap);
+    if (!j->d_px) {
+        HIPTRY(ctx, hipMalloc((void**)&j->d_px, pb));
+        j->px_cap = pb;
+    }
     HIPTRY(ctx, ilm_k_fill_zero4(j->d_accum, 2 * wh, ctx->stream));
     std::memset(&j->dreq, 0, sizeof(j->dreq));
     j->dreq.scene = j->scene->d_scene;
@@ -707,9 +721,11 @@ static int job_prepare(ilm_ctx* ctx, Job* j) {
     return ILM_OK;
 }
 
+/* Return the request's buffers to the pool (stream order keeps a later user of the same buffer
+ * behind every kernel already queued on it). */
 static void job_release_device(Job* j) {
-    if (j->d_accum) (void)hipFree(j->d_accum);
-    if (j->d_px) (void)hipFree(j->d_px);
+    if (j->d_accum && !(j->pool && j->pool->give(j->d_accum, j->accum_cap))) (void)hipFree(j->d_accum);
+    if (j->d_px && !(j->pool && j->pool->give(j->d_px, j->px_cap))) (void)hipFree(j->d_px);
     j->d_accum = nullptr;
     j->d_px = nullptr;
 }
@@ -752,7 +768,8 @@ static void build_chunk(ilm_ctx* ctx, const std::vector<Job*>& jobs, DChunk& c, 
         }
         w.sgroup = 1;
         if (w.px0 == 0)
-            while (w.sgroup * 2 <= (uint32_t)std::max(1, ctx->opts.sample_group) && w.npass % (w.sgroup * 2) == 0) w.sgroup *= 2;
+            const uint32_t sg_max = ctx->opts.sample_group > 0 ? (uint32_t)ctx->opts.sample_group : 64u;
+            while (w.sgroup * 2 <= sg_max && w.npass % (w.sgroup * 2) == 0) w.sgroup *= 2;
         w.pid_base = c.total_paths;
         w.acc_base = c.total_px;
         c.total_paths += (uint32_t)((int64_t)w.npass * w.npx);
@@ -814,11 +831,11 @@ static int run_jobs_sync(ilm_ctx* ctx, std::vector<Job*>& jobs, ilm_render_info*
         chunks++;
         max_paths = std::max<int64_t>(max_paths, hc->total_paths);
     }
-    HIPTRY(ctx, hipEventRecord(e1, ctx->stream));
     for (Job* j : jobs) {
         const int64_t wh = (int64_t)j->p.width * j->p.height;
         HIPTRY(ctx, ilm_k_tonemap(j->d_accum, j->d_px, (int)wh, j->p.spp, j->p.tonemap, ctx->stream));
     }
+    HIPTRY(ctx, hipEventRecord(e1, ctx->stream));   /* device_ms: every kernel of the render, tonemap included */
     HIPTRY(ctx, hipMemcpyAsync(ctx->h_u64, ctx->d_seg_total, sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx->stream));
     HIPTRY(ctx, hipMemcpyAsync(ctx->h_u64 + 1, ctx->d_trav, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx->stream));
     HIPTRY(ctx, hipStreamSynchronize(ctx->stream));

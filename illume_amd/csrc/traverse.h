@@ -291,12 +291,9 @@ __device__ inline void trace_ray(const GLOBAL DScene* S, V3 O, V3 D, Best& B, St
     }
 }
 
-/* One ray batch per wave at a time: a wave reserves EXT_FETCH consecutive queue entries with
- * one atomic (fewer same-address atomics than one per batch) and walks them as consecutive
- * 64-ray batches, each to completion. */
-#ifndef EXT_FETCH
-#define EXT_FETCH 256
-#endif
+/* One ray batch per wave at a time: a wave reserves P.ext_fetch consecutive queue entries with
+ * one atomic (same-address atomics serialise, so one per 64-ray batch throttles the queue) and
+ * walks them as consecutive 64-ray batches, each to completion. */
 template <int TRI, int QUANT, int COUNT, int OVF>
 __global__ __launch_bounds__(EXT_BLOCK) void k_extend(KParams P, int depth, int* ovf_base) {
     __shared__ int stk[LDS_STACK * EXT_BLOCK];
@@ -307,16 +304,17 @@ __global__ __launch_bounds__(EXT_BLOCK) void k_extend(KParams P, int depth, int*
     st.sp = 0;
     GLOBAL DCtl* ctl = gpw(P.ctl);
     const uint32_t count = ctl->count[depth];
+    const uint32_t fetch = (uint32_t)P.ext_fetch;
     const GLOBAL float4* ino = gp(P.in.o);
     const GLOBAL float4* ind = gp(P.in.d);
     const GLOBAL DReq* reqs = gp(P.chunk)->reqs;
     TravCount tc = {0u, 0u};
     for (;;) {
         uint32_t base = 0;
-        if (lane == 0) base = atomicAdd(&ctl->fetch[depth], (uint32_t)EXT_FETCH);
+        if (lane == 0) base = atomicAdd(&ctl->fetch[depth], fetch);
         base = __shfl(base, 0);
         if (base >= count) break;
-        const uint32_t end = base + EXT_FETCH < count ? base + EXT_FETCH : count;
+        const uint32_t end = base + fetch < count ? base + fetch : count;
         for (uint32_t b = base; b < end; b += EXT_BLOCK) {
             const uint32_t i = b + lane;
             if (i < end) {
@@ -440,6 +438,7 @@ __global__ __launch_bounds__(EXT_BLOCK) void k_extend_refill(KParams P, int dept
     st.sp = 0;
     GLOBAL DCtl* ctl = gpw(P.ctl);
     const uint32_t count = ctl->count[depth];
+    const uint32_t fetch = (uint32_t)P.ext_fetch;
     const GLOBAL float4* ino = gp(P.in.o);
     const GLOBAL float4* ind = gp(P.in.d);
     const GLOBAL DReq* reqs = gp(P.chunk)->reqs;
@@ -458,10 +457,10 @@ __global__ __launch_bounds__(EXT_BLOCK) void k_extend_refill(KParams P, int dept
         if (!drained && n_idle >= (uint32_t)P.refill_min) {
             if (pool == pool_end) {
                 uint32_t base = 0;
-                if (lane == 0) base = atomicAdd(&ctl->fetch[depth], (uint32_t)EXT_FETCH);
+                if (lane == 0) base = atomicAdd(&ctl->fetch[depth], fetch);
                 base = __shfl(base, 0);
                 pool = base < count ? base : count;
-                pool_end = base + EXT_FETCH < count ? base + EXT_FETCH : count;
+                pool_end = base + fetch < count ? base + fetch : count;
             }
             const uint32_t take = n_idle < pool_end - pool ? n_idle : pool_end - pool;
             if (idle) {
@@ -593,6 +592,7 @@ __global__ __launch_bounds__(EXT_BLOCK) void k_extend_pers(KParams P, int depth,
     st.sp = 0;
     GLOBAL DCtl* ctl = gpw(P.ctl);
     const uint32_t count = ctl->count[depth];
+    const uint32_t fetch = (uint32_t)P.ext_fetch;
     const GLOBAL float4* ino = gp(P.in.o);
     const GLOBAL float4* ind = gp(P.in.d);
     const GLOBAL DReq* reqs = gp(P.chunk)->reqs;
@@ -613,10 +613,10 @@ __global__ __launch_bounds__(EXT_BLOCK) void k_extend_pers(KParams P, int depth,
         } else if (n_idle >= refill_min) {
             if (pool == pool_end) {
                 uint32_t base = 0;
-                if (lane == 0) base = atomicAdd(&ctl->fetch[depth], (uint32_t)EXT_FETCH);
+                if (lane == 0) base = atomicAdd(&ctl->fetch[depth], fetch);
                 base = __shfl(base, 0);
                 pool = base < count ? base : count;
-                pool_end = base + EXT_FETCH < count ? base + EXT_FETCH : count;
+                pool_end = base + fetch < count ? base + fetch : count;
             }
             const uint32_t take = n_idle < pool_end - pool ? n_idle : pool_end - pool;
             if (idle) {

@@ -189,8 +189,9 @@ typedef struct ilm_opts {
     int32_t verbose;
     /* Path order inside a chunk (results are identical for every choice; only coherence
      * changes): sample_group consecutive path ids take consecutive samples of one pixel (a
-     * power of two, 0 or 1 = pixel-major), tile_rows > 1 walks the image in bands of that many
-     * rows, column by column, so a wave covers a compact block of pixels. */
+     * power of two; 1 = pixel-major; 0 = 64), tile_rows > 1 walks the image in bands of that many
+     * rows, column by column, so a wave covers a compact block of pixels (1 = row order;
+     * 0 = 8). */
     int32_t sample_group;
     int32_t tile_rows;
     /* at most this many passes of one request per chunk (0 = as many as paths_in_flight
@@ -202,6 +203,8 @@ typedef struct ilm_opts {
     const char* bvh_cache_dir;
     /* extend modes 1 and 2: number of idle lanes of a wave that triggers a refill (0 = 8) */
     int32_t refill_min;
+    /* rays an extend wave reserves per queue atomic (multiple of 64; 0 = 256) */
+    int32_t ext_fetch;
 } ilm_opts;
 
 typedef struct ilm_render_info {
@@ -260,8 +263,8 @@ void ilm_close(ilm_ctx* ctx);
 const char* ilm_last_error(ilm_ctx* ctx);
 int ilm_stats_get(ilm_ctx* ctx, ilm_stats* out);
 /* Retune an idle context (no counterpart in the reference, whose launch shape is fixed):
- * extend_mode, sample_group, tile_rows, chunk_passes and paths_in_flight take the values in
- * *opts; other fields are ignored. ILM_E_BUSY while served requests are queued or running.
+ * extend_mode, refill_min, ext_fetch, sample_group, tile_rows, chunk_passes and paths_in_flight
+ * take the values in *opts; other fields are ignored. ILM_E_BUSY while served requests are queued or running.
  * The autotuner (illume_amd/autotune.py) drives this between short trial renders. */
 int ilm_tune(ilm_ctx* ctx, const ilm_opts* opts);
 
