@@ -767,9 +767,10 @@ static void build_chunk(ilm_ctx* ctx, const std::vector<Job*>& jobs, DChunk& c, 
             if (j->next_px >= wh) { j->next_px = 0; j->next_pass += 1; }
         }
         w.sgroup = 1;
-        if (w.px0 == 0)
+        if (w.px0 == 0) {
             const uint32_t sg_max = ctx->opts.sample_group > 0 ? (uint32_t)ctx->opts.sample_group : 64u;
             while (w.sgroup * 2 <= sg_max && w.npass % (w.sgroup * 2) == 0) w.sgroup *= 2;
+        }
         w.pid_base = c.total_paths;
         w.acc_base = c.total_px;
         c.total_paths += (uint32_t)((int64_t)w.npass * w.npx);
