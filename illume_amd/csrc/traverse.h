@@ -22,6 +22,16 @@
 #endif
 #define OVF_STACK (128 - LDS_STACK)  /* spill entries per lane (spilling variant) */
 #define STACK_SENTINEL ((int)0x7ffffffe)
+/* Minimum resident one-wave workgroups per CU the register allocation must allow (0: no
+ * constraint); with LDS_STACK it sets the extend kernels' occupancy. */
+#ifndef EXT_MIN_BLOCKS
+#define EXT_MIN_BLOCKS 0
+#endif
+#if EXT_MIN_BLOCKS > 0
+#define EXT_BOUNDS __launch_bounds__(EXT_BLOCK) __attribute__((amdgpu_waves_per_eu(EXT_MIN_BLOCKS / 4, 8)))
+#else
+#define EXT_BOUNDS __launch_bounds__(EXT_BLOCK)
+#endif
 
 /* conservative slab-test margins (relative); cover rcp and fma rounding of the box test */
 #define BOX_LO 0.99999f
@@ -295,7 +305,7 @@ __device__ inline void trace_ray(const GLOBAL DScene* S, V3 O, V3 D, Best& B, St
  * one atomic (same-address atomics serialise, so one per 64-ray batch throttles the queue) and
  * walks them as consecutive 64-ray batches, each to completion. */
 template <int TRI, int QUANT, int COUNT, int OVF>
-__global__ __launch_bounds__(EXT_BLOCK) void k_extend(KParams P, int depth, int* ovf_base) {
+__global__ EXT_BOUNDS void k_extend(KParams P, int depth, int* ovf_base) {
     __shared__ int stk[LDS_STACK * EXT_BLOCK];
     const int lane = threadIdx.x;
     Stack<OVF> st;
@@ -429,7 +439,7 @@ __device__ inline bool seg_run(TState& T, Best& B, Stack<OVF>& st, const GLOBAL 
 }
 
 template <int TRI, int QUANT, int COUNT, int OVF>
-__global__ __launch_bounds__(EXT_BLOCK) void k_extend_refill(KParams P, int depth, int* ovf_base) {
+__global__ EXT_BOUNDS void k_extend_refill(KParams P, int depth, int* ovf_base) {
     __shared__ int stk[LDS_STACK * EXT_BLOCK];
     const int lane = threadIdx.x;
     Stack<OVF> st;
@@ -583,7 +593,7 @@ __device__ inline bool pstep(PState& T, Best& B, Stack<OVF>& st, TravCount* tc) 
 }
 
 template <int TRI, int QUANT, int COUNT, int OVF>
-__global__ __launch_bounds__(EXT_BLOCK) void k_extend_pers(KParams P, int depth, int* ovf_base) {
+__global__ EXT_BOUNDS void k_extend_pers(KParams P, int depth, int* ovf_base) {
     __shared__ int stk[LDS_STACK * EXT_BLOCK];
     const int lane = threadIdx.x;
     Stack<OVF> st;
