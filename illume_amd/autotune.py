@@ -22,14 +22,28 @@ of it), the best of `repeats` runs.
 """
 import time
 
+# A knob is a setting name, or a tuple of names searched jointly (refill_min only matters for
+# the refill modes, so it is searched together with extend_mode).
 DEFAULT_SPACE = (
     ("sample_group", (1, 4, 16, 64)),
     ("tile_rows", (1, 4, 8)),
-    ("extend_mode", (0, 1, 2)),
-    ("refill_min", (4, 8, 16, 32)),
+    (("extend_mode", "refill_min"), ((0, 0), (1, 16), (1, 32), (2, 8), (2, 16), (2, 32), (2, 48))),
     ("ext_fetch", (128, 256, 512, 1024, 2048)),
     ("paths_in_flight", (1 << 24, 1 << 25, 1 << 26)),
 )
+
+
+def _get(cur, knob):
+    return tuple(cur[k] for k in knob) if isinstance(knob, tuple) else cur[knob]
+
+
+def _set(cur, knob, v):
+    out = dict(cur)
+    if isinstance(knob, tuple):
+        out.update(zip(knob, v))
+    else:
+        out[knob] = v
+    return out
 
 
 def _measure(ctx, run, repeats):
@@ -55,10 +69,9 @@ def autotune(ctx, run, space=DEFAULT_SPACE, repeats=2, log=None, budget_s=60.0):
         for v in values:
             if time.time() - t0 > budget_s:
                 break
-            if v == cur[knob]:
+            if v == _get(cur, knob):
                 continue
-            cand = dict(cur)
-            cand[knob] = v
+            cand = _set(cur, knob, v)
             ctx.tune(**cand)
             run(ctx)
             rate = _measure(ctx, run, repeats)

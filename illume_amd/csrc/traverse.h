@@ -319,12 +319,19 @@ __global__ EXT_BOUNDS void k_extend(KParams P, int depth, int* ovf_base) {
     const GLOBAL float4* ind = gp(P.in.d);
     const GLOBAL DReq* reqs = gp(P.chunk)->reqs;
     TravCount tc = {0u, 0u};
+    /* guided self-scheduling: reservations shrink with the rays left in the queue (down to one
+     * batch), so the last waves of a launch finish together; ext_fetch is the upper bound */
+    const uint32_t guide = gridDim.x * 2u;
+    uint32_t want = fetch;
     for (;;) {
         uint32_t base = 0;
-        if (lane == 0) base = atomicAdd(&ctl->fetch[depth], fetch);
+        if (lane == 0) base = atomicAdd(&ctl->fetch[depth], want);
         base = __shfl(base, 0);
         if (base >= count) break;
-        const uint32_t end = base + fetch < count ? base + fetch : count;
+        const uint32_t end = base + want < count ? base + want : count;
+        const uint32_t left = count - end;
+        const uint32_t g = (left / guide) & ~(uint32_t)(EXT_BLOCK - 1);
+        want = g < (uint32_t)EXT_BLOCK ? (uint32_t)EXT_BLOCK : (g > fetch ? fetch : g);
         for (uint32_t b = base; b < end; b += EXT_BLOCK) {
             const uint32_t i = b + lane;
             if (i < end) {
