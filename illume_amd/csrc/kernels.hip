@@ -457,12 +457,24 @@ __global__ __launch_bounds__(256) void k_accumulate(KParams P) {
         float4 s = acc[p], c = cmp[p];
         const int npass = W->npass;
         const uint32_t npx = W->npx, pid0 = W->pid_base, sg = W->sgroup;
-        for (int k = 0; k < npass; k++) {
-            float4 x = contrib[pid0 + slice_local((uint32_t)k, q, npx, sg)];
-            float y, t;
-            y = x.x - c.x; t = s.x + y; c.x = (t - s.x) - y; s.x = t;
-            y = x.y - c.y; t = s.y + y; c.y = (t - s.y) - y; s.y = t;
-            y = x.z - c.z; t = s.z + y; c.z = (t - s.z) - y; s.z = t;
+        /* eight loads in flight before the dependent Kahan chain: with samples grouped per
+         * pixel each lane walks its own run of contributions, and issuing a whole 128-byte line's
+         * worth at once keeps those lines from being evicted between single loads */
+        for (int k0 = 0; k0 < npass; k0 += 8) {
+            float4 xs[8];
+#pragma unroll
+            for (int j = 0; j < 8; j++)
+                if (k0 + j < npass) xs[j] = contrib[pid0 + slice_local((uint32_t)(k0 + j), q, npx, sg)];
+                else xs[j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                if (k0 + j >= npass) break;
+                const float4 x = xs[j];
+                float y, t;
+                y = x.x - c.x; t = s.x + y; c.x = (t - s.x) - y; s.x = t;
+                y = x.y - c.y; t = s.y + y; c.y = (t - s.y) - y; s.y = t;
+                y = x.z - c.z; t = s.z + y; c.z = (t - s.z) - y; s.z = t;
+            }
         }
         acc[p] = s;
         cmp[p] = c;
