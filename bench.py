@@ -468,7 +468,8 @@ def main():
     ap.add_argument("--layout", default="fp32", choices=["fp32", "quant8"])
     ap.add_argument("--tri", default="mt", choices=["mt", "watertight"])
     ap.add_argument("--paths", type=int, default=0, help="paths in flight per chunk (0: library default)")
-    ap.add_argument("--extend-mode", type=int, default=0, help="0: one ray batch per wave, 1: lane refill")
+    ap.add_argument("--extend-mode", type=int, default=0,
+                    help="0: library default (per-step refill), 1: one ray batch per wave, 2: refill at leaves, 3: per-step refill")
     ap.add_argument("--sample-group", type=int, default=None)
     ap.add_argument("--tile-rows", type=int, default=None)
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (library default launch shape)")

@@ -4,9 +4,9 @@ The tunables change how paths are laid out and queued, never what is computed: e
 gives bit-identical images (samples are summed per pixel in pass order whatever the path order
 or chunking), so the tuner only measures time.
 
-    extend_mode      0: each wave traces one batch of 64 rays to completion
-                     1: idle lanes refill from a wave-local queue at BLAS leaves
-                     2: idle lanes refill after every traversal step
+    extend_mode      1: each wave traces one batch of 64 rays to completion
+                     2: idle lanes refill from a wave-local queue at BLAS leaves
+                     3 (and 0, the default): idle lanes refill after every traversal step
     refill_min       idle lanes that trigger a refill (modes 1 and 2)
     ext_fetch        rays an extend wave reserves per queue atomic (work-queue granularity)
     sample_group     consecutive path ids per pixel (ray coherence of the first bounces)
@@ -27,7 +27,7 @@ import time
 DEFAULT_SPACE = (
     ("sample_group", (1, 4, 16, 64)),
     ("tile_rows", (1, 4, 8)),
-    (("extend_mode", "refill_min"), ((0, 0), (1, 16), (1, 32), (2, 8), (2, 16), (2, 32), (2, 48))),
+    (("extend_mode", "refill_min"), ((1, 0), (2, 16), (2, 32), (3, 8), (3, 16), (3, 48))),
     ("ext_fetch", (128, 256, 512, 1024, 2048)),
     ("paths_in_flight", (1 << 24, 1 << 25, 1 << 26)),
 )

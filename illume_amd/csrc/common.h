@@ -213,7 +213,7 @@ struct KParams {
     unsigned long long* trav_stats;   /* [0] child-box tests, [1] triangle tests, or null */
     int bvh_layout;
     int tri_test;
-    int extend_mode;
+    int extend_mode;        /* ILM_EXTEND_BATCH, _LEAF_REFILL or _STEP_REFILL */
     int use_ovf;            /* 1 when a scene's stack bound exceeds the LDS stack depth */
     int tile_rows;          /* pixel order: bands of this many rows walked column-major (1 = rows) */
     int refill_min;         /* extend modes 1-2: idle lanes that trigger a refill (1..64) */

@@ -103,6 +103,7 @@ struct ilm_ctx {
     ilm_opts opts;
     int n_cu = 0;
     int ext_grid = 0, shade_grid = 0, gen_grid = 0;
+    int ovf_grid = 0;               /* extend grid the spill area (ovf) is sized for, >= ext_grid */
     std::chrono::steady_clock::time_point t0;
 
     /* device buffers */

@@ -426,9 +426,14 @@ static int ctx_alloc_paths(ilm_ctx* ctx, int64_t cap) {
 static bool opts_valid(const ilm_opts& o) {
     const bool sg_ok = o.sample_group >= 0 && o.sample_group <= 4096 && (o.sample_group & (o.sample_group - 1)) == 0;
     return !(o.bvh_layout < 0 || o.bvh_layout > ILM_BVH_QUANT8 || o.tri_test < 0 || o.tri_test > ILM_TRI_WATERTIGHT ||
-             o.extend_mode < 0 || o.extend_mode > 2 || !sg_ok || o.tile_rows < 0 || o.tile_rows > 256 ||
+             o.extend_mode < 0 || o.extend_mode > ILM_EXTEND_STEP_REFILL || !sg_ok || o.tile_rows < 0 || o.tile_rows > 256 ||
              o.refill_min < 0 || o.refill_min > 64 || o.ext_fetch < 0 || o.ext_fetch > 65536 || (o.ext_fetch & 63) != 0 ||
              o.chunk_passes < 0 || o.paths_in_flight < 0 || o.hbm_budget_bytes < 0 || o.max_queued < 0);
+}
+
+/* extend scheduling actually used (ILM_EXTEND_DEFAULT is the per-step refill) */
+static int ext_mode_of(const ilm_opts& o) {
+    return o.extend_mode == ILM_EXTEND_DEFAULT ? ILM_EXTEND_STEP_REFILL : o.extend_mode;
 }
 
 static int64_t cap_of(const ilm_opts& o) {
@@ -440,7 +445,7 @@ static int64_t cap_of(const ilm_opts& o) {
 }
 
 static size_t ovf_bytes(const ilm_ctx* ctx) {
-    return (size_t)ctx->ext_grid * ilm_k_ext_block() * ilm_k_ovf_stack() * sizeof(int);
+    return (size_t)ctx->ovf_grid * ilm_k_ext_block() * ilm_k_ovf_stack() * sizeof(int);
 }
 
 int ilm_open(int device, const ilm_opts* opts, ilm_ctx** out) {
@@ -465,7 +470,7 @@ int ilm_open(int device, const ilm_opts* opts, ilm_ctx** out) {
     HIPTRY(ctx, hipGetDeviceProperties(&prop, device));
     ctx->n_cu = prop.multiProcessorCount;
     HIPTRY(ctx, hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
-    int occ = ilm_k_extend_occupancy(ctx->opts.tri_test, ctx->opts.bvh_layout, ctx->opts.extend_mode);
+    int occ = ilm_k_extend_occupancy(ctx->opts.tri_test, ctx->opts.bvh_layout, ext_mode_of(ctx->opts));
     ctx->ext_grid = ctx->n_cu * std::max(1, occ);
     ctx->gen_grid = ctx->n_cu * 8;
     ctx->shade_grid = ctx->n_cu * 4;   /* 512-thread blocks, block-stride loop */
@@ -476,6 +481,7 @@ int ilm_open(int device, const ilm_opts* opts, ilm_ctx** out) {
         if (e2 == hipSuccess) { ctx->allocs.push_back(*p); ctx->own_bytes += (int64_t)b; }
         return e2;
     };
+    ctx->ovf_grid = ctx->ext_grid;
     HIPTRY(ctx, hipMalloc((void**)&ctx->ovf, ovf_bytes(ctx)));
     ctx->own_bytes += (int64_t)ovf_bytes(ctx);
     HIPTRY(ctx, A((void**)&ctx->ctl, sizeof(DCtl)));
@@ -548,21 +554,23 @@ int ilm_tune(ilm_ctx* ctx, const ilm_opts* o) {
     HIPTRY(ctx, hipStreamSynchronize(ctx->stream));
     for (auto& kv : ctx->graphs) (void)hipGraphExecDestroy(kv.second);
     ctx->graphs.clear();
-    if (o->extend_mode != ctx->opts.extend_mode) {
-        const int occ = ilm_k_extend_occupancy(ctx->opts.tri_test, ctx->opts.bvh_layout, o->extend_mode);
+    if (ext_mode_of(*o) != ext_mode_of(ctx->opts)) {
+        const int occ = ilm_k_extend_occupancy(ctx->opts.tri_test, ctx->opts.bvh_layout, ext_mode_of(*o));
         const int grid = ctx->n_cu * std::max(1, occ);
-        if (grid > ctx->ext_grid) {
-            const size_t old_b = ovf_bytes(ctx);
+        if (grid > ctx->ovf_grid) {
+            /* the spill area is sized for the grid: grow it before the grid grows (on failure
+             * the context keeps its old mode, grid and buffer) */
+            const size_t nb = (size_t)grid * ilm_k_ext_block() * ilm_k_ovf_stack() * sizeof(int);
+            int* fresh = nullptr;
+            HIPTRY(ctx, hipMalloc((void**)&fresh, nb));
             (void)hipFree(ctx->ovf);
-            ctx->ovf = nullptr;
-            ctx->own_bytes -= (int64_t)old_b;
-            ctx->ext_grid = grid;
-            HIPTRY(ctx, hipMalloc((void**)&ctx->ovf, ovf_bytes(ctx)));
-            ctx->own_bytes += (int64_t)ovf_bytes(ctx);
+            ctx->own_bytes += (int64_t)nb - (int64_t)ovf_bytes(ctx);
+            ctx->ovf = fresh;
+            ctx->ovf_grid = grid;
         }
         ctx->ext_grid = grid;
-        ctx->opts.extend_mode = o->extend_mode;
     }
+    ctx->opts.extend_mode = o->extend_mode;
     ctx->opts.sample_group = o->sample_group;
     ctx->opts.tile_rows = o->tile_rows;
     ctx->opts.refill_min = o->refill_min;
@@ -613,9 +621,9 @@ static KParams base_params(ilm_ctx* ctx) {
     P.contrib = ctx->contrib;
     P.bvh_layout = ctx->opts.bvh_layout;
     P.tri_test = ctx->opts.tri_test;
-    P.extend_mode = ctx->opts.extend_mode;
-    P.tile_rows = ctx->opts.tile_rows > 0 ? ctx->opts.tile_rows : 8;
-    P.refill_min = ctx->opts.refill_min > 0 ? ctx->opts.refill_min : 8;
+    P.extend_mode = ext_mode_of(ctx->opts);
+    P.tile_rows = ctx->opts.tile_rows > 0 ? ctx->opts.tile_rows : 4;
+    P.refill_min = ctx->opts.refill_min > 0 ? ctx->opts.refill_min : 32;
     P.ext_fetch = ctx->opts.ext_fetch > 0 ? ctx->opts.ext_fetch : 256;
     P.trav_stats = ctx->opts.count_tests ? ctx->d_trav : nullptr;
     return P;

@@ -535,9 +535,9 @@ hipError_t ilm_k_raygen(const KParams& P, int grid, hipStream_t s) {
 
 template <int T, int Q, int C, int O>
 static void launch_extend(const KParams& P, int depth, int* ovf, int grid, hipStream_t s) {
-    if (P.extend_mode == 2)
+    if (P.extend_mode == ILM_EXTEND_STEP_REFILL)
         ILM_LAUNCH((k_extend_pers<T, Q, C, O>), dim3(grid), dim3(EXT_BLOCK), 0, s, P, depth, ovf);
-    else if (P.extend_mode == 1)
+    else if (P.extend_mode == ILM_EXTEND_LEAF_REFILL)
         ILM_LAUNCH((k_extend_refill<T, Q, C, O>), dim3(grid), dim3(EXT_BLOCK), 0, s, P, depth, ovf);
     else
         ILM_LAUNCH((k_extend<T, Q, C, O>), dim3(grid), dim3(EXT_BLOCK), 0, s, P, depth, ovf);
@@ -596,12 +596,12 @@ static int occ_of(K kernel) {
 /* Blocks per CU of the extend kernel (the smallest over the variants a context may launch). */
 int ilm_k_extend_occupancy(int tri_test, int bvh_layout, int extend_mode) {
     int a, b;
-    if (extend_mode == 2) {
+    if (extend_mode == ILM_EXTEND_STEP_REFILL) {
         if (!tri_test && !bvh_layout) { a = occ_of(k_extend_pers<0, 0, 0, 0>); b = occ_of(k_extend_pers<0, 0, 0, 1>); }
         else if (tri_test && !bvh_layout) { a = occ_of(k_extend_pers<1, 0, 0, 0>); b = occ_of(k_extend_pers<1, 0, 0, 1>); }
         else if (!tri_test) { a = occ_of(k_extend_pers<0, 1, 0, 0>); b = occ_of(k_extend_pers<0, 1, 0, 1>); }
         else { a = occ_of(k_extend_pers<1, 1, 0, 0>); b = occ_of(k_extend_pers<1, 1, 0, 1>); }
-    } else if (extend_mode == 1) {
+    } else if (extend_mode == ILM_EXTEND_LEAF_REFILL) {
         if (!tri_test && !bvh_layout) { a = occ_of(k_extend_refill<0, 0, 0, 0>); b = occ_of(k_extend_refill<0, 0, 0, 1>); }
         else if (tri_test && !bvh_layout) { a = occ_of(k_extend_refill<1, 0, 0, 0>); b = occ_of(k_extend_refill<1, 0, 0, 1>); }
         else if (!tri_test) { a = occ_of(k_extend_refill<0, 1, 0, 0>); b = occ_of(k_extend_refill<0, 1, 0, 1>); }

@@ -18,14 +18,14 @@
 
 #define EXT_BLOCK 64            /* one wave per workgroup: the LDS stack belongs to one wave */
 #ifndef LDS_STACK
-#define LDS_STACK 32            /* stack entries per lane kept in LDS */
+#define LDS_STACK 20            /* stack entries per lane kept in LDS (5 KB per wave) */
 #endif
 #define OVF_STACK (128 - LDS_STACK)  /* spill entries per lane (spilling variant) */
 #define STACK_SENTINEL ((int)0x7ffffffe)
 /* Minimum resident one-wave workgroups per CU the register allocation must allow (0: no
  * constraint); with LDS_STACK it sets the extend kernels' occupancy. */
 #ifndef EXT_MIN_BLOCKS
-#define EXT_MIN_BLOCKS 0
+#define EXT_MIN_BLOCKS 28       /* 7 waves per SIMD: registers capped at 72 */
 #endif
 #if EXT_MIN_BLOCKS > 0
 #define EXT_BOUNDS __launch_bounds__(EXT_BLOCK) __attribute__((amdgpu_waves_per_eu(EXT_MIN_BLOCKS / 4, 8)))

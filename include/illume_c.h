@@ -175,6 +175,11 @@ typedef struct ilm_pixel {
     uint64_t r, g, b;
 } ilm_pixel;
 
+/* Extend-kernel scheduling (results are identical for every mode). BATCH: each wave traces 64
+ * rays to completion; LEAF_REFILL: idle lanes take new rays when the wave reaches BLAS leaves;
+ * STEP_REFILL: idle lanes take new rays after every traversal step (the default). */
+enum { ILM_EXTEND_DEFAULT = 0, ILM_EXTEND_BATCH = 1, ILM_EXTEND_LEAF_REFILL = 2, ILM_EXTEND_STEP_REFILL = 3 };
+
 typedef struct ilm_opts {
     int32_t bvh_layout;         /* ILM_BVH_* (default FP32) */
     int32_t tri_test;           /* ILM_TRI_* (default Moller-Trumbore) */
@@ -182,8 +187,7 @@ typedef struct ilm_opts {
     int64_t hbm_budget_bytes;   /* serving admission budget; 0 = 90% of free device memory */
     int32_t use_graphs;         /* capture the per-chunk pipeline as a hipGraph */
     int32_t max_queued;         /* serving: admission queue bound (0 = 4096) */
-    int32_t extend_mode;        /* 0: one coherent ray batch per wave; 1: idle lanes refill at BLAS
-                                   leaves; 2: idle lanes refill after every traversal step */
+    int32_t extend_mode;        /* ILM_EXTEND_*: how extend waves schedule their rays */
     int32_t profile;            /* time every extend launch with HIP events (disables graphs) */
     int32_t count_tests;        /* count box/triangle tests of the extend kernel (slower) */
     int32_t verbose;
@@ -191,7 +195,7 @@ typedef struct ilm_opts {
      * changes): sample_group consecutive path ids take consecutive samples of one pixel (a
      * power of two; 1 = pixel-major; 0 = 64), tile_rows > 1 walks the image in bands of that many
      * rows, column by column, so a wave covers a compact block of pixels (1 = row order;
-     * 0 = 8). */
+     * 0 = 4). */
     int32_t sample_group;
     int32_t tile_rows;
     /* at most this many passes of one request per chunk (0 = as many as paths_in_flight
@@ -201,7 +205,7 @@ typedef struct ilm_opts {
      * device layouts; replaces the reference's unversioned .bvh files, src/accel/bvh.cpp:399-439),
      * or NULL for none. The string is copied at ilm_open. */
     const char* bvh_cache_dir;
-    /* extend modes 1 and 2: number of idle lanes of a wave that triggers a refill (0 = 8) */
+    /* refill modes: number of idle lanes of a wave that triggers a refill (0 = 32) */
     int32_t refill_min;
     /* rays an extend wave reserves per queue atomic (multiple of 64; 0 = 256) */
     int32_t ext_fetch;

@@ -79,7 +79,7 @@ def test_closest_hit_gold_100k(ctx, gold):
     assert t_exact
 
 
-@pytest.mark.parametrize("mode", [1, 2])
+@pytest.mark.parametrize("mode", [1, 2, 3])
 def test_closest_hit_refill_extend_modes(gold, mode):
     import illume_amd
     spec, st, _, orc = gold
