@@ -201,23 +201,29 @@ __device__ inline bool inner_step(const GLOBAL BNode* nodes, const GLOBAL QNode*
         box4q(qnodes + node, id, noid, tmax, tn, hm, nc, ch);
         if (COUNT) tc->boxes += nc;
         if (hm == 0) return false;
-        /* visit nearest first: push the others far-to-near */
-        int best = -1;
-        float bt = 0.0f;
-#pragma unroll
-        for (int k = 0; k < 4; k++)
-            if ((hm >> k) & 1u) { if (best < 0 || tn[k] < bt) { best = k; bt = tn[k]; } }
-        uint32_t rest = hm & ~(1u << best);
-        while (rest) {
-            int far = -1;
-            float ft = 0.0f;
-#pragma unroll
-            for (int k = 0; k < 4; k++)
-                if ((rest >> k) & 1u) { if (far < 0 || tn[k] > ft) { far = k; ft = tn[k]; } }
-            st.push(far == 0 ? ch.x : far == 1 ? ch.y : far == 2 ? ch.z : ch.w);
-            rest &= ~(1u << far);
+        /* order the hit children near to far with a 4-input sorting network (missed children
+         * sort last as +inf), descend into the nearest and push the rest far-to-near: no
+         * data-dependent loops, so the lanes of a wave stay together */
+        float k0 = (hm & 1u) ? tn[0] : __builtin_inff(), k1 = (hm & 2u) ? tn[1] : __builtin_inff();
+        float k2 = (hm & 4u) ? tn[2] : __builtin_inff(), k3 = (hm & 8u) ? tn[3] : __builtin_inff();
+        int c0 = ch.x, c1 = ch.y, c2 = ch.z, c3 = ch.w;
+#define ILM_CE(ka, ca, kb, cb)                                        \
+        {                                                             \
+            const bool sw = kb < ka;                                  \
+            const float tk = sw ? kb : ka; kb = sw ? ka : kb; ka = tk; \
+            const int tc_ = sw ? cb : ca; cb = sw ? ca : cb; ca = tc_; \
         }
-        node = best == 0 ? ch.x : best == 1 ? ch.y : best == 2 ? ch.z : ch.w;
+        ILM_CE(k0, c0, k1, c1)
+        ILM_CE(k2, c2, k3, c3)
+        ILM_CE(k0, c0, k2, c2)
+        ILM_CE(k1, c1, k3, c3)
+        ILM_CE(k1, c1, k2, c2)
+#undef ILM_CE
+        const int nhit = __builtin_popcount(hm);
+        if (nhit > 3) st.push(c3);
+        if (nhit > 2) st.push(c2);
+        if (nhit > 1) st.push(c1);
+        node = c0;
         return true;
     } else {
         const GLOBAL BNode* n = nodes + node;
