@@ -24,6 +24,11 @@
 #define STACK_SENTINEL ((int)0x7ffffffe)
 /* Minimum resident one-wave workgroups per CU the register allocation must allow (0: no
  * constraint); with LDS_STACK it sets the extend kernels' occupancy. */
+/* Per-step refill: leaf trips run when waiting-at-leaf lanes * EXT_PHASE_W >= other busy lanes
+ * (0: every trip serves every lane) */
+#ifndef EXT_PHASE_W
+#define EXT_PHASE_W 0
+#endif
 #ifndef EXT_MIN_BLOCKS
 #define EXT_MIN_BLOCKS 28       /* 7 waves per SIMD: registers capped at 72 */
 #endif
@@ -655,7 +660,19 @@ __global__ EXT_BOUNDS void k_extend_pers(KParams P, int depth, int* ovf_base) {
             pool += take;
             if (pool == pool_end && pool_end >= count) drained = true;
         }
+#if EXT_PHASE_W > 0
+        /* leaf work and inner-node work alternate at wave level: a trip runs the BLAS-leaf
+         * triangle tests only once enough lanes wait at leaves, else only node visits, so a
+         * trip pays for one of the two code paths instead of both */
+        const bool has = ridx != NO_RAY;
+        const bool at_leaf = has && T.node < 0 && T.blas;
+        const uint32_t n_leaf = (uint32_t)__popcll(__ballot(at_leaf));
+        const uint32_t n_rest = (uint32_t)__popcll(__ballot(has)) - n_leaf;
+        const bool leaf_trip = n_leaf * (uint32_t)EXT_PHASE_W >= n_rest;
+        if (has && at_leaf == leaf_trip) {
+#else
         if (ridx != NO_RAY) {
+#endif
             if (pstep<TRI, QUANT, COUNT, OVF>(T, B, st, &tc)) {
                 write_hit(P, ridx, B);
                 ridx = NO_RAY;

@@ -30,6 +30,17 @@
  *     typo that vanishes for affine matrices
  * One deliberate deviation: get_bounds (src/primitives/mesh.cu:342-361) starts from an
  * uninitialised AABB; here it starts empty, the evident intent.
+ *
+ * How each part is pinned (tests/test_oracle.py, tests/test_bvh.py): PCG32 by its published
+ * known-answer sequence; wang_hash by its integer recipe; portable math against double libm;
+ * Fresnel, slab, sphere, Moller-Trumbore, tonemap, transforms and camera by closed forms and
+ * constructed cases; samplers by their textbook distributions; the medium stage by Beer-Lambert
+ * transmittance; traversal by brute force over every triangle.
+ * Parity unpinned (nothing but the reference's source text fixes them, and the reference cannot
+ * run here): the Henyey-Greenstein sinT = sqrt(1 - cosT) quirk (only cosT is checked), the
+ * swapped arguments of the plastic diffuse_fresnel beyond its closed-form value, texture
+ * bilinear weights between texel centres (CUDA tex2D's are not reproducible), and the
+ * reference's cuRAND stream (replaced by PCG32 with the reference's draw order).
  */
 #include "../include/illume_c.h"
 #include "../include/illume_pmath.h"
