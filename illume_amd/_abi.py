@@ -95,7 +95,8 @@ class Opts(C.Structure):
                 ("hbm_budget_bytes", C.c_int64), ("use_graphs", C.c_int32), ("max_queued", C.c_int32),
                 ("extend_mode", C.c_int32), ("profile", C.c_int32), ("count_tests", C.c_int32), ("verbose", C.c_int32),
                 ("sample_group", C.c_int32), ("tile_rows", C.c_int32), ("chunk_passes", C.c_int32),
-                ("bvh_cache_dir", C.c_char_p), ("refill_min", C.c_int32), ("ext_fetch", C.c_int32)]
+                ("bvh_cache_dir", C.c_char_p), ("refill_min", C.c_int32), ("ext_fetch", C.c_int32),
+                ("leaf_wait_pct", C.c_int32)]
 
 
 class RenderInfo(C.Structure):

@@ -24,11 +24,6 @@
 #define STACK_SENTINEL ((int)0x7ffffffe)
 /* Minimum resident one-wave workgroups per CU the register allocation must allow (0: no
  * constraint); with LDS_STACK it sets the extend kernels' occupancy. */
-/* Per-step refill: leaf trips run when waiting-at-leaf lanes * EXT_PHASE_W >= other busy lanes
- * (0: every trip serves every lane) */
-#ifndef EXT_PHASE_W
-#define EXT_PHASE_W 0
-#endif
 #ifndef EXT_MIN_BLOCKS
 #define EXT_MIN_BLOCKS 28       /* 7 waves per SIMD: registers capped at 72 */
 #endif
@@ -626,6 +621,7 @@ __global__ EXT_BOUNDS void k_extend_pers(KParams P, int depth, int* ovf_base) {
     const GLOBAL DReq* reqs = gp(P.chunk)->reqs;
     const unsigned long long lt_mask = (1ull << lane) - 1ull;
     const uint32_t refill_min = (uint32_t)P.refill_min;
+    const uint32_t leaf_pct = (uint32_t)P.leaf_pct;
     uint32_t ridx = NO_RAY;
     bool drained = false;
     uint32_t pool = 0, pool_end = 0;
@@ -660,19 +656,17 @@ __global__ EXT_BOUNDS void k_extend_pers(KParams P, int depth, int* ovf_base) {
             pool += take;
             if (pool == pool_end && pool_end >= count) drained = true;
         }
-#if EXT_PHASE_W > 0
         /* leaf work and inner-node work alternate at wave level: a trip runs the BLAS-leaf
-         * triangle tests only once enough lanes wait at leaves, else only node visits, so a
-         * trip pays for one of the two code paths instead of both */
+         * triangle tests only once the lanes waiting at leaves reach leaf_pct percent of the
+         * other busy lanes, else only node visits (and instance entries), so a trip pays for one
+         * of the two code paths instead of both. Either group is non-empty when chosen unless
+         * no lane is busy, so every trip makes progress. */
         const bool has = ridx != NO_RAY;
         const bool at_leaf = has && T.node < 0 && T.blas;
         const uint32_t n_leaf = (uint32_t)__popcll(__ballot(at_leaf));
         const uint32_t n_rest = (uint32_t)__popcll(__ballot(has)) - n_leaf;
-        const bool leaf_trip = n_leaf * (uint32_t)EXT_PHASE_W >= n_rest;
+        const bool leaf_trip = n_leaf * 100u >= n_rest * leaf_pct;
         if (has && at_leaf == leaf_trip) {
-#else
-        if (ridx != NO_RAY) {
-#endif
             if (pstep<TRI, QUANT, COUNT, OVF>(T, B, st, &tc)) {
                 write_hit(P, ridx, B);
                 ridx = NO_RAY;
