@@ -219,6 +219,7 @@ struct KParams {
     int refill_min;         /* extend modes 1-2: idle lanes that trigger a refill (1..64) */
     int ext_fetch;          /* rays reserved per queue atomic by an extend wave (multiple of 64) */
     int leaf_pct;           /* per-step refill: leaf trips when leaf lanes >= leaf_pct% of the others */
+    int shade_sort;         /* shade each block's paths in shading-key order (staged in LDS) */
 };
 
 /* Path id order of a slice: local index l <-> (sample k of the slice, pixel slot q). Samples

@@ -428,7 +428,7 @@ static bool opts_valid(const ilm_opts& o) {
     return !(o.bvh_layout < 0 || o.bvh_layout > ILM_BVH_QUANT8 || o.tri_test < 0 || o.tri_test > ILM_TRI_WATERTIGHT ||
              o.extend_mode < 0 || o.extend_mode > ILM_EXTEND_STEP_REFILL || !sg_ok || o.tile_rows < 0 || o.tile_rows > 256 ||
              o.refill_min < 0 || o.refill_min > 64 || o.ext_fetch < 0 || o.ext_fetch > 65536 || (o.ext_fetch & 63) != 0 ||
-             o.leaf_wait_pct < 0 || o.leaf_wait_pct > 10000 ||
+             o.leaf_wait_pct < 0 || o.leaf_wait_pct > 10000 || o.shade_sort < 0 || o.shade_sort > 2 ||
              o.chunk_passes < 0 || o.paths_in_flight < 0 || o.hbm_budget_bytes < 0 || o.max_queued < 0);
 }
 
@@ -577,6 +577,7 @@ int ilm_tune(ilm_ctx* ctx, const ilm_opts* o) {
     ctx->opts.refill_min = o->refill_min;
     ctx->opts.ext_fetch = o->ext_fetch;
     ctx->opts.leaf_wait_pct = o->leaf_wait_pct;
+    ctx->opts.shade_sort = o->shade_sort;
     ctx->opts.chunk_passes = o->chunk_passes;
     const int64_t cap = cap_of(*o);
     if (cap != ctx->cap) {
@@ -628,6 +629,7 @@ static KParams base_params(ilm_ctx* ctx) {
     P.refill_min = ctx->opts.refill_min > 0 ? ctx->opts.refill_min : 32;
     P.ext_fetch = ctx->opts.ext_fetch > 0 ? ctx->opts.ext_fetch : 256;
     P.leaf_pct = ctx->opts.leaf_wait_pct > 0 ? ctx->opts.leaf_wait_pct : 100;
+    P.shade_sort = ctx->opts.shade_sort == 2 ? 0 : 1;
     P.trav_stats = ctx->opts.count_tests ? ctx->d_trav : nullptr;
     return P;
 }

@@ -360,11 +360,30 @@ __device__ inline int shade_path(const GLOBAL DScene* S, const float ray_bias, c
     return ST_ALIVE;
 }
 
+/* Shading key for the block-local sort: paths inside a medium, misses, spheres and instances
+ * each get their own buckets (an instance stands in for its material), so a wave mostly runs
+ * one material's code. INVALID sorts the block's empty slots last. */
+#define SHADE_KEYS 17
+#define SHADE_KEY_INVALID 16
+__device__ inline uint32_t shade_key(int medium, int hinst) {
+    if (medium >= 0) return 15u;
+    if (hinst == -1) return 14u;
+    if (hinst < -1) return 9u + ((uint32_t)(-2 - hinst) & 3u);
+    return (uint32_t)hinst % 9u;
+}
+
+template <int SORT>
 __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(KParams P, int depth) {
     /* survivors of the block's 512 paths get their queue slots from one atomic: per-wave counts
      * meet in LDS, wave 0 lane 0 reserves the block's range, each wave takes its offset */
     __shared__ uint32_t wave_n[SHADE_WAVES];
     __shared__ uint32_t block_base;
+    /* SORT: the block's path states are staged in LDS and shaded in key order */
+    constexpr int SB = SORT ? SHADE_BLOCK : 1;
+    __shared__ float4 s_o[SB], s_d[SB], s_c[SB], s_h[SB], s_n[SB];
+    __shared__ uint2 s_r[SB];
+    __shared__ uint16_t s_ord[SB];
+    __shared__ uint32_t s_cnt[SHADE_KEYS];
     GLOBAL DCtl* ctl = gpw(P.ctl);
     const uint32_t count = ctl->count[depth];
     const int lane = threadIdx.x & 63;
@@ -384,14 +403,38 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(KParams P, int depth) {
      * same trips and reaches every barrier */
     for (uint32_t tbase = blockIdx.x * (uint32_t)SHADE_BLOCK; tbase < count; tbase += gridDim.x * (uint32_t)SHADE_BLOCK) {
         uint32_t i = tbase + threadIdx.x;
+        bool valid = i < count;
+        float4 o4 = make_float4(0, 0, 0, 0), d4 = o4, c4 = o4, h4 = o4, n4 = o4;
+        uint2 rg = make_uint2(0, 0);
+        if (valid) {
+            o4 = ino[i]; d4 = ind[i]; c4 = inc4[i];
+            rg = inr[i];
+            h4 = hh[i];
+            n4 = hn[i];
+        }
+        if (SORT) {
+            const uint32_t key = valid ? shade_key((int)(f2u(o4.w) & 0xffffu) - 1, __float_as_int(n4.w)) : SHADE_KEY_INVALID;
+            const int t = threadIdx.x;
+            s_o[t] = o4; s_d[t] = d4; s_c[t] = c4; s_h[t] = h4; s_n[t] = n4; s_r[t] = rg;
+            if (t < SHADE_KEYS) s_cnt[t] = 0u;
+            __syncthreads();
+            atomicAdd(&s_cnt[key], 1u);
+            __syncthreads();
+            if (t == 0) {
+                uint32_t run = 0;
+                for (int k = 0; k < SHADE_KEYS; k++) { const uint32_t c = s_cnt[k]; s_cnt[k] = run; run += c; }
+            }
+            __syncthreads();
+            s_ord[atomicAdd(&s_cnt[key], 1u)] = (uint16_t)t;
+            __syncthreads();
+            const int j = s_ord[t];
+            o4 = s_o[j]; d4 = s_d[j]; c4 = s_c[j]; h4 = s_h[j]; n4 = s_n[j]; rg = s_r[j];
+            valid = tbase + (uint32_t)j < count;
+        }
         int status = ST_DEAD;
         float4 oo = make_float4(0, 0, 0, 0), od = oo, oc = oo;
         uint2 org = make_uint2(0, 0);
-        if (i < count) {
-            float4 o4 = ino[i], d4 = ind[i], c4 = inc4[i];
-            uint2 rg = inr[i];
-            float4 h4 = hh[i];
-            float4 n4 = hn[i];
+        if (valid) {
             int hinst = __float_as_int(n4.w);
             uint32_t tag = f2u(o4.w);
             int req = (int)(tag >> 16);
@@ -426,7 +469,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(KParams P, int depth) {
         __syncthreads();
         uint32_t base = block_base;
         for (int w = 0; w < wid; w++) base += wave_n[w];
-        __syncthreads();   /* wave_n and block_base are rewritten by the next trip */
+        __syncthreads();   /* wave_n, block_base and the staging arrays are rewritten next trip */
         if (alive) {
             const uint32_t slot = base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
             outo[slot] = oo;
@@ -564,7 +607,8 @@ hipError_t ilm_k_extend(const KParams& P, int depth, int* ovf, int grid, hipStre
     return hipGetLastError();
 }
 hipError_t ilm_k_shade(const KParams& P, int depth, int grid, hipStream_t s) {
-    ILM_LAUNCH(k_shade, dim3(grid), dim3(SHADE_BLOCK), 0, s, P, depth);
+    if (P.shade_sort) ILM_LAUNCH(k_shade<1>, dim3(grid), dim3(SHADE_BLOCK), 0, s, P, depth);
+    else ILM_LAUNCH(k_shade<0>, dim3(grid), dim3(SHADE_BLOCK), 0, s, P, depth);
     return hipGetLastError();
 }
 hipError_t ilm_k_accumulate(const KParams& P, int grid, hipStream_t s) {

@@ -52,7 +52,8 @@ def test_graph_replays_match_direct_launches(scenes):
                                    dict(sample_group=2, tile_rows=3, paths_in_flight=3000),
                                    dict(extend_mode=1), dict(extend_mode=2, refill_min=4), dict(extend_mode=3, refill_min=1),
                                    dict(extend_mode=3, refill_min=48, sample_group=16),
-                                   dict(extend_mode=3, leaf_wait_pct=10), dict(extend_mode=3, leaf_wait_pct=5000)])
+                                   dict(extend_mode=3, leaf_wait_pct=10), dict(extend_mode=3, leaf_wait_pct=5000),
+                                   dict(shade_sort=2), dict(shade_sort=1, paths_in_flight=5000)])
 def test_path_order_leaves_results_unchanged(ctx, scenes, order):
     """Samples grouped per pixel and tiled pixel walks only reorder path ids: every pixel still
     receives the same samples, summed in pass order, so the images are bit-identical."""
