@@ -103,7 +103,9 @@ def test_served_requests_equal_plain_renders(ctx, scenes):
 
 def test_cancel_and_admission_limits(scenes):
     spec, st, _ = scenes["gold"]
-    big = _p(st, width=512, height=512, spp=256)
+    # long enough (512x512 x 4096 spp, about a billion segments) that a request is still queued
+    # or running when it is cancelled
+    big = _p(st, width=512, height=512, spp=4096)
     # a 512x512 request holds 512*512*56 B = 14.7 MB of HBM while admitted; the budget fits one
     with illume_amd.Context(0, max_queued=2, hbm_budget_bytes=20 * 2 ** 20) as c:
         s = c.scene(spec)
