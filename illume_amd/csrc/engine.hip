@@ -474,7 +474,8 @@ int ilm_open(int device, const ilm_opts* opts, ilm_ctx** out) {
     int occ = ilm_k_extend_occupancy(ctx->opts.tri_test, ctx->opts.bvh_layout, ext_mode_of(ctx->opts));
     ctx->ext_grid = ctx->n_cu * std::max(1, occ);
     ctx->gen_grid = ctx->n_cu * 8;
-    ctx->shade_grid = ctx->n_cu * 4;   /* 512-thread blocks, block-stride loop */
+    /* 512-thread blocks, block-stride loop: one resident wave of blocks, no stragglers */
+    ctx->shade_grid = ctx->n_cu * ilm_k_shade_occupancy(ctx->opts.shade_sort != 2);
     int rc = ctx_alloc_paths(ctx, cap_of(ctx->opts));
     if (rc) return rc;
     auto A = [&](void** p, size_t b) -> hipError_t {
@@ -578,6 +579,7 @@ int ilm_tune(ilm_ctx* ctx, const ilm_opts* o) {
     ctx->opts.ext_fetch = o->ext_fetch;
     ctx->opts.leaf_wait_pct = o->leaf_wait_pct;
     ctx->opts.shade_sort = o->shade_sort;
+    ctx->shade_grid = ctx->n_cu * ilm_k_shade_occupancy(ctx->opts.shade_sort != 2);
     ctx->opts.chunk_passes = o->chunk_passes;
     const int64_t cap = cap_of(*o);
     if (cap != ctx->cap) {

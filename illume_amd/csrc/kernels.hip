@@ -658,6 +658,13 @@ int ilm_k_extend_occupancy(int tri_test, int bvh_layout, int extend_mode) {
     }
     return a > b ? a : b;
 }
+/* Resident 512-thread shading blocks per CU (the LDS staging of the sorted variant limits it). */
+int ilm_k_shade_occupancy(int sort) {
+    int nb = 0;
+    hipError_t e = sort ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_shade<1>, SHADE_BLOCK, 0)
+                        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_shade<0>, SHADE_BLOCK, 0);
+    return (e != hipSuccess || nb <= 0) ? 2 : nb;
+}
 int ilm_k_ovf_stack() { return OVF_STACK; }
 int ilm_k_lds_stack() { return LDS_STACK; }
 int ilm_k_ext_block() { return EXT_BLOCK; }
