@@ -519,9 +519,18 @@ __global__ EXT_BOUNDS void k_extend_refill(KParams P, int depth, int* ovf_base) 
  * wave re-checks for idle lanes on every trip, so a lane whose ray finished takes a new one
  * after at most one trip instead of waiting for the rest of the wave to reach a leaf. The world
  * ray stays in registers for the return from an instance. Selected with extend_mode = 2. */
+/* PSTATE_LEAN: the reciprocal direction and -o/d are recomputed at each node visit from the
+ * current ray instead of being carried across loop trips (fewer live registers and fewer
+ * register copies on the hot path, for a few extra VALU operations per visit). */
+#ifndef PSTATE_LEAN
+#define PSTATE_LEAN 0
+#endif
 struct PState {
     V3 O, D;                /* world ray */
-    V3 o, d, id, noid;      /* ray in the current (world or object) space */
+    V3 o, d;                /* ray in the current (world or object) space */
+#if !PSTATE_LEAN
+    V3 id, noid;
+#endif
     const GLOBAL BNode* nodes;
     const GLOBAL QNode* qnodes;
     const GLOBAL float4* tris;
@@ -548,8 +557,10 @@ __device__ inline void pstep_init(PState& T, Best& B, Stack<OVF>& st, const GLOB
     T.D = D;
     T.o = O;
     T.d = D;
+#if !PSTATE_LEAN
     T.id = rcp3(D);
     T.noid = neg_o_over_d(O, T.id);
+#endif
     st.sp = 0;
 }
 
@@ -561,7 +572,13 @@ __device__ inline bool pstep(PState& T, Best& B, Stack<OVF>& st, TravCount* tc) 
     if (node == POP_MARK) {
         pop = true;
     } else if (node >= 0) {
+#if PSTATE_LEAN
+        const V3 id = rcp3(T.d);
+        const V3 noid = neg_o_over_d(T.o, id);
+        pop = !inner_step<QUANT, OVF, COUNT>(T.nodes, T.qnodes, node, id, noid, B.t, st, tc);
+#else
         pop = !inner_step<QUANT, OVF, COUNT>(T.nodes, T.qnodes, node, T.id, T.noid, B.t, st, tc);
+#endif
     } else if (!T.blas) {
         const int k = gp(T.S->tlas_prims)[ilm_leaf_first(node)];
         const GLOBAL DInst* I = gp(T.S->inst) + k;
@@ -572,8 +589,10 @@ __device__ inline bool pstep(PState& T, Best& B, Stack<OVF>& st, TravCount* tc) 
             load_rows12(I->inv, inv);
             T.o = xform3(inv, T.O, 1.0f);
             T.d = xform3(inv, T.D, 0.0f);
+#if !PSTATE_LEAN
             T.id = rcp3(T.d);
             T.noid = neg_o_over_d(T.o, T.id);
+#endif
             if (TRI) T.wt = wt_setup(T.d);
             T.nodes = gp(M->nodes);
             T.qnodes = gp(M->qnodes);
@@ -594,8 +613,10 @@ __device__ inline bool pstep(PState& T, Best& B, Stack<OVF>& st, TravCount* tc) 
             if (node != STACK_SENTINEL) break;
             T.o = T.O;
             T.d = T.D;
+#if !PSTATE_LEAN
             T.id = rcp3(T.D);
             T.noid = neg_o_over_d(T.O, T.id);
+#endif
             T.blas = 0;
             T.nodes = gp(T.S->tlas);
             T.qnodes = gp(T.S->qtlas);
