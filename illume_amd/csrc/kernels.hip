@@ -454,24 +454,45 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(KParams P, int depth) {
             oc = make_float4(color.x, color.y, color.z, u2f(pid));
             org = make_uint2((uint32_t)rs, (uint32_t)(rs >> 32));
         }
-        /* compaction: ballot + prefix within the wave, wave offsets within the block, one
-         * atomic per block */
         const bool alive = status == ST_ALIVE;
-        const unsigned long long mask = __ballot(alive);
-        const uint32_t n = (uint32_t)__popcll(mask);
-        if (lane == 0) wave_n[wid] = n;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            uint32_t total = 0;
-            for (int w = 0; w < SHADE_WAVES; w++) total += wave_n[w];
-            block_base = total ? atomicAdd(&ctl->count[depth + 1], total) : 0u;
+        uint32_t slot;
+        if (SORT) {
+            /* survivors leave the block grouped by direction octant, so the extend waves that
+             * take them 64 at a time trace rays from nearby origins in similar directions; one
+             * atomic reserves the block's range */
+            const int t = threadIdx.x;
+            const uint32_t okey = alive ? ((od.x < 0.0f ? 1u : 0u) | (od.y < 0.0f ? 2u : 0u) | (od.z < 0.0f ? 4u : 0u)) : 8u;
+            if (t < 9) s_cnt[t] = 0u;
+            __syncthreads();
+            atomicAdd(&s_cnt[okey], 1u);
+            __syncthreads();
+            if (t == 0) {
+                uint32_t run = 0;
+                for (int k = 0; k < 8; k++) { const uint32_t c = s_cnt[k]; s_cnt[k] = run; run += c; }
+                block_base = run ? atomicAdd(&ctl->count[depth + 1], run) : 0u;
+            }
+            __syncthreads();
+            slot = alive ? block_base + atomicAdd(&s_cnt[okey], 1u) : 0u;
+            __syncthreads();   /* s_cnt, block_base and the staging arrays are rewritten next trip */
+        } else {
+            /* compaction: ballot + prefix within the wave, wave offsets within the block, one
+             * atomic per block */
+            const unsigned long long mask = __ballot(alive);
+            const uint32_t n = (uint32_t)__popcll(mask);
+            if (lane == 0) wave_n[wid] = n;
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                uint32_t total = 0;
+                for (int w = 0; w < SHADE_WAVES; w++) total += wave_n[w];
+                block_base = total ? atomicAdd(&ctl->count[depth + 1], total) : 0u;
+            }
+            __syncthreads();
+            uint32_t base = block_base;
+            for (int w = 0; w < wid; w++) base += wave_n[w];
+            __syncthreads();   /* wave_n and block_base are rewritten by the next trip */
+            slot = base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
         }
-        __syncthreads();
-        uint32_t base = block_base;
-        for (int w = 0; w < wid; w++) base += wave_n[w];
-        __syncthreads();   /* wave_n, block_base and the staging arrays are rewritten next trip */
         if (alive) {
-            const uint32_t slot = base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
             outo[slot] = oo;
             outd[slot] = od;
             outc[slot] = oc;

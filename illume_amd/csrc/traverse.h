@@ -523,7 +523,7 @@ __global__ EXT_BOUNDS void k_extend_refill(KParams P, int depth, int* ovf_base) 
  * current ray instead of being carried across loop trips (fewer live registers and fewer
  * register copies on the hot path, for a few extra VALU operations per visit). */
 #ifndef PSTATE_LEAN
-#define PSTATE_LEAN 0
+#define PSTATE_LEAN 1
 #endif
 struct PState {
     V3 O, D;                /* world ray */
