@@ -172,6 +172,25 @@ __device__ inline void spheres_first(const GLOBAL DScene* S, V3 O, V3 D, Best& B
     }
 }
 
+/* Free-flight cap for a ray inside an active medium. The shading stage's first draw of the
+ * segment is the scatter distance s = -log(u) / sigma_s (src/renderer.cu:152-160), and the path
+ * scatters when s < t_hit, so hits beyond s never matter and the closest-hit query may stop
+ * there. The draw is made here from a copy of the RNG state (shade makes it again, unchanged);
+ * the cap is the next float above s so that a hit at exactly s is still found and shade's
+ * comparison decides as it would have. FLT_MAX when there is no cap (no medium, inactive
+ * medium, or s not finite). */
+__device__ inline float medium_tmax(const GLOBAL DScene* S, uint32_t tag, uint2 rg, uint32_t pixel) {
+    const int medium = (int)(tag & 0xffffu) - 1;
+    if (medium < 0) return ILM_FLT_MAX;
+    const GLOBAL DMedium* md = gp(S->mediums) + medium;
+    if (!md->active) return ILM_FLT_MAX;
+    uint64_t rs = ((uint64_t)rg.y << 32) | (uint64_t)rg.x;
+    const float u = ilm_pcg32_uniform(&rs, ilm_pcg32_inc(pixel));
+    const float sdist = -pm_logf(u) / md->scattering;
+    if (!(sdist < ILM_FLT_MAX)) return ILM_FLT_MAX;
+    return __uint_as_float(__float_as_uint(sdist) + 1u);   /* sdist >= 0 */
+}
+
 /* Intersect the triangles of BLAS leaf `node`. */
 template <int TRI, int COUNT>
 __device__ inline void leaf_tris(const GLOBAL float4* tris, int node, V3 o, V3 d, const WTRay& wt, int cur, Best& B,
@@ -247,8 +266,10 @@ __device__ inline bool inner_step(const GLOBAL BNode* nodes, const GLOBAL QNode*
 }
 
 template <int TRI, int QUANT, int COUNT, int OVF>
-__device__ inline void trace_ray(const GLOBAL DScene* S, V3 O, V3 D, Best& B, Stack<OVF>& st, TravCount* tc) {
+__device__ inline void trace_ray(const GLOBAL DScene* S, V3 O, V3 D, float tmax0, Best& B, Stack<OVF>& st,
+                                 TravCount* tc) {
     best_init(B);
+    B.t = tmax0;
     spheres_first(S, O, D, B);
     int node = QUANT ? S->qtlas_root : S->tlas_root;
     if (node == ILM_NO_CHILD) return;
@@ -323,6 +344,7 @@ __global__ EXT_BOUNDS void k_extend(KParams P, int depth, int* ovf_base) {
     const uint32_t fetch = (uint32_t)P.ext_fetch;
     const GLOBAL float4* ino = gp(P.in.o);
     const GLOBAL float4* ind = gp(P.in.d);
+    const GLOBAL uint2* inr = gp(P.in.rng);
     const GLOBAL DReq* reqs = gp(P.chunk)->reqs;
     TravCount tc = {0u, 0u};
     /* guided self-scheduling: reservations shrink with the rays left in the queue (down to one
@@ -344,8 +366,9 @@ __global__ EXT_BOUNDS void k_extend(KParams P, int depth, int* ovf_base) {
                 float4 o4 = ino[i];
                 float4 d4 = ind[i];
                 const GLOBAL DScene* S = gp(reqs[f2u(o4.w) >> 16].scene);
+                const float tmax0 = medium_tmax(S, f2u(o4.w), inr[i], f2u(d4.w));
                 Best B;
-                trace_ray<TRI, QUANT, COUNT, OVF>(S, mk3(o4.x, o4.y, o4.z), mk3(d4.x, d4.y, d4.z), B, st, &tc);
+                trace_ray<TRI, QUANT, COUNT, OVF>(S, mk3(o4.x, o4.y, o4.z), mk3(d4.x, d4.y, d4.z), tmax0, B, st, &tc);
                 write_hit(P, i, B);
             }
         }
@@ -378,8 +401,9 @@ struct TState {
 };
 
 template <int QUANT, int OVF>
-__device__ inline void seg_init(TState& T, Best& B, Stack<OVF>& st, const GLOBAL DScene* S, V3 O, V3 D) {
+__device__ inline void seg_init(TState& T, Best& B, Stack<OVF>& st, const GLOBAL DScene* S, V3 O, V3 D, float tmax0) {
     best_init(B);
+    B.t = tmax0;
     spheres_first(S, O, D, B);
     T.S = S;
     T.node = QUANT ? S->qtlas_root : S->tlas_root;
@@ -464,6 +488,7 @@ __global__ EXT_BOUNDS void k_extend_refill(KParams P, int depth, int* ovf_base) 
     const uint32_t fetch = (uint32_t)P.ext_fetch;
     const GLOBAL float4* ino = gp(P.in.o);
     const GLOBAL float4* ind = gp(P.in.d);
+    const GLOBAL uint2* inr = gp(P.in.rng);
     const GLOBAL DReq* reqs = gp(P.chunk)->reqs;
     const unsigned long long lt_mask = (1ull << lane) - 1ull;
     uint32_t ridx = NO_RAY;
@@ -493,7 +518,8 @@ __global__ EXT_BOUNDS void k_extend_refill(KParams P, int depth, int* ovf_base) 
                     ridx = r;
                     float4 o4 = ino[r], d4 = ind[r];
                     const GLOBAL DScene* S = gp(reqs[f2u(o4.w) >> 16].scene);
-                    seg_init<QUANT, OVF>(T, B, st, S, mk3(o4.x, o4.y, o4.z), mk3(d4.x, d4.y, d4.z));
+                    seg_init<QUANT, OVF>(T, B, st, S, mk3(o4.x, o4.y, o4.z), mk3(d4.x, d4.y, d4.z),
+                                         medium_tmax(S, f2u(o4.w), inr[r], f2u(d4.w)));
                 }
             }
             pool += take;
@@ -542,8 +568,9 @@ struct PState {
 };
 
 template <int QUANT, int OVF>
-__device__ inline void pstep_init(PState& T, Best& B, Stack<OVF>& st, const GLOBAL DScene* S, V3 O, V3 D) {
+__device__ inline void pstep_init(PState& T, Best& B, Stack<OVF>& st, const GLOBAL DScene* S, V3 O, V3 D, float tmax0) {
     best_init(B);
+    B.t = tmax0;
     spheres_first(S, O, D, B);
     T.S = S;
     T.node = QUANT ? S->qtlas_root : S->tlas_root;
@@ -639,6 +666,7 @@ __global__ EXT_BOUNDS void k_extend_pers(KParams P, int depth, int* ovf_base) {
     const uint32_t fetch = (uint32_t)P.ext_fetch;
     const GLOBAL float4* ino = gp(P.in.o);
     const GLOBAL float4* ind = gp(P.in.d);
+    const GLOBAL uint2* inr = gp(P.in.rng);
     const GLOBAL DReq* reqs = gp(P.chunk)->reqs;
     const unsigned long long lt_mask = (1ull << lane) - 1ull;
     const uint32_t refill_min = (uint32_t)P.refill_min;
@@ -671,7 +699,8 @@ __global__ EXT_BOUNDS void k_extend_pers(KParams P, int depth, int* ovf_base) {
                     ridx = r;
                     float4 o4 = ino[r], d4 = ind[r];
                     const GLOBAL DScene* S = gp(reqs[f2u(o4.w) >> 16].scene);
-                    pstep_init<QUANT, OVF>(T, B, st, S, mk3(o4.x, o4.y, o4.z), mk3(d4.x, d4.y, d4.z));
+                    pstep_init<QUANT, OVF>(T, B, st, S, mk3(o4.x, o4.y, o4.z), mk3(d4.x, d4.y, d4.z),
+                                           medium_tmax(S, f2u(o4.w), inr[r], f2u(d4.w)));
                 }
             }
             pool += take;
