@@ -429,6 +429,7 @@ static bool opts_valid(const ilm_opts& o) {
              o.extend_mode < 0 || o.extend_mode > ILM_EXTEND_STEP_REFILL || !sg_ok || o.tile_rows < 0 || o.tile_rows > 256 ||
              o.refill_min < 0 || o.refill_min > 64 || o.ext_fetch < 0 || o.ext_fetch > 65536 || (o.ext_fetch & 63) != 0 ||
              o.leaf_wait_pct < 0 || o.leaf_wait_pct > 10000 || o.shade_sort < 0 || o.shade_sort > 2 ||
+             o.sched_fair < 0 || o.sched_fair > 1 ||
              o.chunk_passes < 0 || o.paths_in_flight < 0 || o.hbm_budget_bytes < 0 || o.max_queued < 0);
 }
 
@@ -750,6 +751,31 @@ static bool job_has_work(const Job* j) { return j->next_pass < j->p.spp; }
 /* Fill a chunk from jobs in priority order. Each job contributes at most one slice per chunk
  * (all pixels x several passes, or a pixel range of one pass), so no pixel appears twice in a
  * chunk and every pixel's passes are accumulated in pass order. */
+static int64_t job_demand(const Job* j) {
+    const int64_t wh = (int64_t)j->p.width * j->p.height;
+    return (int64_t)(j->p.spp - j->next_pass) * wh - j->next_px;
+}
+
+/* Fair-share quotas (water filling): jobs needing less than an equal share of the chunk get
+ * what they need, the others split the rest equally. */
+static void fair_quotas(const std::vector<Job*>& jobs, int64_t cap, std::vector<int64_t>& quota) {
+    std::vector<std::pair<int64_t, size_t>> d;
+    for (size_t k = 0; k < jobs.size(); k++) {
+        quota[k] = 0;
+        if (job_has_work(jobs[k]) && !jobs[k]->cancel.load()) d.push_back({job_demand(jobs[k]), k});
+    }
+    std::sort(d.begin(), d.end());
+    int64_t left = cap;
+    size_t n = d.size();
+    for (auto& e : d) {
+        const int64_t level = left / (int64_t)std::max<size_t>(n, 1);
+        const int64_t q = std::min(e.first, std::max<int64_t>(level, 1));
+        quota[e.second] = q;
+        left -= q;
+        n--;
+    }
+}
+
 static void build_chunk(ilm_ctx* ctx, const std::vector<Job*>& jobs, DChunk& c, std::vector<Job*>& in_chunk) {
     c.total_paths = 0;
     c.total_px = 0;
@@ -758,23 +784,27 @@ static void build_chunk(ilm_ctx* ctx, const std::vector<Job*>& jobs, DChunk& c, 
     in_chunk.clear();
     int64_t left = ctx->cap;
     int nreq = 0;
-    for (Job* j : jobs) {
+    std::vector<int64_t> quota(jobs.size(), ctx->cap);
+    if (ctx->opts.sched_fair && jobs.size() > 1) fair_quotas(jobs, ctx->cap, quota);
+    for (size_t jk = 0; jk < jobs.size(); jk++) {
+        Job* j = jobs[jk];
         if (!job_has_work(j) || j->cancel.load()) continue;
         if (nreq >= ILM_MAX_JOBS || c.n_work >= ILM_MAX_WORK || left <= 0) break;
         const int64_t wh = (int64_t)j->p.width * j->p.height;
+        const int64_t lim = std::min(left, std::max<int64_t>(quota[jk], 1));
         DWork w;
         std::memset(&w, 0, sizeof(w));
         w.req = nreq;
         w.pass0 = j->next_pass;
-        if (j->next_px == 0 && wh <= left) {
-            int64_t np = std::min<int64_t>(j->p.spp - j->next_pass, left / wh);
+        if (j->next_px == 0 && wh <= lim) {
+            int64_t np = std::min<int64_t>(j->p.spp - j->next_pass, lim / wh);
             if (ctx->opts.chunk_passes > 0) np = std::min<int64_t>(np, ctx->opts.chunk_passes);
             w.npass = (int)np;
             w.px0 = 0;
             w.npx = (uint32_t)wh;
             j->next_pass += (int)np;
         } else {
-            int64_t npx = std::min<int64_t>(wh - j->next_px, left);
+            int64_t npx = std::min<int64_t>(wh - j->next_px, lim);
             w.npass = 1;
             w.px0 = (uint32_t)j->next_px;
             w.npx = (uint32_t)npx;

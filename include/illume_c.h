@@ -215,6 +215,11 @@ typedef struct ilm_opts {
     /* shade the paths of each 512-path block grouped by medium / miss / sphere / instance
      * (0 = default: on, 1 = on, 2 = off) */
     int32_t shade_sort;
+    /* chunk packing across requests (served and batched forms): 0 = in arrival order, each
+     * request taking all it can; 1 = fair share (requests needing less than an equal share get
+     * it all, the rest split the chunk), so short requests finish without waiting behind long
+     * ones. Results are identical either way. */
+    int32_t sched_fair;
 } ilm_opts;
 
 typedef struct ilm_render_info {

@@ -53,7 +53,8 @@ def test_graph_replays_match_direct_launches(scenes):
                                    dict(extend_mode=1), dict(extend_mode=2, refill_min=4), dict(extend_mode=3, refill_min=1),
                                    dict(extend_mode=3, refill_min=48, sample_group=16),
                                    dict(extend_mode=3, leaf_wait_pct=10), dict(extend_mode=3, leaf_wait_pct=5000),
-                                   dict(shade_sort=2), dict(shade_sort=1, paths_in_flight=5000)])
+                                   dict(shade_sort=2), dict(shade_sort=1, paths_in_flight=5000),
+                                   dict(sched_fair=1, paths_in_flight=3000)])
 def test_path_order_leaves_results_unchanged(ctx, scenes, order):
     """Samples grouped per pixel and tiled pixel walks only reorder path ids: every pixel still
     receives the same samples, summed in pass order, so the images are bit-identical."""
@@ -99,6 +100,25 @@ def test_served_requests_equal_plain_renders(ctx, scenes):
         r.free()
     s = ctx.stats()
     assert s["requests_done"] >= len(reqs) and s["p99_ms"] >= s["p50_ms"] > 0
+
+
+def test_fair_share_served_and_batched_equal_plain(ctx, scenes):
+    """Fair-share packing with a small chunk: every chunk mixes several requests' slices (pixel
+    ranges and pass groups); each result still equals its plain render."""
+    with illume_amd.Context(0, sched_fair=1, paths_in_flight=20000) as c:
+        local = {n: c.scene(scenes[n][0]) for n in scenes}
+        ps = [(n, _p(scenes[n][1], **kw)) for n, kw in MIX]
+        reqs = [(c.submit(local[n], p), n, p) for n, p in ps]
+        batch, _ = c.render_batch([local[n] for n, _ in ps], [p for _, p in ps])
+        for (r, n, p), (acc_b, px_b) in zip(reqs, batch):
+            r.wait(120000)
+            acc, px = r.result()
+            ref, refpx, _ = ctx.render(scenes[n][2], p)
+            assert np.array_equal(acc, ref) and np.array_equal(px, refpx), n
+            assert np.array_equal(acc_b, ref) and np.array_equal(px_b, refpx), n
+            r.free()
+        for sc in local.values():
+            sc.free()
 
 
 def test_cancel_and_admission_limits(scenes):
