@@ -390,33 +390,38 @@ def bench_served(args):
                 % (name, segs / span / 1e6, span, forms[name]["p50_ms"], forms[name]["p99_ms"]))
             return out
 
-        # served: submissions at arrival times, the scheduler batches whatever is admitted
-        s0 = ctx.stats()
-        handles = []
-        start = time.perf_counter()
-        _replay(None, trace, start, lambda r: handles.append(ctx.submit(sc[r["scene"]], params[len(handles)])))
-        served_acc = []
-        lat = []
-        for h in handles:
-            h.wait()
-        span = time.perf_counter() - start
-        for h in handles:
-            a, _ = h.result()
-            served_acc.append(a)
-            st = h.poll()
-            lat.append(st["end_ms"] - st["submit_ms"])
-        s1 = ctx.stats()
-        segs = s1["segments"] - s0["segments"]
-        forms["served"] = dict(Mrays_s=segs / span / 1e6, makespan_s=span, segments=segs,
+        def served_form(name, fair):
+            """submissions at arrival times; the scheduler packs whatever is admitted into chunks"""
+            saved = ctx.settings()
+            ctx.tune(sched_fair=fair)
+            s0 = ctx.stats()
+            handles = []
+            start = time.perf_counter()
+            _replay(None, trace, start, lambda r: handles.append(ctx.submit(sc[r["scene"]], params[len(handles)])))
+            out, lat = [], []
+            for h in handles:
+                h.wait()
+            span = time.perf_counter() - start
+            for h in handles:
+                a, _ = h.result()
+                out.append(a)
+                st = h.poll()
+                lat.append(st["end_ms"] - st["submit_ms"])
+                h.free()
+            segs = ctx.stats()["segments"] - s0["segments"]
+            ctx.tune(**saved)
+            forms[name] = dict(Mrays_s=segs / span / 1e6, makespan_s=span, segments=segs,
                                p50_ms=percentile(lat, 0.5), p99_ms=percentile(lat, 0.99))
-        log("served: %.1f Mrays/s, makespan %.2f s, p50 %.0f ms, p99 %.0f ms"
-            % (segs / span / 1e6, span, forms["served"]["p50_ms"], forms["served"]["p99_ms"]))
-        for h in handles:
-            h.free()
-        accs["served"] = served_acc
+            log("%s: %.1f Mrays/s, makespan %.2f s, p50 %.0f ms, p99 %.0f ms"
+                % (name, segs / span / 1e6, span, forms[name]["p50_ms"], forms[name]["p99_ms"]))
+            return out
+
+        accs["served"] = served_form("served", 0)
+        accs["served_fair"] = served_form("served_fair", 1)
         accs["plain"] = plain_form("plain")
         plain_form("plain_per_pass", chunk_passes=1)
-        same = sum(int(np.array_equal(a, b)) for a, b in zip(accs["served"], accs["plain"]))
+        same = sum(int(np.array_equal(a, b) and np.array_equal(a, f))
+                   for a, b, f in zip(accs["served"], accs["plain"], accs["served_fair"]))
         for s in sc.values():
             s.free()
     rm = None
@@ -433,8 +438,10 @@ def bench_served(args):
               "requests": len(trace), "parallelism": "1 GPU"}
     out = base_line(args, "served path segments/s (Mrays/s); p50/p99 request latency", sv["Mrays_s"], "Mrays/s",
                     sv["makespan_s"] * 1e3, config)
+    fa = forms["served_fair"]
     out["latency_ms"] = {"p50": round(sv["p50_ms"], 1), "p99": round(sv["p99_ms"], 1),
-                         "form": "served (completion - arrival)"}
+                         "form": "served, arrival-order packing (completion - arrival)",
+                         "fair_share": {"p50": round(fa["p50_ms"], 1), "p99": round(fa["p99_ms"], 1)}}
     out["forms"] = {k: {kk: round(vv, 3) for kk, vv in v.items()} for k, v in forms.items()}
     out["served_vs_plain"] = round(sv["Mrays_s"] / forms["plain"]["Mrays_s"], 3)
     out["served_vs_plain_per_pass"] = round(sv["Mrays_s"] / forms["plain_per_pass"]["Mrays_s"], 3)

@@ -123,8 +123,8 @@ class Context:
         self.h = h
         self.opts = o
 
-    TUNABLE = ("extend_mode", "refill_min", "ext_fetch", "leaf_wait_pct", "shade_sort", "sample_group",
-               "tile_rows", "chunk_passes", "paths_in_flight")
+    TUNABLE = ("extend_mode", "refill_min", "ext_fetch", "leaf_wait_pct", "shade_sort", "sched_fair",
+               "sample_group", "tile_rows", "chunk_passes", "paths_in_flight")
 
     def tune(self, **kw):
         """Change the tunable options (TUNABLE) of an idle context; others keep their values."""

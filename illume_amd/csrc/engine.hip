@@ -580,6 +580,7 @@ int ilm_tune(ilm_ctx* ctx, const ilm_opts* o) {
     ctx->opts.ext_fetch = o->ext_fetch;
     ctx->opts.leaf_wait_pct = o->leaf_wait_pct;
     ctx->opts.shade_sort = o->shade_sort;
+    ctx->opts.sched_fair = o->sched_fair;
     ctx->shade_grid = ctx->n_cu * ilm_k_shade_occupancy(ctx->opts.shade_sort != 2);
     ctx->opts.chunk_passes = o->chunk_passes;
     const int64_t cap = cap_of(*o);

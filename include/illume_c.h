@@ -278,8 +278,8 @@ void ilm_close(ilm_ctx* ctx);
 const char* ilm_last_error(ilm_ctx* ctx);
 int ilm_stats_get(ilm_ctx* ctx, ilm_stats* out);
 /* Retune an idle context (no counterpart in the reference, whose launch shape is fixed):
- * extend_mode, refill_min, ext_fetch, leaf_wait_pct, shade_sort, sample_group, tile_rows, chunk_passes
- * and paths_in_flight
+ * extend_mode, refill_min, ext_fetch, leaf_wait_pct, shade_sort, sched_fair, sample_group, tile_rows,
+ * chunk_passes and paths_in_flight
  * take the values in *opts; other fields are ignored. ILM_E_BUSY while served requests are queued or running.
  * The autotuner (illume_amd/autotune.py) drives this between short trial renders. */
 int ilm_tune(ilm_ctx* ctx, const ilm_opts* opts);
