@@ -188,7 +188,10 @@ __device__ inline float medium_tmax(const GLOBAL DScene* S, uint32_t tag, uint2 
     const float u = ilm_pcg32_uniform(&rs, ilm_pcg32_inc(pixel));
     const float sdist = -pm_logf(u) / md->scattering;
     if (!(sdist < ILM_FLT_MAX)) return ILM_FLT_MAX;
-    return __uint_as_float(__float_as_uint(sdist) + 1u);   /* sdist >= 0 */
+    /* u = 1 gives sdist = -0.0: the path scatters whatever it hits, and the cap must stay
+     * positive (the next float above +0) so that shade's "sdist < t" still holds */
+    const float s0 = sdist > 0.0f ? sdist : 0.0f;
+    return __uint_as_float(__float_as_uint(s0) + 1u);
 }
 
 /* Intersect the triangles of BLAS leaf `node`. */

@@ -807,7 +807,11 @@ template <class R> static RenderInfo<R> make_render_info(const Scene<R>& sc, int
 }
 
 /* ---------------- one path (src/renderer.cu:81-110 and 139-388) ---------------- */
-struct PathStats { int segments = 0; };
+struct PathStats {
+    int segments = 0;
+    float* rec = nullptr;   /* nullable: 9 floats per segment (origin, direction, inst, prim, t) */
+    int rec_max = 0;
+};
 
 template <class R> static V3<R> trace_path(const Scene<R>& sc, const RenderInfo<R>& ri, const ilm_render_params& p,
                                             uint32_t pass, uint32_t pixel, PathStats* st) {
@@ -839,6 +843,13 @@ template <class R> static V3<R> trace_path(const Scene<R>& sc, const RenderInfo<
     for (int depth = 0; depth < p.max_depth; depth++) {
         st->segments++;
         Hit<R> min = get_min_hit(sc, ray, nullptr, nullptr);
+        if (st->rec && depth < st->rec_max) {
+            float* q = st->rec + 9 * depth;
+            q[0] = (float)ray.o.x; q[1] = (float)ray.o.y; q[2] = (float)ray.o.z;
+            q[3] = (float)ray.d.x; q[4] = (float)ray.d.y; q[5] = (float)ray.d.z;
+            q[6] = min.is ? (float)min.inst : -1.0f; q[7] = min.is ? (float)min.prim : -1.0f;
+            q[8] = min.is ? (float)min.d : FLT_MAX;
+        }
         if (medium >= 0 && sc.mediums[medium].active) {
             const Medium<R>& md = sc.mediums[medium];
             R u = rnd();
@@ -1268,6 +1279,19 @@ int orc_path_radiance(orc_scene* s, const ilm_render_params* p, const uint32_t* 
         }
     }
     return 0;
+}
+
+/* The segments of one FP32 path: rec holds 9 floats per segment (ray origin, ray direction,
+ * closest-hit instance, primitive, distance; -1, -1, FLT_MAX for a miss), at most rec_max of them.
+ * Returns the path's segment count. */
+int orc_path_segments(orc_scene* s, const ilm_render_params* p, uint32_t pixel, uint32_t pass,
+                      float* rec, int rec_max) {
+    RenderInfo<float> ri = make_render_info(*s->f, p->width, p->height);
+    PathStats st;
+    st.rec = rec;
+    st.rec_max = rec_max;
+    trace_path(*s->f, ri, *p, pass, pixel, &st);
+    return st.segments;
 }
 
 /* Render pixels [pix_begin, pix_end) of the image for passes first_sample..first_sample+spp-1.

@@ -35,6 +35,7 @@ def lib():
         "orc_ref_bvh_export": (C.c_int, [vp, C.c_int, fp, P(C.c_uint32), ip]),
         "orc_closest_hit": (C.c_int, [vp, fp, C.c_int, ip, ip, fp, C.c_int, ip, C.c_int]),
         "orc_path_radiance": (C.c_int, [vp, P(RenderParams), P(C.c_uint32), C.c_int, vp, ip, C.c_int, C.c_int]),
+        "orc_path_segments": (C.c_int, [vp, P(RenderParams), C.c_uint32, C.c_uint32, fp, C.c_int]),
         "orc_render": (C.c_double, [vp, P(RenderParams), C.c_int64, C.c_int64, C.c_int, C.c_int, vp, P(Pixel),
                                     P(C.c_uint64), C.c_int]),
         "orc_tonemap_image": (C.c_int, [fp, C.c_int64, C.c_int, C.c_int, P(Pixel)]),
@@ -119,6 +120,12 @@ class OracleScene:
                                      rgb.ctypes.data_as(C.c_void_p), i32p(seg), int(fp64), int(threads))
         assert rc == 0
         return rgb, seg
+
+    def path_segments(self, params, pixel, pass_, max_segments=64):
+        """Segments of one FP32 path as (n, 9) floats: origin, direction, inst, prim, t."""
+        rec = np.zeros((max_segments, 9), np.float32)
+        n = lib().orc_path_segments(self.h, C.byref(params), int(pixel), int(pass_), f32p(rec), max_segments)
+        return rec[:min(n, max_segments)]
 
     def render(self, params, pix_begin=0, pix_end=None, kahan=True, fp64=False, threads=0, pixels=True):
         """Returns (accum (n,3), pixels (n,3) uint64 or None, segments, seconds)."""

@@ -346,3 +346,26 @@ def test_medium_absorption_matches_beer_lambert():
     assert np.allclose(rgb, want[None, :], rtol=1e-4)
     assert np.all(seg == 3)   # enter, exit, environment
     orc.free()
+
+
+def test_path_segments_record_the_sphere_chords():
+    """The segment recorder (used to locate where a device path first parts from the oracle)
+    on the same axis ray: hit the sphere's near side at 5 - R, its far side a chord 2R later,
+    then miss; the count equals path_radiance's."""
+    from oracle.pyoracle import OracleScene
+    from illume_amd._abi import render_params
+    R = 0.75
+    spec = dict(meshes=[], instances=[], spheres=[dict(center=[0, 0, 0], radius=R, material=0)],
+                materials=[dict(type="refractive", ior=1.0, albedo=[1, 1, 1], medium=0)],
+                mediums=[dict(sigma_a=[0.3, 0.9, 2.0], sigma_s=0.0, g=0.0)], environment=[1.0, 1.0, 1.0],
+                camera=dict(fov=0.01, dof=1.0, aperture=0.0, transform=dict(translation=[0, 0, -5])))
+    orc = OracleScene(spec)
+    p = render_params(width=5, height=5, spp=1, max_depth=8, ray_bias=1e-5, tonemap="linear")
+    rec = orc.path_segments(p, 12, 0)
+    _, seg = orc.path_radiance(p, np.array([[12, 0]], np.uint32))
+    assert len(rec) == seg[0] == 3
+    assert np.allclose(rec[:2, 6:8], [[-2, -1], [-2, -1]])      # sphere 0 is instance -2, no primitive
+    assert abs(rec[0, 8] - (5 - R)) < 1e-5 and abs(rec[1, 8] - 2 * R) < 1e-4
+    assert rec[2, 6] == -1 and rec[2, 8] == np.float32(3.4028235e38)
+    assert np.allclose(rec[:, 3:6] / np.linalg.norm(rec[:, 3:6], axis=1, keepdims=True), [0, 0, 1], atol=1e-4)
+    orc.free()
