@@ -648,13 +648,21 @@ static int stage_chunk(ilm_ctx* ctx, const DChunk& hc) {
     return ILM_OK;
 }
 
+/* A chunk's first launch after k_ctl_init. With per-step refill the first depth's extend and
+ * shade form the camera rays where they use them, so only the per-path results are cleared;
+ * the other extend modes read the rays k_raygen writes out. */
+static hipError_t start_paths(ilm_ctx* ctx, KParams& P, hipStream_t s) {
+    P.in = ctx->A;
+    P.cam0 = P.extend_mode == ILM_EXTEND_STEP_REFILL ? 1 : 0;
+    return P.cam0 ? ilm_k_path_clear(P, ctx->gen_grid, s) : ilm_k_raygen(P, ctx->gen_grid, s);
+}
+
 /* The launch sequence of one chunk with depth_steps depth iterations. */
 static int enqueue_pipeline(ilm_ctx* ctx, const KParams& base, int depth_steps, ProfileSink* prof) {
     hipStream_t s = ctx->stream;
     KParams P = base;
     HIPTRY(ctx, ilm_k_ctl_init(P, s));
-    P.in = ctx->A;
-    HIPTRY(ctx, ilm_k_raygen(P, ctx->gen_grid, s));
+    HIPTRY(ctx, start_paths(ctx, P, s));
     for (int d = 0; d < depth_steps; d++) {
         P.in = (d & 1) ? ctx->B : ctx->A;
         P.out = (d & 1) ? ctx->A : ctx->B;
@@ -1072,7 +1080,7 @@ int ilm_debug_path_radiance(ilm_ctx* ctx, ilm_scene* scene, const ilm_render_par
         P.use_ovf = scene_needs_ovf(ctx, scene) ? 1 : 0;
         P.list = d_list;
         P.seg_count = ctx->seg_count;
-        if (ilm_k_ctl_init(P, ctx->stream) != hipSuccess || ilm_k_raygen(P, ctx->gen_grid, ctx->stream) != hipSuccess) {
+        if (ilm_k_ctl_init(P, ctx->stream) != hipSuccess || start_paths(ctx, P, ctx->stream) != hipSuccess) {
             rc = fail(ctx, ILM_E_DEVICE, "raygen launch");
             break;
         }

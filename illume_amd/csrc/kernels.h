@@ -5,6 +5,7 @@
 hipError_t ilm_k_ctl_init(const KParams& P, hipStream_t s);
 hipError_t ilm_k_raygen(const KParams& P, int grid, hipStream_t s);
 hipError_t ilm_k_extend(const KParams& P, int depth, int* ovf, int grid, hipStream_t s);
+hipError_t ilm_k_path_clear(const KParams& P, int grid, hipStream_t s);
 hipError_t ilm_k_shade(const KParams& P, int depth, int grid, hipStream_t s);
 hipError_t ilm_k_accumulate(const KParams& P, int grid, hipStream_t s);
 hipError_t ilm_k_chunk_end(const KParams& P, unsigned long long* seg_total, hipStream_t s);

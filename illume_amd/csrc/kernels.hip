@@ -38,62 +38,27 @@ __global__ void k_ctl_init(DCtl* ctl_, const DChunk* chunk) {
     }
 }
 
-__device__ inline int find_work_pid(const GLOBAL DWork* work, int n, uint32_t i) {
-    int lo = 0, hi = n - 1;
-    while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (work[mid].pid_base <= i) lo = mid; else hi = mid - 1;
+/* src/renderer.cu:81-110 (and the per-pass seeding of :61-69): the camera rays written out, for
+ * the extend modes that read the first depth's rays from memory */
+__global__ __launch_bounds__(256) void k_raygen(KParams P) {
+    const uint32_t total = gp(P.chunk)->total_paths;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
+        float4 o4, d4;
+        uint2 rg;
+        camera_ray(P, i, o4, d4, rg);
+        gpw(P.in.o)[i] = o4;
+        gpw(P.in.d)[i] = d4;
+        gpw(P.in.c)[i] = make_float4(1.0f, 1.0f, 1.0f, u2f(i));
+        gpw(P.in.rng)[i] = rg;
+        gpw(P.contrib)[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (P.seg_count) gpw(P.seg_count)[i] = 0;
     }
-    return lo;
 }
 
-/* src/renderer.cu:81-110 (and the per-pass seeding of :61-69) */
-__global__ __launch_bounds__(256) void k_raygen(KParams P) {
-    const GLOBAL DChunk* C = gp(P.chunk);
-    const uint32_t total = C->total_paths;
+/* Per-path results cleared for a chunk whose first depth forms its camera rays in place. */
+__global__ __launch_bounds__(256) void k_path_clear(KParams P) {
+    const uint32_t total = gp(P.chunk)->total_paths;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
-        int req;
-        uint32_t pixel, pass;
-        if (P.list) {
-            req = 0;
-            pixel = gp(P.list)[2 * i];
-            pass = gp(P.list)[2 * i + 1];
-        } else {
-            const GLOBAL DWork* W = C->work + find_work_pid(C->work, C->n_work, i);
-            uint32_t k, q;
-            slice_decode(i - W->pid_base, W->npx, W->sgroup, k, q);
-            req = W->req;
-            pass = (uint32_t)W->pass0 + k;
-            pixel = W->px0 + q;
-        }
-        const GLOBAL DReq* R = C->reqs + req;
-        if (!P.list) pixel = tile_pixel(pixel, (uint32_t)R->width, (uint32_t)R->height, P.tile_rows);
-        const GLOBAL DScene* S = gp(R->scene);
-        uint64_t rs = ilm_path_seed(pass, pixel);
-        const uint64_t inc = ilm_pcg32_inc(pixel);
-        const float pixel_size = R->pixel_size;
-        int Wd = (int)R->image_width;
-        int x = (int)pixel % Wd, y = (int)pixel / Wd;
-        float left_edge = R->left + pixel_size * (float)x;
-        float top_edge = R->top - pixel_size * (float)y;
-        float rx = left_edge + pixel_size * ilm_pcg32_uniform(&rs, inc);
-        float ry = top_edge - pixel_size * ilm_pcg32_uniform(&rs, inc);
-        V3 pos = mk3(0.0f, 0.0f, 0.0f);
-        const float aperture = S->aperture;
-        if (aperture > ILM_FLT_EPSILON) {
-            float u1 = ilm_pcg32_uniform(&rs, inc);
-            float u2 = ilm_pcg32_uniform(&rs, inc);
-            pos = mul3(sample_circle(u1, u2), aperture);
-        }
-        float cam[12];
-        load_rows12(S->cam, cam);
-        V3 origin = xform3(cam, pos, 1.0f);
-        V3 image_pos = xform3(cam, mk3(rx, ry, S->dof), 1.0f);
-        V3 dir = normalize3(sub3(image_pos, origin));
-        gpw(P.in.o)[i] = make_float4(origin.x, origin.y, origin.z, u2f((uint32_t)req << 16));
-        gpw(P.in.d)[i] = make_float4(dir.x, dir.y, dir.z, u2f(pixel));
-        gpw(P.in.c)[i] = make_float4(1.0f, 1.0f, 1.0f, u2f(i));
-        gpw(P.in.rng)[i] = make_uint2((uint32_t)rs, (uint32_t)(rs >> 32));
         gpw(P.contrib)[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         if (P.seg_count) gpw(P.seg_count)[i] = 0;
     }
@@ -372,7 +337,7 @@ __device__ inline uint32_t shade_key(int medium, int hinst) {
     return (uint32_t)hinst % 9u;
 }
 
-template <int SORT>
+template <int SORT, int CAM>
 __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(KParams P, int depth) {
     /* survivors of the block's 512 paths get their queue slots from one atomic: per-wave counts
      * meet in LDS, wave 0 lane 0 reserves the block's range, each wave takes its offset */
@@ -407,8 +372,13 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(KParams P, int depth) {
         float4 o4 = make_float4(0, 0, 0, 0), d4 = o4, c4 = o4, h4 = o4, n4 = o4;
         uint2 rg = make_uint2(0, 0);
         if (valid) {
-            o4 = ino[i]; d4 = ind[i]; c4 = inc4[i];
-            rg = inr[i];
+            if (CAM) {
+                camera_ray(P, i, o4, d4, rg);
+                c4 = make_float4(1.0f, 1.0f, 1.0f, u2f(i));
+            } else {
+                o4 = ino[i]; d4 = ind[i]; c4 = inc4[i];
+                rg = inr[i];
+            }
             h4 = hh[i];
             n4 = hn[i];
         }
@@ -596,11 +566,17 @@ hipError_t ilm_k_raygen(const KParams& P, int grid, hipStream_t s) {
     ILM_LAUNCH(k_raygen, dim3(grid), dim3(256), 0, s, P);
     return hipGetLastError();
 }
+hipError_t ilm_k_path_clear(const KParams& P, int grid, hipStream_t s) {
+    ILM_LAUNCH(k_path_clear, dim3(grid), dim3(256), 0, s, P);
+    return hipGetLastError();
+}
 
 template <int T, int Q, int C, int O>
 static void launch_extend(const KParams& P, int depth, int* ovf, int grid, hipStream_t s) {
-    if (P.extend_mode == ILM_EXTEND_STEP_REFILL)
-        ILM_LAUNCH((k_extend_pers<T, Q, C, O>), dim3(grid), dim3(EXT_BLOCK), 0, s, P, depth, ovf);
+    if (P.extend_mode == ILM_EXTEND_STEP_REFILL && P.cam0 && depth == 0)
+        ILM_LAUNCH((k_extend_pers<T, Q, C, O, 1>), dim3(grid), dim3(EXT_BLOCK), 0, s, P, depth, ovf);
+    else if (P.extend_mode == ILM_EXTEND_STEP_REFILL)
+        ILM_LAUNCH((k_extend_pers<T, Q, C, O, 0>), dim3(grid), dim3(EXT_BLOCK), 0, s, P, depth, ovf);
     else if (P.extend_mode == ILM_EXTEND_LEAF_REFILL)
         ILM_LAUNCH((k_extend_refill<T, Q, C, O>), dim3(grid), dim3(EXT_BLOCK), 0, s, P, depth, ovf);
     else
@@ -628,8 +604,11 @@ hipError_t ilm_k_extend(const KParams& P, int depth, int* ovf, int grid, hipStre
     return hipGetLastError();
 }
 hipError_t ilm_k_shade(const KParams& P, int depth, int grid, hipStream_t s) {
-    if (P.shade_sort) ILM_LAUNCH(k_shade<1>, dim3(grid), dim3(SHADE_BLOCK), 0, s, P, depth);
-    else ILM_LAUNCH(k_shade<0>, dim3(grid), dim3(SHADE_BLOCK), 0, s, P, depth);
+    const bool cam = P.cam0 && depth == 0;
+    if (P.shade_sort && cam) ILM_LAUNCH((k_shade<1, 1>), dim3(grid), dim3(SHADE_BLOCK), 0, s, P, depth);
+    else if (P.shade_sort) ILM_LAUNCH((k_shade<1, 0>), dim3(grid), dim3(SHADE_BLOCK), 0, s, P, depth);
+    else if (cam) ILM_LAUNCH((k_shade<0, 1>), dim3(grid), dim3(SHADE_BLOCK), 0, s, P, depth);
+    else ILM_LAUNCH((k_shade<0, 0>), dim3(grid), dim3(SHADE_BLOCK), 0, s, P, depth);
     return hipGetLastError();
 }
 hipError_t ilm_k_accumulate(const KParams& P, int grid, hipStream_t s) {
@@ -662,10 +641,10 @@ static int occ_of(K kernel) {
 int ilm_k_extend_occupancy(int tri_test, int bvh_layout, int extend_mode) {
     int a, b;
     if (extend_mode == ILM_EXTEND_STEP_REFILL) {
-        if (!tri_test && !bvh_layout) { a = occ_of(k_extend_pers<0, 0, 0, 0>); b = occ_of(k_extend_pers<0, 0, 0, 1>); }
-        else if (tri_test && !bvh_layout) { a = occ_of(k_extend_pers<1, 0, 0, 0>); b = occ_of(k_extend_pers<1, 0, 0, 1>); }
-        else if (!tri_test) { a = occ_of(k_extend_pers<0, 1, 0, 0>); b = occ_of(k_extend_pers<0, 1, 0, 1>); }
-        else { a = occ_of(k_extend_pers<1, 1, 0, 0>); b = occ_of(k_extend_pers<1, 1, 0, 1>); }
+        if (!tri_test && !bvh_layout) { a = occ_of(k_extend_pers<0, 0, 0, 0, 0>); b = occ_of(k_extend_pers<0, 0, 0, 1, 0>); }
+        else if (tri_test && !bvh_layout) { a = occ_of(k_extend_pers<1, 0, 0, 0, 0>); b = occ_of(k_extend_pers<1, 0, 0, 1, 0>); }
+        else if (!tri_test) { a = occ_of(k_extend_pers<0, 1, 0, 0, 0>); b = occ_of(k_extend_pers<0, 1, 0, 1, 0>); }
+        else { a = occ_of(k_extend_pers<1, 1, 0, 0, 0>); b = occ_of(k_extend_pers<1, 1, 0, 1, 0>); }
     } else if (extend_mode == ILM_EXTEND_LEAF_REFILL) {
         if (!tri_test && !bvh_layout) { a = occ_of(k_extend_refill<0, 0, 0, 0>); b = occ_of(k_extend_refill<0, 0, 0, 1>); }
         else if (tri_test && !bvh_layout) { a = occ_of(k_extend_refill<1, 0, 0, 0>); b = occ_of(k_extend_refill<1, 0, 0, 1>); }
@@ -682,8 +661,8 @@ int ilm_k_extend_occupancy(int tri_test, int bvh_layout, int extend_mode) {
 /* Resident 512-thread shading blocks per CU (the LDS staging of the sorted variant limits it). */
 int ilm_k_shade_occupancy(int sort) {
     int nb = 0;
-    hipError_t e = sort ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_shade<1>, SHADE_BLOCK, 0)
-                        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_shade<0>, SHADE_BLOCK, 0);
+    hipError_t e = sort ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_shade<1, 0>, SHADE_BLOCK, 0)
+                        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_shade<0, 0>, SHADE_BLOCK, 0);
     return (e != hipSuccess || nb <= 0) ? 2 : nb;
 }
 int ilm_k_ovf_stack() { return OVF_STACK; }

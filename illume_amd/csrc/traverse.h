@@ -40,6 +40,17 @@
 __device__ inline uint32_t f2u(float f) { return __float_as_uint(f); }
 __device__ inline float u2f(uint32_t u) { return __uint_as_float(u); }
 
+/* The work item of the chunk that holds path id i (work items sorted by pid_base). */
+__device__ inline int find_work_pid(const GLOBAL DWork* work, int n, uint32_t i) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (work[mid].pid_base <= i) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+
 struct Best {
     float t, u, v;
     int inst, prim;
@@ -160,6 +171,55 @@ __device__ inline void load_rows12(const GLOBAL float* src, float* dst) {
     dst[0] = a.x; dst[1] = a.y; dst[2] = a.z; dst[3] = a.w;
     dst[4] = b.x; dst[5] = b.y; dst[6] = b.z; dst[7] = b.w;
     dst[8] = c.x; dst[9] = c.y; dst[10] = c.z; dst[11] = c.w;
+}
+
+/* Camera ray of path id i, src/renderer.cu:81-110 with the per-pass seeding of :61-69: origin
+ * (w = bits of req << 16: no medium), unit direction (w = bits of the pixel) and the PCG32
+ * state after the ray's draws. A function of the chunk descriptor and i alone, so the first
+ * depth's extend and shade launches form it where they need it instead of reading it back. */
+__device__ inline void camera_ray(const KParams& P, uint32_t i, float4& o4, float4& d4, uint2& rg) {
+    const GLOBAL DChunk* C = gp(P.chunk);
+    int req;
+    uint32_t pixel, pass;
+    if (P.list) {
+        req = 0;
+        pixel = gp(P.list)[2 * i];
+        pass = gp(P.list)[2 * i + 1];
+    } else {
+        const GLOBAL DWork* W = C->work + find_work_pid(C->work, C->n_work, i);
+        uint32_t k, q;
+        slice_decode(i - W->pid_base, W->npx, W->sgroup, k, q);
+        req = W->req;
+        pass = (uint32_t)W->pass0 + k;
+        pixel = W->px0 + q;
+    }
+    const GLOBAL DReq* R = C->reqs + req;
+    if (!P.list) pixel = tile_pixel(pixel, (uint32_t)R->width, (uint32_t)R->height, P.tile_rows);
+    const GLOBAL DScene* S = gp(R->scene);
+    uint64_t rs = ilm_path_seed(pass, pixel);
+    const uint64_t inc = ilm_pcg32_inc(pixel);
+    const float pixel_size = R->pixel_size;
+    int Wd = (int)R->image_width;
+    int x = (int)pixel % Wd, y = (int)pixel / Wd;
+    float left_edge = R->left + pixel_size * (float)x;
+    float top_edge = R->top - pixel_size * (float)y;
+    float rx = left_edge + pixel_size * ilm_pcg32_uniform(&rs, inc);
+    float ry = top_edge - pixel_size * ilm_pcg32_uniform(&rs, inc);
+    V3 pos = mk3(0.0f, 0.0f, 0.0f);
+    const float aperture = S->aperture;
+    if (aperture > ILM_FLT_EPSILON) {
+        float u1 = ilm_pcg32_uniform(&rs, inc);
+        float u2 = ilm_pcg32_uniform(&rs, inc);
+        pos = mul3(sample_circle(u1, u2), aperture);
+    }
+    float cam[12];
+    load_rows12(S->cam, cam);
+    V3 origin = xform3(cam, pos, 1.0f);
+    V3 image_pos = xform3(cam, mk3(rx, ry, S->dof), 1.0f);
+    V3 dir = normalize3(sub3(image_pos, origin));
+    o4 = make_float4(origin.x, origin.y, origin.z, u2f((uint32_t)req << 16));
+    d4 = make_float4(dir.x, dir.y, dir.z, u2f(pixel));
+    rg = make_uint2((uint32_t)rs, (uint32_t)(rs >> 32));
 }
 
 __device__ inline void spheres_first(const GLOBAL DScene* S, V3 O, V3 D, Best& B) {
@@ -656,7 +716,7 @@ __device__ inline bool pstep(PState& T, Best& B, Stack<OVF>& st, TravCount* tc) 
     return false;
 }
 
-template <int TRI, int QUANT, int COUNT, int OVF>
+template <int TRI, int QUANT, int COUNT, int OVF, int CAM>
 __global__ EXT_BOUNDS void k_extend_pers(KParams P, int depth, int* ovf_base) {
     __shared__ int stk[LDS_STACK * EXT_BLOCK];
     const int lane = threadIdx.x;
@@ -700,10 +760,19 @@ __global__ EXT_BOUNDS void k_extend_pers(KParams P, int depth, int* ovf_base) {
                 if (rank < take) {
                     const uint32_t r = pool + rank;
                     ridx = r;
-                    float4 o4 = ino[r], d4 = ind[r];
-                    const GLOBAL DScene* S = gp(reqs[f2u(o4.w) >> 16].scene);
-                    pstep_init<QUANT, OVF>(T, B, st, S, mk3(o4.x, o4.y, o4.z), mk3(d4.x, d4.y, d4.z),
-                                           medium_tmax(S, f2u(o4.w), inr[r], f2u(d4.w)));
+                    if (CAM) {
+                        /* camera rays start outside every medium: no free-flight cap */
+                        float4 o4, d4;
+                        uint2 rg;
+                        camera_ray(P, r, o4, d4, rg);
+                        const GLOBAL DScene* S = gp(reqs[f2u(o4.w) >> 16].scene);
+                        pstep_init<QUANT, OVF>(T, B, st, S, mk3(o4.x, o4.y, o4.z), mk3(d4.x, d4.y, d4.z), ILM_FLT_MAX);
+                    } else {
+                        float4 o4 = ino[r], d4 = ind[r];
+                        const GLOBAL DScene* S = gp(reqs[f2u(o4.w) >> 16].scene);
+                        pstep_init<QUANT, OVF>(T, B, st, S, mk3(o4.x, o4.y, o4.z), mk3(d4.x, d4.y, d4.z),
+                                               medium_tmax(S, f2u(o4.w), inr[r], f2u(d4.w)));
+                    }
                 }
             }
             pool += take;
