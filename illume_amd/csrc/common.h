@@ -195,9 +195,8 @@ struct DPaths {
 };
 
 struct DHits {
-    float4* h;          /* t, u, v, bits of prim */
-    float4* n;          /* object-space face normal cross(v1-v0, v2-v0) (triangle hits), bits of
-                           the instance (-2-s for sphere s, -1 for a miss) */
+    float4* h;          /* t, u, v, bits of prim (original triangle index) */
+    int* inst;          /* instance of the hit: -2-s for sphere s, -1 for a miss */
 };
 
 /* Everything a kernel launch needs. Passed by value. */

@@ -417,7 +417,7 @@ static int ctx_alloc_paths(ilm_ctx* ctx, int64_t cap) {
         HIPTRY(ctx, A((void**)&P->rng, cap * sizeof(uint2)));
     }
     HIPTRY(ctx, A((void**)&ctx->hits.h, cap * sizeof(float4)));
-    HIPTRY(ctx, A((void**)&ctx->hits.n, cap * sizeof(float4)));
+    HIPTRY(ctx, A((void**)&ctx->hits.inst, cap * sizeof(int)));
     HIPTRY(ctx, A((void**)&ctx->contrib, cap * sizeof(float4)));
     ctx->cap = cap;
     return ILM_OK;
@@ -1001,7 +1001,7 @@ int ilm_debug_closest_hit(ilm_ctx* ctx, ilm_scene* scene, const float* rays, int
     std::unique_ptr<DChunk> hc(new DChunk());
     std::memset(hc.get(), 0, sizeof(DChunk));
     hc->reqs[0].scene = scene->d_scene;
-    std::vector<float4> o4, d4, h4, n4;
+    std::vector<float4> o4, d4, h4;
     std::vector<int> hi;
     for (int64_t base = 0; base < n; base += ctx->cap) {
         int64_t m = std::min<int64_t>(ctx->cap, n - base);
@@ -1024,16 +1024,13 @@ int ilm_debug_closest_hit(ilm_ctx* ctx, ilm_scene* scene, const float* rays, int
         HIPTRY(ctx, ilm_k_ctl_init(P, ctx->stream));
         HIPTRY(ctx, ilm_k_extend(P, 0, ctx->ovf, ctx->ext_grid, ctx->stream));
         h4.resize(m);
-        n4.resize(m);
         hi.resize(m);
         HIPTRY(ctx, hipMemcpyAsync(h4.data(), ctx->hits.h, m * sizeof(float4), hipMemcpyDeviceToHost, ctx->stream));
-        HIPTRY(ctx, hipMemcpyAsync(n4.data(), ctx->hits.n, m * sizeof(float4), hipMemcpyDeviceToHost, ctx->stream));
+        HIPTRY(ctx, hipMemcpyAsync(hi.data(), ctx->hits.inst, m * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
         HIPTRY(ctx, hipStreamSynchronize(ctx->stream));
         for (int64_t i = 0; i < m; i++) {
-            int32_t pr, in;
+            int32_t pr;
             std::memcpy(&pr, &h4[i].w, 4);
-            std::memcpy(&in, &n4[i].w, 4);
-            hi[i] = in;
             inst[base + i] = hi[i];
             prim[base + i] = hi[i] >= 0 ? pr : -1;
             t[base + i] = hi[i] == -1 ? 3.402823466e+38f : h4[i].x;

@@ -99,8 +99,7 @@ __device__ inline V3 conductor3(const GLOBAL DMat* m, float cosI) {
 }
 
 /* One path segment after its closest hit: src/renderer.cu:152-386. */
-__device__ inline int shade_path(const GLOBAL DScene* S, const float ray_bias, const float4 h4, const float4 n4,
-                                 int hinst, V3& ro, V3& rd, V3& color, int& medium, uint64_t& rs, uint64_t inc) {
+__device__ inline int shade_path(const GLOBAL DScene* S, const float ray_bias, const float4 h4, int hinst, V3& ro, V3& rd, V3& color, int& medium, uint64_t& rs, uint64_t inc) {
     const float tmin = h4.x;
     const bool is = hinst != -1;
     if (medium >= 0) {
@@ -141,7 +140,15 @@ __device__ inline int shade_path(const GLOBAL DScene* S, const float ray_bias, c
         const int prim = __float_as_int(h4.w);
         const float bu = h4.y, bv = h4.z;
         if (flags & ILM_INST_FACE_NORMALS) {
-            n = normalize3(mk3(n4.x, n4.y, n4.z));   /* cross(v1 - v0, v2 - v0) from the extend kernel */
+            /* cross(v1 - v0, v2 - v0) of the triangle's vertices, as src/primitives/mesh.cu:541 */
+            const GLOBAL DMesh* M = gp(S->meshes) + I->mesh;
+            const GLOBAL int* idx = gp(M->idx);
+            const GLOBAL float* pos = gp(M->pos);
+            int i0 = idx[3 * prim], i1 = idx[3 * prim + 1], i2 = idx[3 * prim + 2];
+            V3 p0 = mk3(pos[3 * i0], pos[3 * i0 + 1], pos[3 * i0 + 2]);
+            V3 p1 = mk3(pos[3 * i1], pos[3 * i1 + 1], pos[3 * i1 + 2]);
+            V3 p2 = mk3(pos[3 * i2], pos[3 * i2 + 1], pos[3 * i2 + 2]);
+            n = normalize3(cross3(sub3(p1, p0), sub3(p2, p0)));
         } else {
             const GLOBAL DMesh* M = gp(S->meshes) + I->mesh;
             const GLOBAL int* idx = gp(M->idx);
@@ -345,8 +352,9 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(KParams P, int depth) {
     __shared__ uint32_t block_base;
     /* SORT: the block's path states are staged in LDS and shaded in key order */
     constexpr int SB = SORT ? SHADE_BLOCK : 1;
-    __shared__ float4 s_o[SB], s_d[SB], s_c[SB], s_h[SB], s_n[SB];
+    __shared__ float4 s_o[SB], s_d[SB], s_c[SB], s_h[SB];
     __shared__ uint2 s_r[SB];
+    __shared__ int s_i[SB];
     __shared__ uint16_t s_ord[SB];
     __shared__ uint32_t s_cnt[SHADE_KEYS];
     GLOBAL DCtl* ctl = gpw(P.ctl);
@@ -359,7 +367,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(KParams P, int depth) {
     const GLOBAL float4* inc4 = gp(P.in.c);
     const GLOBAL uint2* inr = gp(P.in.rng);
     const GLOBAL float4* hh = gp(P.hits.h);
-    const GLOBAL float4* hn = gp(P.hits.n);
+    const GLOBAL int* hinst_in = gp(P.hits.inst);
     GLOBAL float4* outo = gpw(P.out.o);
     GLOBAL float4* outd = gpw(P.out.d);
     GLOBAL float4* outc = gpw(P.out.c);
@@ -369,8 +377,9 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(KParams P, int depth) {
     for (uint32_t tbase = blockIdx.x * (uint32_t)SHADE_BLOCK; tbase < count; tbase += gridDim.x * (uint32_t)SHADE_BLOCK) {
         uint32_t i = tbase + threadIdx.x;
         bool valid = i < count;
-        float4 o4 = make_float4(0, 0, 0, 0), d4 = o4, c4 = o4, h4 = o4, n4 = o4;
+        float4 o4 = make_float4(0, 0, 0, 0), d4 = o4, c4 = o4, h4 = o4;
         uint2 rg = make_uint2(0, 0);
+        int hinst = -1;
         if (valid) {
             if (CAM) {
                 camera_ray(P, i, o4, d4, rg);
@@ -380,12 +389,12 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(KParams P, int depth) {
                 rg = inr[i];
             }
             h4 = hh[i];
-            n4 = hn[i];
+            hinst = hinst_in[i];
         }
         if (SORT) {
-            const uint32_t key = valid ? shade_key((int)(f2u(o4.w) & 0xffffu) - 1, __float_as_int(n4.w)) : SHADE_KEY_INVALID;
+            const uint32_t key = valid ? shade_key((int)(f2u(o4.w) & 0xffffu) - 1, hinst) : SHADE_KEY_INVALID;
             const int t = threadIdx.x;
-            s_o[t] = o4; s_d[t] = d4; s_c[t] = c4; s_h[t] = h4; s_n[t] = n4; s_r[t] = rg;
+            s_o[t] = o4; s_d[t] = d4; s_c[t] = c4; s_h[t] = h4; s_i[t] = hinst; s_r[t] = rg;
             if (t < SHADE_KEYS) s_cnt[t] = 0u;
             __syncthreads();
             atomicAdd(&s_cnt[key], 1u);
@@ -398,14 +407,13 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(KParams P, int depth) {
             s_ord[atomicAdd(&s_cnt[key], 1u)] = (uint16_t)t;
             __syncthreads();
             const int j = s_ord[t];
-            o4 = s_o[j]; d4 = s_d[j]; c4 = s_c[j]; h4 = s_h[j]; n4 = s_n[j]; rg = s_r[j];
+            o4 = s_o[j]; d4 = s_d[j]; c4 = s_c[j]; h4 = s_h[j]; hinst = s_i[j]; rg = s_r[j];
             valid = tbase + (uint32_t)j < count;
         }
         int status = ST_DEAD;
         float4 oo = make_float4(0, 0, 0, 0), od = oo, oc = oo;
         uint2 org = make_uint2(0, 0);
         if (valid) {
-            int hinst = __float_as_int(n4.w);
             uint32_t tag = f2u(o4.w);
             int req = (int)(tag >> 16);
             int medium = (int)(tag & 0xffffu) - 1;
@@ -416,7 +424,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(KParams P, int depth) {
             uint64_t rs = ((uint64_t)rg.y << 32) | (uint64_t)rg.x;
             V3 ro = mk3(o4.x, o4.y, o4.z), rd = mk3(d4.x, d4.y, d4.z), color = mk3(c4.x, c4.y, c4.z);
             if (P.seg_count) gpw(P.seg_count)[pid] += 1;
-            status = shade_path(S, R->ray_bias, h4, n4, hinst, ro, rd, color, medium, rs, ilm_pcg32_inc(pixel));
+            status = shade_path(S, R->ray_bias, h4, hinst, ro, rd, color, medium, rs, ilm_pcg32_inc(pixel));
             if (status == ST_EMIT) gpw(P.contrib)[pid] = make_float4(color.x, color.y, color.z, 0.0f);
             if (status == ST_ALIVE && depth + 1 >= R->max_depth) status = ST_DEAD;   /* src/renderer.cu:549 */
             oo = make_float4(ro.x, ro.y, ro.z, u2f(((uint32_t)req << 16) | (uint32_t)(medium + 1)));

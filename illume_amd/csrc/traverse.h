@@ -54,22 +54,19 @@ __device__ inline int find_work_pid(const GLOBAL DWork* work, int n, uint32_t i)
 struct Best {
     float t, u, v;
     int inst, prim;
-    V3 n;               /* cross(v1 - v0, v2 - v0) of the closest triangle */
 };
 
 __device__ inline void best_init(Best& B) {
-    B.t = ILM_FLT_MAX; B.u = 0.0f; B.v = 0.0f; B.inst = -1; B.prim = -1; B.n = mk3(0.0f, 0.0f, 0.0f);
+    B.t = ILM_FLT_MAX; B.u = 0.0f; B.v = 0.0f; B.inst = -1; B.prim = -1;
 }
-/* record a closer triangle hit; the face normal is formed exactly as
- * src/primitives/mesh.cu:541 does, from the same vertex differences */
-__device__ inline void best_tri(Best& B, float t, float u, float v, int inst, float4 A, float4 Bq, float4 C) {
+/* record a closer triangle hit (A.w carries the original triangle index); shading forms the
+ * face normal from the mesh's own vertex arrays */
+__device__ inline void best_tri(Best& B, float t, float u, float v, int inst, float4 A) {
     B.t = t; B.u = u; B.v = v; B.inst = inst; B.prim = __float_as_int(A.w);
-    V3 v0 = mk3(A.x, A.y, A.z);
-    B.n = cross3(sub3(mk3(Bq.x, Bq.y, Bq.z), v0), sub3(mk3(C.x, C.y, C.z), v0));
 }
 __device__ inline void write_hit(const KParams& P, uint32_t i, const Best& B) {
     gpw(P.hits.h)[i] = make_float4(B.t, B.u, B.v, __int_as_float(B.prim));
-    gpw(P.hits.n)[i] = make_float4(B.n.x, B.n.y, B.n.z, __int_as_float(B.inst));
+    gpw(P.hits.inst)[i] = B.inst;
 }
 
 /* Reciprocal direction for the slab test. Components below 1e-30 in magnitude (axis-parallel
@@ -266,7 +263,7 @@ __device__ inline void leaf_tris(const GLOBAL float4* tris, int node, V3 o, V3 d
         float u = 0.0f, v = 0.0f, t;
         if (TRI) t = tri_wt(wt, mk3(A.x, A.y, A.z), mk3(Bq.x, Bq.y, Bq.z), mk3(C.x, C.y, C.z), o, u, v);
         else t = tri_mt(mk3(A.x, A.y, A.z), mk3(Bq.x, Bq.y, Bq.z), mk3(C.x, C.y, C.z), o, d, u, v);
-        if (t > 0.0f && t < B.t) best_tri(B, t, u, v, cur, A, Bq, C);
+        if (t > 0.0f && t < B.t) best_tri(B, t, u, v, cur, A);
     }
 }
 
