@@ -20,7 +20,19 @@
  */
 #include "traverse.h"
 
+#ifndef SHADE_BLOCK
 #define SHADE_BLOCK 512                       /* 8 waves share one compaction atomic */
+#endif
+/* Waves per SIMD the shading kernel's registers must allow: 6 caps them at 80 without spills,
+ * so three 512-thread blocks stay resident per CU instead of two */
+#ifndef SHADE_WPE
+#define SHADE_WPE 6
+#endif
+#if SHADE_WPE > 0
+#define SHADE_BOUNDS __launch_bounds__(SHADE_BLOCK) __attribute__((amdgpu_waves_per_eu(SHADE_WPE, 8)))
+#else
+#define SHADE_BOUNDS __launch_bounds__(SHADE_BLOCK)
+#endif
 #define SHADE_WAVES (SHADE_BLOCK / 64)
 
 __global__ void k_ctl_init(DCtl* ctl_, const DChunk* chunk) {
@@ -345,7 +357,7 @@ __device__ inline uint32_t shade_key(int medium, int hinst) {
 }
 
 template <int SORT, int CAM>
-__global__ __launch_bounds__(SHADE_BLOCK) void k_shade(KParams P, int depth) {
+__global__ SHADE_BOUNDS void k_shade(KParams P, int depth) {
     /* survivors of the block's 512 paths get their queue slots from one atomic: per-wave counts
      * meet in LDS, wave 0 lane 0 reserves the block's range, each wave takes its offset */
     __shared__ uint32_t wave_n[SHADE_WAVES];
