@@ -347,6 +347,51 @@ __device__ inline int shade_path(const GLOBAL DScene* S, const float ray_bias, c
 /* Shading key for the block-local sort: paths inside a medium, misses, spheres and instances
  * each get their own buckets (an instance stands in for its material), so a wave mostly runs
  * one material's code. INVALID sorts the block's empty slots last. */
+/* Stable split of a shading block's threads by key (0 <= key < NK): the thread's position when
+ * the block is ordered by key, then wave, then lane. Each wave counts its keys with one ballot
+ * per distinct key present (usually one to three); wave 0 turns the waves x keys counts into
+ * start positions. Every thread of the block calls it with all lanes active. On return
+ * s_wk[k] (wave 0's entry for key k) is the number of threads with a key below k. */
+template <int NK>
+__device__ inline uint32_t block_split(uint32_t key, uint32_t* s_wk) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const unsigned long long lt = (1ull << lane) - 1ull;
+    if (lane < NK) s_wk[wid * NK + lane] = 0u;
+    unsigned long long rem = __ballot(1);
+    uint32_t rank = 0;
+    while (rem) {
+        const int leader = __builtin_ctzll(rem);
+        const uint32_t k = (uint32_t)__builtin_amdgcn_readlane((int)key, leader);
+        const unsigned long long m = __ballot(key == k);
+        if (key == k) rank = (uint32_t)__popcll(m & lt);
+        if (lane == leader) s_wk[wid * NK + k] = (uint32_t)__popcll(m);
+        rem &= ~m;
+    }
+    __syncthreads();
+    if (wid == 0) {
+        uint32_t col[SHADE_WAVES];
+        uint32_t tot = 0;
+#pragma unroll
+        for (int w = 0; w < SHADE_WAVES; w++) {
+            col[w] = lane < NK ? s_wk[w * NK + lane] : 0u;
+            tot += col[w];
+        }
+        uint32_t incl = tot;
+#pragma unroll
+        for (int d = 1; d < 32; d <<= 1) {
+            const uint32_t v = (uint32_t)__shfl_up((int)incl, d);
+            if (lane >= d) incl += v;
+        }
+        uint32_t run = incl - tot;
+        if (lane < NK) {
+#pragma unroll
+            for (int w = 0; w < SHADE_WAVES; w++) { s_wk[w * NK + lane] = run; run += col[w]; }
+        }
+    }
+    __syncthreads();
+    return s_wk[wid * NK + key] + rank;
+}
+
 #define SHADE_KEYS 17
 #define SHADE_KEY_INVALID 16
 __device__ inline uint32_t shade_key(int medium, int hinst) {
@@ -368,7 +413,7 @@ __global__ SHADE_BOUNDS void k_shade(KParams P, int depth) {
     __shared__ uint2 s_r[SB];
     __shared__ int s_i[SB];
     __shared__ uint16_t s_ord[SB];
-    __shared__ uint32_t s_cnt[SHADE_KEYS];
+    __shared__ uint32_t s_wk[SORT ? SHADE_WAVES * SHADE_KEYS : 1];
     GLOBAL DCtl* ctl = gpw(P.ctl);
     const uint32_t count = ctl->count[depth];
     const int lane = threadIdx.x & 63;
@@ -407,16 +452,7 @@ __global__ SHADE_BOUNDS void k_shade(KParams P, int depth) {
             const uint32_t key = valid ? shade_key((int)(f2u(o4.w) & 0xffffu) - 1, hinst) : SHADE_KEY_INVALID;
             const int t = threadIdx.x;
             s_o[t] = o4; s_d[t] = d4; s_c[t] = c4; s_h[t] = h4; s_i[t] = hinst; s_r[t] = rg;
-            if (t < SHADE_KEYS) s_cnt[t] = 0u;
-            __syncthreads();
-            atomicAdd(&s_cnt[key], 1u);
-            __syncthreads();
-            if (t == 0) {
-                uint32_t run = 0;
-                for (int k = 0; k < SHADE_KEYS; k++) { const uint32_t c = s_cnt[k]; s_cnt[k] = run; run += c; }
-            }
-            __syncthreads();
-            s_ord[atomicAdd(&s_cnt[key], 1u)] = (uint16_t)t;
+            s_ord[block_split<SHADE_KEYS>(key, s_wk)] = (uint16_t)t;
             __syncthreads();
             const int j = s_ord[t];
             o4 = s_o[j]; d4 = s_d[j]; c4 = s_c[j]; h4 = s_h[j]; hinst = s_i[j]; rg = s_r[j];
@@ -452,18 +488,14 @@ __global__ SHADE_BOUNDS void k_shade(KParams P, int depth) {
              * atomic reserves the block's range */
             const int t = threadIdx.x;
             const uint32_t okey = alive ? ((od.x < 0.0f ? 1u : 0u) | (od.y < 0.0f ? 2u : 0u) | (od.z < 0.0f ? 4u : 0u)) : 8u;
-            if (t < 9) s_cnt[t] = 0u;
-            __syncthreads();
-            atomicAdd(&s_cnt[okey], 1u);
-            __syncthreads();
+            const uint32_t opos = block_split<9>(okey, s_wk);
             if (t == 0) {
-                uint32_t run = 0;
-                for (int k = 0; k < 8; k++) { const uint32_t c = s_cnt[k]; s_cnt[k] = run; run += c; }
+                const uint32_t run = s_wk[8];   /* survivors: keys 0..7 */
                 block_base = run ? atomicAdd(&ctl->count[depth + 1], run) : 0u;
             }
             __syncthreads();
-            slot = alive ? block_base + atomicAdd(&s_cnt[okey], 1u) : 0u;
-            __syncthreads();   /* s_cnt, block_base and the staging arrays are rewritten next trip */
+            slot = alive ? block_base + opos : 0u;
+            __syncthreads();   /* s_wk, block_base and the staging arrays are rewritten next trip */
         } else {
             /* compaction: ballot + prefix within the wave, wave offsets within the block, one
              * atomic per block */
