@@ -71,16 +71,25 @@ def workload_name(cfg, spec, st):
                                                        names.get(cfg, "%d tris" % tris))
 
 
-def traffic_from_profiles(cfg):
+def traffic_from_profiles(cfg, layout):
     """HBM bytes per extend launch measured by rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE, scaled by
-    the copy-kernel calibration), recorded in profiles/extend_config<cfg>_traffic.txt."""
+    the copy-kernel calibration), recorded in profiles/extend_config<cfg>_traffic.txt for the
+    node layout named there; None when no profile covers this configuration and layout."""
     path = os.path.join(ROOT, "profiles", "extend_config%d_traffic.txt" % cfg)
     if not os.path.exists(path):
         return None
+    value, prof_layout = None, "fp32"
     for line in open(path):
         if line.startswith("traffic_bytes_per_launch:"):
-            return float(line.split(":")[1])
-    return None
+            value = float(line.split(":")[1])
+        if line.startswith("bvh_layout:"):
+            prof_layout = line.split(":")[1].strip()
+    return value if prof_layout == layout else None
+
+
+# bytes a traversal reads per child-box test: an FP32 child box (24 B) and its child index; for
+# the 8-bit BVH4 a quarter of the 64-byte node
+BOX_BYTES = {"fp32": 28.0, "quant8": 16.0}
 
 
 def cpu_baseline(spec, st, target_s=15.0, target_1core_s=5.0):
@@ -213,25 +222,23 @@ def bench_plain(args):
     wall = time.perf_counter() - t_start
     value = segs / (dev_ms / 1e3) / 1e6
 
-    # traversal test counts on the same workload, FP32 nodes (a separate, untimed render)
+    layout = "quant8" if ctx.settings()["bvh_layout"] == 1 else "fp32"
+    # traversal test counts on the same workload and node layout (a separate, untimed render)
     roof = None
     if not args.no_count:
-        copts = ctx_opts(args)
-        copts["bvh_layout"] = "fp32"
-        cctx = illume_amd.Context(0, count_tests=1, **copts)
-        if tuned:
-            cctx.tune(**tuned)
+        cctx = illume_amd.Context(0, count_tests=1, **ctx_opts(args))
+        cctx.tune(**ctx.settings())
         cscene = cctx.scene(spec)
         _, _, cinfo = cctx.render(cscene, p, want_accum=False, want_pixels=False)
         n_box = cinfo["box_tests"] / max(1, cinfo["segments"])
         n_tri = cinfo["tri_tests"] / max(1, cinfo["segments"])
         cscene.free()
         cctx.close()
-        b_seg = 28.0 * n_box + 36.0 * n_tri + 128.0
+        b_seg = BOX_BYTES[layout] * n_box + 36.0 * n_tri + 128.0
         per_launch_bytes = b_seg * segs / max(1, ext_launches)
         avg_launch_s = ext_ms / 1e3 / max(1, ext_launches)
         achieved = per_launch_bytes / avg_launch_s / 1e9
-        traffic = traffic_from_profiles(args.config)
+        traffic = traffic_from_profiles(args.config, layout)
         roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": "k_extend", "b_seg": round(b_seg, 1), "n_box": round(n_box, 2), "n_tri": round(n_tri, 2),
@@ -250,7 +257,7 @@ def bench_plain(args):
             log("cpu baseline failed: %r" % (e,))
 
     config = {"workload": workload_name(args.config, spec, st), "width": st["width"], "height": st["height"],
-              "spp": st["spp"], "max_depth": st["max_depth"], "bvh_layout": args.layout, "tri_test": args.tri,
+              "spp": st["spp"], "max_depth": st["max_depth"], "bvh_layout": layout, "tri_test": args.tri,
               "segments_per_step": segs // max(1, args.steps), "parallelism": "1 GPU",
               "launch_shape": ctx.settings(), "autotuned": tuned is not None}
     out = base_line(args, METRIC, value, "Mrays/s", dev_ms / args.steps, config)

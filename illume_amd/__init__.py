@@ -123,7 +123,7 @@ class Context:
         self.h = h
         self.opts = o
 
-    TUNABLE = ("extend_mode", "refill_min", "ext_fetch", "leaf_wait_pct", "shade_sort", "sched_fair",
+    TUNABLE = ("bvh_layout", "extend_mode", "refill_min", "ext_fetch", "leaf_wait_pct", "shade_sort", "sched_fair",
                "sample_group", "tile_rows", "chunk_passes", "paths_in_flight")
 
     def tune(self, **kw):
@@ -133,6 +133,8 @@ class Context:
         for k, v in kw.items():
             if k not in self.TUNABLE:
                 raise ValueError("not tunable: %s" % k)
+            if k == "bvh_layout" and isinstance(v, str):
+                v = BVH[v]
             setattr(o, k, int(v))
         _check(lib().ilm_tune(self.h, C.byref(o)), self.h)
         self.opts = o

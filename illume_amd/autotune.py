@@ -2,8 +2,11 @@
 
 The tunables change how paths are laid out and queued, never what is computed: every choice
 gives bit-identical images (samples are summed per pixel in pass order whatever the path order
-or chunking), so the tuner only measures time.
+or chunking), except bvh_layout, whose two trees can resolve a distance tie between two
+triangles differently. The tuner only measures time.
 
+    bvh_layout       0: FP32 BVH2 nodes, 1: 8-bit BVH4 nodes (both are resident for every scene;
+                     closest hits agree except at distance ties)
     extend_mode      1: each wave traces one batch of 64 rays to completion
                      2: idle lanes refill from a wave-local queue at BLAS leaves
                      3 (and 0, the default): idle lanes refill after every traversal step
@@ -20,7 +23,8 @@ or chunking), so the tuner only measures time.
     chunk_passes     passes of one request per chunk (0: as many as fit)
 
 The search is coordinate descent from the context's current settings: each knob in turn is set
-to every candidate value with the others held, and the fastest is kept. The objective is device
+to every candidate value with the others held, and the fastest is kept if it beats the current
+choice by more than min_gain (run-to-run noise does not move the settings). The objective is device
 time per segment of a caller-supplied trial (normally the workload itself or a reduced-spp copy
 of it), the best of `repeats` runs.
 """
@@ -29,6 +33,7 @@ import time
 # A knob is a setting name, or a tuple of names searched jointly (refill_min only matters for
 # the refill modes, so it is searched together with extend_mode).
 DEFAULT_SPACE = (
+    ("bvh_layout", (0, 1)),
     ("sample_group", (1, 4, 16, 64)),
     ("tile_rows", (1, 4, 8)),
     (("extend_mode", "refill_min"), ((1, 0), (2, 16), (2, 32), (3, 8), (3, 16), (3, 48))),
@@ -61,7 +66,7 @@ def _measure(ctx, run, repeats):
     return best
 
 
-def autotune(ctx, run, space=DEFAULT_SPACE, repeats=2, log=None, budget_s=60.0):
+def autotune(ctx, run, space=DEFAULT_SPACE, repeats=2, log=None, budget_s=60.0, min_gain=0.02):
     """Tune ctx in place. run(ctx) renders the trial workload and returns (segments, device_ms).
     Returns (best settings dict, list of trial records)."""
     t0 = time.time()
@@ -84,7 +89,7 @@ def autotune(ctx, run, space=DEFAULT_SPACE, repeats=2, log=None, budget_s=60.0):
             trials.append(dict(cand, rate=rate))
             if log:
                 log("autotune %s=%s: %.1f Mseg/s" % (knob, v, rate / 1e3))
-            if rate > best_rate:
+            if rate > best_rate * (1.0 + min_gain):
                 best_rate, cur = rate, cand
         ctx.tune(**cur)
     if log:

@@ -544,9 +544,10 @@ void ilm_close(ilm_ctx* ctx) {
     ctx_unref(ctx);
 }
 
-/* Retune an idle context: extend_mode, refill_min, ext_fetch, leaf_wait_pct, sample_group, tile_rows,
- * chunk_passes and paths_in_flight take the values in *o (the other fields are ignored). Path buffers are
- * reallocated when the chunk size changes; captured graphs are dropped. */
+/* Retune an idle context: bvh_layout, extend_mode, refill_min, ext_fetch, leaf_wait_pct, shade_sort,
+ * sched_fair, sample_group, tile_rows, chunk_passes and paths_in_flight take the values in *o (the
+ * other fields are ignored). Every scene carries both node layouts, so a layout change needs no
+ * upload. Path buffers are reallocated when the chunk size changes; captured graphs are dropped. */
 int ilm_tune(ilm_ctx* ctx, const ilm_opts* o) {
     if (!ctx || !o) return fail(ctx, ILM_E_INVALID, "null argument");
     if (!opts_valid(*o)) return fail(ctx, ILM_E_INVALID, "bad ilm_opts field");
@@ -557,8 +558,8 @@ int ilm_tune(ilm_ctx* ctx, const ilm_opts* o) {
     HIPTRY(ctx, hipStreamSynchronize(ctx->stream));
     for (auto& kv : ctx->graphs) (void)hipGraphExecDestroy(kv.second);
     ctx->graphs.clear();
-    if (ext_mode_of(*o) != ext_mode_of(ctx->opts)) {
-        const int occ = ilm_k_extend_occupancy(ctx->opts.tri_test, ctx->opts.bvh_layout, ext_mode_of(*o));
+    if (ext_mode_of(*o) != ext_mode_of(ctx->opts) || o->bvh_layout != ctx->opts.bvh_layout) {
+        const int occ = ilm_k_extend_occupancy(ctx->opts.tri_test, o->bvh_layout, ext_mode_of(*o));
         const int grid = ctx->n_cu * std::max(1, occ);
         if (grid > ctx->ovf_grid) {
             /* the spill area is sized for the grid: grow it before the grid grows (on failure
@@ -573,6 +574,7 @@ int ilm_tune(ilm_ctx* ctx, const ilm_opts* o) {
         }
         ctx->ext_grid = grid;
     }
+    ctx->opts.bvh_layout = o->bvh_layout;
     ctx->opts.extend_mode = o->extend_mode;
     ctx->opts.sample_group = o->sample_group;
     ctx->opts.tile_rows = o->tile_rows;
@@ -849,6 +851,15 @@ static int validate_params(ilm_ctx* ctx, const ilm_render_params* p) {
     return ILM_OK;
 }
 
+/* A scene uploaded under one layout may have a BLAS too deep for the other layout's stack
+ * (ilm_tune can switch layouts after the upload). */
+static int check_scene_stack(ilm_ctx* ctx, const ilm_scene* s) {
+    const int cap = ilm_k_lds_stack() + ilm_k_ovf_stack();
+    const int bound = ctx->opts.bvh_layout == ILM_BVH_QUANT8 ? s->stack_bound_q : s->stack_bound_fp32;
+    if (bound > cap) return fail(ctx, ILM_E_UNSUPPORTED, "BVH too deep for the traversal stack");
+    return ILM_OK;
+}
+
 static void accum4_to_rgb(const std::vector<float4>& a4, float* rgb, int64_t n) {
     for (int64_t i = 0; i < n; i++) { rgb[3 * i] = a4[i].x; rgb[3 * i + 1] = a4[i].y; rgb[3 * i + 2] = a4[i].z; }
 }
@@ -939,6 +950,7 @@ int ilm_render(ilm_ctx* ctx, ilm_scene* scene, const ilm_render_params* params, 
                ilm_render_info* info) {
     if (!ctx || !scene) return fail(ctx, ILM_E_INVALID, "null context or scene");
     int rc = validate_params(ctx, params);
+    if (!rc) rc = check_scene_stack(ctx, scene);
     if (rc) return rc;
     Job j;
     j.scene = scene;
@@ -960,6 +972,8 @@ int ilm_render_batch(ilm_ctx* ctx, int32_t n, ilm_scene* const* scenes, const il
         int rc = validate_params(ctx, &params[i]);
         if (rc) return rc;
         if (!scenes[i]) return fail(ctx, ILM_E_INVALID, "null scene in batch");
+        rc = check_scene_stack(ctx, scenes[i]);
+        if (rc) return rc;
         js[i].scene = scenes[i];
         js[i].p = params[i];
         jobs[i] = &js[i];
@@ -996,6 +1010,7 @@ int ilm_render_batch(ilm_ctx* ctx, int32_t n, ilm_scene* const* scenes, const il
 int ilm_debug_closest_hit(ilm_ctx* ctx, ilm_scene* scene, const float* rays, int32_t n, int32_t* inst,
                           int32_t* prim, float* t) {
     if (!ctx || !scene || !rays || n < 0 || !inst || !prim || !t) return fail(ctx, ILM_E_INVALID, "bad arguments");
+    if (int rc = check_scene_stack(ctx, scene)) return rc;
     std::lock_guard<std::mutex> dev(ctx->dev_mu);
     HIPTRY(ctx, hipSetDevice(ctx->device));
     std::unique_ptr<DChunk> hc(new DChunk());
@@ -1043,6 +1058,7 @@ int ilm_debug_path_radiance(ilm_ctx* ctx, ilm_scene* scene, const ilm_render_par
                             const uint32_t* pixel_pass, int32_t n, float* rgb, int32_t* segments) {
     if (!ctx || !scene || !pixel_pass || n < 0 || !rgb) return fail(ctx, ILM_E_INVALID, "bad arguments");
     int rc = validate_params(ctx, params);
+    if (!rc) rc = check_scene_stack(ctx, scene);
     if (rc) return rc;
     std::lock_guard<std::mutex> dev(ctx->dev_mu);
     HIPTRY(ctx, hipSetDevice(ctx->device));
@@ -1288,6 +1304,7 @@ int ilm_submit(ilm_ctx* ctx, ilm_scene* scene, const ilm_render_params* params, 
     if (!ctx || !scene || !out) return fail(ctx, ILM_E_INVALID, "null argument");
     *out = nullptr;
     int rc = validate_params(ctx, params);
+    if (!rc) rc = check_scene_stack(ctx, scene);
     if (rc) return rc;
     const int64_t wh = (int64_t)params->width * params->height;
     Job* j = new Job();

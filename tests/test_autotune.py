@@ -5,7 +5,7 @@ from illume_amd.autotune import autotune
 
 class FakeCtx:
     def __init__(self):
-        self.s = dict(extend_mode=0, refill_min=0, ext_fetch=0, leaf_wait_pct=0, shade_sort=0, sched_fair=0, sample_group=0, tile_rows=0, chunk_passes=0, paths_in_flight=0)
+        self.s = dict(bvh_layout=0, extend_mode=0, refill_min=0, ext_fetch=0, leaf_wait_pct=0, shade_sort=0, sched_fair=0, sample_group=0, tile_rows=0, chunk_passes=0, paths_in_flight=0)
         self.tunes = 0
 
     def settings(self):
@@ -44,3 +44,16 @@ def test_budget_stops_search():
     ctx = FakeCtx()
     best, trials = autotune(ctx, lambda c: (1.0, 1.0), repeats=1, budget_s=0.0)
     assert len(trials) == 1 and best == FakeCtx().settings()
+
+
+def test_gains_within_noise_do_not_move_settings():
+    """A candidate must beat the current choice by more than min_gain: a 0.3 % better layout is
+    left alone (run-to-run noise), a 5 % better one is taken."""
+    for gain, want in [(0.003, 0), (0.05, 1)]:
+        ctx = FakeCtx()
+
+        def run(c):
+            return 1000.0 * (1.0 + gain * c.settings()["bvh_layout"]), 1.0
+
+        best, _ = autotune(ctx, run, repeats=1)
+        assert best["bvh_layout"] == want

@@ -174,6 +174,27 @@ def test_first_sample_offsets_compose(ctx, scenes):
     assert np.allclose(full, a + b, rtol=1e-5, atol=1e-6)
 
 
+def test_layout_switch_by_tune_matches_a_context_opened_with_it(ctx_quant, scenes):
+    """Every scene carries both node layouts: retuning a live context to the 8-bit BVH4 renders
+    what a context opened with it renders, and switching back restores the FP32 image."""
+    spec, st, _ = scenes["gold"]
+    p = _p(st, width=64, height=48, spp=3, max_depth=6)
+    with illume_amd.Context(0) as c:
+        s = c.scene(spec)
+        a, _, ia = c.render(s, p)
+        c.tune(bvh_layout="quant8")
+        assert c.settings()["bvh_layout"] == 1
+        b, _, ib = c.render(s, p)
+        c.tune(bvh_layout="fp32")
+        a2, _, ia2 = c.render(s, p)
+        s.free()
+    sq = ctx_quant.scene(spec)
+    q, _, iq = ctx_quant.render(sq, p)
+    sq.free()
+    assert np.array_equal(b, q) and ib["segments"] == iq["segments"]
+    assert np.array_equal(a, a2) and ia["segments"] == ia2["segments"]
+
+
 def test_quantized_and_watertight_renders_agree(ctx_quant, ctx_wt, ctx, scenes):
     spec, st, s = scenes["tiny"]
     p = _p(st, width=64, height=48, spp=4)
