@@ -205,6 +205,12 @@ void qnode_child_box(const QNode& n, int k, float lo[3], float hi[3]) {
     }
 }
 
+int qnodes_max_exponent(const std::vector<QNode>& nodes) {
+    int e = 0;
+    for (const QNode& n : nodes) e = std::max(e, (int)std::max(n.ex, std::max(n.ey, n.ez)));
+    return e - 127;
+}
+
 static void quantize_node(QNode& q, const Box& nb, const Box* cb, int nc) {
     const float org[3] = {nb.lo[0], nb.lo[1], nb.lo[2]};
     q.ox = org[0]; q.oy = org[1]; q.oz = org[2];

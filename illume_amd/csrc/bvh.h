@@ -65,6 +65,12 @@ struct LayoutQ {
 };
 LayoutQ make_layout_q(const Tree& t);
 
+/* Largest child-box quantum of the 8-bit layout the traversal accepts (2^24): with
+ * |1/d| <= 1e30 the kernel's per-node products quantum * (1/d) stay finite. */
+#define ILM_QUANT_MAX_EXP 24
+/* Largest quantum exponent over a set of QNodes (-127 when empty). */
+int qnodes_max_exponent(const std::vector<QNode>& nodes);
+
 /* Dequantise a QNode's child k box exactly as the kernel does. */
 void qnode_child_box(const QNode& n, int k, float lo[3], float hi[3]);
 

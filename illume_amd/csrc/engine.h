@@ -31,6 +31,7 @@ struct ilm_scene {
     int64_t mesh_tris = 0;
     int stack_bound_fp32 = 0;
     int stack_bound_q = 0;
+    int quant_max_exp = -127;   /* largest child-box quantum exponent of the 8-bit trees */
 };
 
 /* One render request. */

@@ -179,6 +179,7 @@ int ilm_scene_upload(ilm_ctx* ctx, const ilm_scene_desc* d, ilm_scene** out) {
         }
         blas_depth = std::max(blas_depth, bvh.depth);
         blas_qdepth = std::max(blas_qdepth, bvh.qdepth);
+        s->quant_max_exp = std::max(s->quant_max_exp, qnodes_max_exponent(bvh.qnodes));
         std::vector<float4> tris(3 * (size_t)m.num_triangles);
         for (int k = 0; k < m.num_triangles; k++) {
             int t = bvh.order[k];
@@ -267,6 +268,7 @@ int ilm_scene_upload(ilm_ctx* ctx, const ilm_scene_desc* d, ilm_scene** out) {
         LayoutQ TQ = make_layout_q(tt);
         tlas_depth = tt.max_depth;
         tlas_qdepth = TQ.max_depth;
+        s->quant_max_exp = std::max(s->quant_max_exp, qnodes_max_exponent(TQ.nodes));
         tlas_prims = tt.order;
         BNode* bn; QNode* qn; int* tp;
         HIPTRY(ctx, dupload(s.get(), &bn, T2.nodes.data(), T2.nodes.size()));
@@ -282,6 +284,8 @@ int ilm_scene_upload(ilm_ctx* ctx, const ilm_scene_desc* d, ilm_scene** out) {
     s->stack_bound_q = 3 * (tlas_qdepth + 1) + 1 + 3 * (blas_qdepth + 1);
     if (s->stack_bound_fp32 > cap || (ctx->opts.bvh_layout == ILM_BVH_QUANT8 && s->stack_bound_q > cap))
         return fail(ctx, ILM_E_UNSUPPORTED, "BVH too deep for the traversal stack");
+    if (ctx->opts.bvh_layout == ILM_BVH_QUANT8 && s->quant_max_exp > ILM_QUANT_MAX_EXP)
+        return fail(ctx, ILM_E_UNSUPPORTED, "scene extent too large for 8-bit child boxes");
 
     /* ---- materials, mediums, spheres, environment, camera ---- */
     std::vector<DMat> dmats(d->num_materials);
@@ -851,12 +855,15 @@ static int validate_params(ilm_ctx* ctx, const ilm_render_params* p) {
     return ILM_OK;
 }
 
-/* A scene uploaded under one layout may have a BLAS too deep for the other layout's stack
- * (ilm_tune can switch layouts after the upload). */
+/* A scene uploaded under one layout may not suit the other (ilm_tune can switch layouts after
+ * the upload): a BLAS too deep for its stack, or 8-bit child boxes past the largest quantum. */
 static int check_scene_stack(ilm_ctx* ctx, const ilm_scene* s) {
     const int cap = ilm_k_lds_stack() + ilm_k_ovf_stack();
-    const int bound = ctx->opts.bvh_layout == ILM_BVH_QUANT8 ? s->stack_bound_q : s->stack_bound_fp32;
-    if (bound > cap) return fail(ctx, ILM_E_UNSUPPORTED, "BVH too deep for the traversal stack");
+    const bool q = ctx->opts.bvh_layout == ILM_BVH_QUANT8;
+    if ((q ? s->stack_bound_q : s->stack_bound_fp32) > cap)
+        return fail(ctx, ILM_E_UNSUPPORTED, "BVH too deep for the traversal stack");
+    if (q && s->quant_max_exp > ILM_QUANT_MAX_EXP)
+        return fail(ctx, ILM_E_UNSUPPORTED, "scene extent too large for 8-bit child boxes");
     return ILM_OK;
 }
 

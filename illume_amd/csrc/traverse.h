@@ -118,15 +118,25 @@ __device__ inline void box4q(const GLOBAL QNode* np, V3 id, V3 noid, float tmax,
     const uint32_t lx = f2u(w1.x), ly = f2u(w1.y), lz = f2u(w1.z);
     const uint32_t hx = f2u(w1.w), hy = f2u(w2.x), hz = f2u(w2.y);
     hitmask = 0;
+    /* slab distance of plane q: (o_n + q*s - o) / d = q * (s/d) + (o_n/d - o/d). s is a power of
+     * two, so A = s * (1/d) is exact while it is a normal float, and a plane costs one fma on top
+     * of the byte conversion; its rounding error (that of B = fma(o_n, 1/d, -o/d), then of the
+     * fma) is no larger than the direct form's, whose plane o_n + q*s is rounded at the
+     * magnitude of o_n. |1/d| <= 1e30 (safe_dir) and s <= 2^ILM_QUANT_MAX_EXP (scenes that
+     * need a larger quantum do not get the 8-bit layout) keep A below 2^124; a subnormal A only
+     * adds an absolute error below 2^-140. */
+    const float ax = sx * id.x, ay = sy * id.y, az = sz * id.z;
+    const float bx = __builtin_fmaf(w0.x, id.x, noid.x), by = __builtin_fmaf(w0.y, id.y, noid.y);
+    const float bz = __builtin_fmaf(w0.z, id.z, noid.z);
 #pragma unroll
     for (int k = 0; k < 4; k++) {
         const int sh = 8 * k;
-        float bx0 = w0.x + (float)((lx >> sh) & 0xffu) * sx, bx1 = w0.x + (float)((hx >> sh) & 0xffu) * sx;
-        float by0 = w0.y + (float)((ly >> sh) & 0xffu) * sy, by1 = w0.y + (float)((hy >> sh) & 0xffu) * sy;
-        float bz0 = w0.z + (float)((lz >> sh) & 0xffu) * sz, bz1 = w0.z + (float)((hz >> sh) & 0xffu) * sz;
-        float x0 = __builtin_fmaf(bx0, id.x, noid.x), x1 = __builtin_fmaf(bx1, id.x, noid.x);
-        float y0 = __builtin_fmaf(by0, id.y, noid.y), y1 = __builtin_fmaf(by1, id.y, noid.y);
-        float z0 = __builtin_fmaf(bz0, id.z, noid.z), z1 = __builtin_fmaf(bz1, id.z, noid.z);
+        const float x0 = __builtin_fmaf((float)((lx >> sh) & 0xffu), ax, bx);
+        const float x1 = __builtin_fmaf((float)((hx >> sh) & 0xffu), ax, bx);
+        const float y0 = __builtin_fmaf((float)((ly >> sh) & 0xffu), ay, by);
+        const float y1 = __builtin_fmaf((float)((hy >> sh) & 0xffu), ay, by);
+        const float z0 = __builtin_fmaf((float)((lz >> sh) & 0xffu), az, bz);
+        const float z1 = __builtin_fmaf((float)((hz >> sh) & 0xffu), az, bz);
         float tn = __builtin_fmaxf(__builtin_fmaxf(__builtin_fminf(x0, x1), __builtin_fminf(y0, y1)),
                                    __builtin_fmaxf(__builtin_fminf(z0, z1), 0.0f));
         float tf = __builtin_fminf(__builtin_fminf(__builtin_fmaxf(x0, x1), __builtin_fmaxf(y0, y1)),
