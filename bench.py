@@ -190,6 +190,8 @@ def bench_plain(args):
     log("scene upload + BVH build %.2f s, %.1f MB on device" % (upload_s, scene.device_bytes / 1e6))
     p = render_params(**st)
 
+    if args.shape:
+        ctx.tune(**{k: int(v) for k, v in (kv.split("=") for kv in args.shape.split(","))})
     tuned = None
     if not args.no_tune:
         from illume_amd.autotune import autotune
@@ -479,7 +481,8 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", type=int, default=1)
-    ap.add_argument("--layout", default="fp32", choices=["fp32", "quant8"])
+    ap.add_argument("--layout", default="quant8", choices=["fp32", "quant8"],
+                    help="node layout the run starts from (the autotuner may switch it)")
     ap.add_argument("--tri", default="mt", choices=["mt", "watertight"])
     ap.add_argument("--paths", type=int, default=0, help="paths in flight per chunk (0: library default)")
     ap.add_argument("--extend-mode", type=int, default=0,
@@ -487,6 +490,9 @@ def main():
     ap.add_argument("--sample-group", type=int, default=None)
     ap.add_argument("--tile-rows", type=int, default=None)
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (library default launch shape)")
+    ap.add_argument("--shape", default=None,
+                    help="launch shape to apply before measuring, as key=value[,key=value] over the tunables "
+                         "(e.g. ext_fetch=128,leaf_wait_pct=50); with --no-tune this reproduces a tuned run")
     ap.add_argument("--tune-budget", type=float, default=30.0)
     ap.add_argument("--requests", type=int, default=0, help="configs 3/4: number of requests")
     ap.add_argument("--no-rmse", action="store_true")
