@@ -106,12 +106,9 @@ __device__ inline void box2(const float4 a, const float4 b, const float4 c, V3 i
 }
 
 /* Up to four dequantised child boxes of a QNode (dequantisation as bvh.cpp's qnode_child_box). */
-__device__ inline void box4q(const GLOBAL QNode* np, V3 id, V3 noid, float tmax, float* tn_out, uint32_t& hitmask,
-                             int& nchild, int4& ch) {
-    const float4 w0 = ((const GLOBAL float4*)np)[0];   /* ox oy oz | ex ey ez nchild */
-    const float4 w1 = ((const GLOBAL float4*)np)[1];   /* qlo[3][4] qhi[0][4] ... */
-    const float4 w2 = ((const GLOBAL float4*)np)[2];
-    ch = ((const GLOBAL int4*)np)[3];
+/* w0 = ox oy oz | ex ey ez nchild, w1 = qlo[x][0..3] qlo[y] qlo[z] qhi[x], w2 = qhi[y] qhi[z] */
+__device__ inline void box4q(const float4 w0, const float4 w1, const float4 w2, V3 id, V3 noid, float tmax,
+                             float* tn_out, uint32_t& hitmask, int& nchild) {
     const uint32_t e = f2u(w0.w);
     const float sx = u2f((e & 0xffu) << 23), sy = u2f(((e >> 8) & 0xffu) << 23), sz = u2f(((e >> 16) & 0xffu) << 23);
     nchild = (int)(e >> 24);
@@ -143,7 +140,10 @@ __device__ inline void box4q(const GLOBAL QNode* np, V3 id, V3 noid, float tmax,
                                    __builtin_fmaxf(z0, z1));
         tn = tn * BOX_LO;
         tn_out[k] = tn;
-        if (k < nchild && tn <= tf * BOX_HI && tn <= tmax) hitmask |= 1u << k;
+        /* evaluated without short-circuit: a branch per child would split the walk into
+         * exec-masked blocks (and make the compiler re-canonicalise the slab values in each) */
+        const uint32_t hit = (uint32_t)(k < nchild) & (uint32_t)(tn <= tf * BOX_HI) & (uint32_t)(tn <= tmax);
+        hitmask |= hit << k;
     }
 }
 
@@ -278,16 +278,16 @@ __device__ inline void leaf_tris(const GLOBAL float4* tris, int node, V3 o, V3 d
 }
 
 /* Push the hit children of an inner node nearest-last and continue with the nearest; returns
- * false when no child is hit. */
+ * false when no child is hit. n0..n2 are the node's box words (FP32 BNode a, b, c or the QNode's
+ * first three), ch its child codes. */
 template <int QUANT, int OVF, int COUNT>
-__device__ inline bool inner_step(const GLOBAL BNode* nodes, const GLOBAL QNode* qnodes, int& node, V3 id, V3 noid,
-                                  float tmax, Stack<OVF>& st, TravCount* tc) {
+__device__ inline bool inner_step_words(const float4 n0, const float4 n1, const float4 n2, const int4 ch, int& node,
+                                        V3 id, V3 noid, float tmax, Stack<OVF>& st, TravCount* tc) {
     if (QUANT) {
         float tn[4];
         uint32_t hm;
         int nc;
-        int4 ch;
-        box4q(qnodes + node, id, noid, tmax, tn, hm, nc, ch);
+        box4q(n0, n1, n2, id, noid, tmax, tn, hm, nc);
         if (COUNT) tc->boxes += nc;
         if (hm == 0) return false;
         /* order the hit children near to far with a 4-input sorting network (missed children
@@ -315,12 +315,9 @@ __device__ inline bool inner_step(const GLOBAL BNode* nodes, const GLOBAL QNode*
         node = c0;
         return true;
     } else {
-        const GLOBAL BNode* n = nodes + node;
-        float4 a = n->a, b = n->b, c = n->c;
-        int4 ch = n->d;
         float t0, t1;
         bool h0, h1;
-        box2(a, b, c, id, noid, tmax, t0, t1, h0, h1);
+        box2(n0, n1, n2, id, noid, tmax, t0, t1, h0, h1);
         if (COUNT) tc->boxes += 2;
         if (h0 && h1) {
             int nearc = ch.x, farc = ch.y;
@@ -333,6 +330,42 @@ __device__ inline bool inner_step(const GLOBAL BNode* nodes, const GLOBAL QNode*
         if (h1) { node = ch.y; return true; }
         return false;
     }
+}
+
+/* The four 16-byte words of a node (both layouts are 64 B: three box words, then the codes). */
+template <int QUANT>
+__device__ inline void load_node(const GLOBAL BNode* nodes, const GLOBAL QNode* qnodes, int node, float4& n0,
+                                 float4& n1, float4& n2, int4& ch) {
+    const GLOBAL float4* p = QUANT ? (const GLOBAL float4*)(qnodes + node) : (const GLOBAL float4*)(nodes + node);
+    n0 = p[0];
+    n1 = p[1];
+    n2 = p[2];
+    ch = ((const GLOBAL int4*)p)[3];
+}
+
+template <int QUANT, int OVF, int COUNT>
+__device__ inline bool inner_step(const GLOBAL BNode* nodes, const GLOBAL QNode* qnodes, int& node, V3 id, V3 noid,
+                                  float tmax, Stack<OVF>& st, TravCount* tc) {
+    float4 n0, n1, n2;
+    int4 ch;
+    load_node<QUANT>(nodes, qnodes, node, n0, n1, n2, ch);
+    return inner_step_words<QUANT, OVF, COUNT>(n0, n1, n2, ch, node, id, noid, tmax, st, tc);
+}
+
+/* The same visit for a ray kept as (o, d): the node's loads are issued first and 1/d, -o/d are
+ * formed while they are in flight. The empty asm takes the child codes, so all four words are
+ * loaded together (the compiler would otherwise move the codes' load behind the box tests,
+ * where only hit nodes need them, and a visit would wait on memory twice). */
+template <int QUANT, int OVF, int COUNT>
+__device__ inline bool inner_step_od(const GLOBAL BNode* nodes, const GLOBAL QNode* qnodes, int& node, V3 o, V3 d,
+                                     float tmax, Stack<OVF>& st, TravCount* tc) {
+    float4 n0, n1, n2;
+    int4 ch;
+    load_node<QUANT>(nodes, qnodes, node, n0, n1, n2, ch);
+    const V3 id = rcp3(d);
+    const V3 noid = neg_o_over_d(o, id);
+    asm volatile("" ::"v"(ch.x), "v"(ch.y), "v"(ch.z), "v"(ch.w));
+    return inner_step_words<QUANT, OVF, COUNT>(n0, n1, n2, ch, node, id, noid, tmax, st, tc);
 }
 
 template <int TRI, int QUANT, int COUNT, int OVF>
@@ -670,9 +703,7 @@ __device__ inline bool pstep(PState& T, Best& B, Stack<OVF>& st, TravCount* tc) 
         pop = true;
     } else if (node >= 0) {
 #if PSTATE_LEAN
-        const V3 id = rcp3(T.d);
-        const V3 noid = neg_o_over_d(T.o, id);
-        pop = !inner_step<QUANT, OVF, COUNT>(T.nodes, T.qnodes, node, id, noid, B.t, st, tc);
+        pop = !inner_step_od<QUANT, OVF, COUNT>(T.nodes, T.qnodes, node, T.o, T.d, B.t, st, tc);
 #else
         pop = !inner_step<QUANT, OVF, COUNT>(T.nodes, T.qnodes, node, T.id, T.noid, B.t, st, tc);
 #endif
