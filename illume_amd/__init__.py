@@ -124,6 +124,7 @@ class Context:
         self.opts = o
 
     TUNABLE = ("bvh_layout", "extend_mode", "refill_min", "ext_fetch", "leaf_wait_pct", "shade_sort", "sched_fair",
+               "camera_packets",
                "sample_group", "tile_rows", "chunk_passes", "paths_in_flight")
 
     def tune(self, **kw):

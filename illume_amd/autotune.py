@@ -16,6 +16,7 @@ triangles differently. The tuner only measures time.
                      busy lanes) before a wave spends a trip on triangle tests
     shade_sort       1: each 512-path shading block runs its paths grouped by material stand-in
                      (staged through LDS), 2: in queue order
+    camera_packets   1: the first depth traces each wave's 64 camera rays as one packet, 2: per lane
     sample_group     consecutive path ids per pixel (ray coherence of the first bounces)
     tile_rows        pixel walk in bands of this many rows, column by column (wave footprint)
     paths_in_flight  paths per chunk (queue size): larger chunks amortise launch tails,
@@ -39,6 +40,7 @@ DEFAULT_SPACE = (
     (("extend_mode", "refill_min"), ((1, 0), (2, 16), (2, 32), (3, 8), (3, 16), (3, 48))),
     ("leaf_wait_pct", (50, 100, 150, 200, 400)),
     ("shade_sort", (1, 2)),
+    ("camera_packets", (1, 2)),
     ("ext_fetch", (128, 256, 512, 1024, 2048)),
     ("paths_in_flight", (1 << 24, 1 << 25, 1 << 26)),
 )

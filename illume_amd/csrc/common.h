@@ -220,6 +220,7 @@ struct KParams {
     int leaf_pct;           /* per-step refill: leaf trips when leaf lanes >= leaf_pct% of the others */
     int shade_sort;         /* shade each block's paths in shading-key order (staged in LDS) */
     int cam0;               /* the first depth forms camera rays in extend and shade (no raygen) */
+    int packet0;            /* ... and its extend traces each wave's camera rays as one packet */
 };
 
 /* Path id order of a slice: local index l <-> (sample k of the slice, pixel slot q). Samples

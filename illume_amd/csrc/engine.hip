@@ -433,7 +433,7 @@ static bool opts_valid(const ilm_opts& o) {
              o.extend_mode < 0 || o.extend_mode > ILM_EXTEND_STEP_REFILL || !sg_ok || o.tile_rows < 0 || o.tile_rows > 256 ||
              o.refill_min < 0 || o.refill_min > 64 || o.ext_fetch < 0 || o.ext_fetch > 65536 || (o.ext_fetch & 63) != 0 ||
              o.leaf_wait_pct < 0 || o.leaf_wait_pct > 10000 || o.shade_sort < 0 || o.shade_sort > 2 ||
-             o.sched_fair < 0 || o.sched_fair > 1 ||
+             o.sched_fair < 0 || o.sched_fair > 1 || o.camera_packets < 0 || o.camera_packets > 2 ||
              o.chunk_passes < 0 || o.paths_in_flight < 0 || o.hbm_budget_bytes < 0 || o.max_queued < 0);
 }
 
@@ -549,7 +549,7 @@ void ilm_close(ilm_ctx* ctx) {
 }
 
 /* Retune an idle context: bvh_layout, extend_mode, refill_min, ext_fetch, leaf_wait_pct, shade_sort,
- * sched_fair, sample_group, tile_rows, chunk_passes and paths_in_flight take the values in *o (the
+ * sched_fair, camera_packets, sample_group, tile_rows, chunk_passes and paths_in_flight take the values in *o (the
  * other fields are ignored). Every scene carries both node layouts, so a layout change needs no
  * upload. Path buffers are reallocated when the chunk size changes; captured graphs are dropped. */
 int ilm_tune(ilm_ctx* ctx, const ilm_opts* o) {
@@ -587,6 +587,7 @@ int ilm_tune(ilm_ctx* ctx, const ilm_opts* o) {
     ctx->opts.leaf_wait_pct = o->leaf_wait_pct;
     ctx->opts.shade_sort = o->shade_sort;
     ctx->opts.sched_fair = o->sched_fair;
+    ctx->opts.camera_packets = o->camera_packets;
     ctx->shade_grid = ctx->n_cu * ilm_k_shade_occupancy(ctx->opts.shade_sort != 2);
     ctx->opts.chunk_passes = o->chunk_passes;
     const int64_t cap = cap_of(*o);
@@ -640,6 +641,7 @@ static KParams base_params(ilm_ctx* ctx) {
     P.ext_fetch = ctx->opts.ext_fetch > 0 ? ctx->opts.ext_fetch : 256;
     P.leaf_pct = ctx->opts.leaf_wait_pct > 0 ? ctx->opts.leaf_wait_pct : 100;
     P.shade_sort = ctx->opts.shade_sort == 2 ? 0 : 1;
+    P.packet0 = ctx->opts.camera_packets == 2 ? 0 : 1;
     P.trav_stats = ctx->opts.count_tests ? ctx->d_trav : nullptr;
     return P;
 }

@@ -625,7 +625,9 @@ hipError_t ilm_k_path_clear(const KParams& P, int grid, hipStream_t s) {
 
 template <int T, int Q, int C, int O>
 static void launch_extend(const KParams& P, int depth, int* ovf, int grid, hipStream_t s) {
-    if (P.extend_mode == ILM_EXTEND_STEP_REFILL && P.cam0 && depth == 0)
+    if (P.extend_mode == ILM_EXTEND_STEP_REFILL && P.cam0 && P.packet0 && depth == 0)
+        ILM_LAUNCH((k_extend_packet<T, Q, C>), dim3(grid), dim3(EXT_BLOCK), 0, s, P, depth);
+    else if (P.extend_mode == ILM_EXTEND_STEP_REFILL && P.cam0 && depth == 0)
         ILM_LAUNCH((k_extend_pers<T, Q, C, O, 1>), dim3(grid), dim3(EXT_BLOCK), 0, s, P, depth, ovf);
     else if (P.extend_mode == ILM_EXTEND_STEP_REFILL)
         ILM_LAUNCH((k_extend_pers<T, Q, C, O, 0>), dim3(grid), dim3(EXT_BLOCK), 0, s, P, depth, ovf);

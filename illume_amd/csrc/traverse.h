@@ -431,6 +431,193 @@ __device__ inline void trace_ray(const GLOBAL DScene* S, V3 O, V3 D, float tmax0
     }
 }
 
+/* ---------------- camera rays as packets ----------------
+ * At the first depth a wave's 64 consecutive path ids are samples of one pixel (or of a few
+ * neighbouring pixels), so their camera rays are nearly the same. The wave walks one BVH
+ * traversal for all of them: the node, the stack (in LDS) and every branch are uniform, each
+ * lane tests the node's children against its own ray and its own closest hit so far, and a
+ * child is visited when any lane hits it, nearest first by the first hitting lane's entry
+ * distance. A lane that misses a subtree only repeats tests that cannot change its result, so
+ * every lane ends with the closest hit of its own ray (the order of visits can only change which
+ * of two triangles at exactly the same distance is kept). */
+#define PKT_STACK 128   /* = LDS_STACK + OVF_STACK: the largest stack bound a scene is accepted with */
+
+__device__ inline int uni_i(int v) { return __builtin_amdgcn_readfirstlane(v); }
+template <class T>
+__device__ inline const GLOBAL T* uni_p(const GLOBAL T* p) {
+    const uint64_t b = (uint64_t)p;
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)b);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(b >> 32));
+    return (const GLOBAL T*)(((uint64_t)hi << 32) | lo);
+}
+
+/* The packet walk over the lanes that call it (all of them on scene S). */
+template <int TRI, int QUANT, int COUNT>
+__device__ inline void packet_trace(const GLOBAL DScene* S, V3 O, V3 D, Best& B, LDSAS int* stk, TravCount* tc) {
+    spheres_first(S, O, D, B);
+    int node = uni_i(QUANT ? S->qtlas_root : S->tlas_root);
+    if (node == ILM_NO_CHILD) return;
+    const GLOBAL BNode* nodes = uni_p(gp(S->tlas));
+    const GLOBAL QNode* qnodes = uni_p(gp(S->qtlas));
+    const GLOBAL float4* tris = nullptr;
+    bool blas = false;
+    int cur = -1;
+    V3 o = O, d = D;
+    WTRay wt;
+    int sp = 0;
+    for (;;) {
+        bool pop = true;
+        if (node >= 0) {
+            float4 n0, n1, n2;
+            int4 ch;
+            load_node<QUANT>(nodes, qnodes, node, n0, n1, n2, ch);
+            const V3 id = rcp3(d);
+            const V3 noid = neg_o_over_d(o, id);
+            float tn[4];
+            uint32_t hm;
+            if (QUANT) {
+                int nc;
+                box4q(n0, n1, n2, id, noid, B.t, tn, hm, nc);
+                if (COUNT) tc->boxes += nc;
+            } else {
+                bool h0, h1;
+                box2(n0, n1, n2, id, noid, B.t, tn[0], tn[1], h0, h1);
+                tn[2] = tn[3] = 0.0f;
+                hm = (h0 ? 1u : 0u) | (h1 ? 2u : 0u);
+                if (COUNT) tc->boxes += 2;
+            }
+            const uint32_t any = (__ballot(hm & 1u) ? 1u : 0u) | (__ballot(hm & 2u) ? 2u : 0u) |
+                                 (__ballot(hm & 4u) ? 4u : 0u) | (__ballot(hm & 8u) ? 8u : 0u);
+            if (any) {
+                const int lead = (int)__builtin_ctzll(__ballot(hm != 0u));
+                const uint32_t lh = (uint32_t)__builtin_amdgcn_readlane((int)hm, lead);
+                float k[4];
+                int c[4] = {uni_i(ch.x), uni_i(ch.y), uni_i(ch.z), uni_i(ch.w)};
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const float lt = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(tn[j]), lead));
+                    k[j] = !((any >> j) & 1u) ? __builtin_inff() : (((lh >> j) & 1u) ? lt : ILM_FLT_MAX);
+                }
+#define ILM_CE(a, b)                                                                  \
+                {                                                                     \
+                    const bool sw = k[b] < k[a];                                      \
+                    const float tk = sw ? k[b] : k[a]; k[b] = sw ? k[a] : k[b]; k[a] = tk; \
+                    const int tc_ = sw ? c[b] : c[a]; c[b] = sw ? c[a] : c[b]; c[a] = tc_; \
+                }
+                ILM_CE(0, 1)
+                ILM_CE(2, 3)
+                ILM_CE(0, 2)
+                ILM_CE(1, 3)
+                ILM_CE(1, 2)
+#undef ILM_CE
+                const int n = __builtin_popcount(any);
+                for (int j = n - 1; j >= 1; j--) {
+                    if (sp < PKT_STACK) stk[sp] = uni_i(c[j]);
+                    sp++;
+                }
+                node = uni_i(c[0]);
+                pop = false;
+            }
+        } else if (!blas) {
+            const int k = uni_i(gp(S->tlas_prims)[ilm_leaf_first(node)]);
+            const GLOBAL DInst* I = gp(S->inst) + k;
+            const int mesh = uni_i(I->mesh);
+            const GLOBAL DMesh* M = gp(S->meshes) + mesh;
+            const int broot = uni_i(QUANT ? M->qroot : M->root);
+            if (broot != ILM_NO_CHILD) {
+                float inv[12];
+                load_rows12(I->inv, inv);
+                o = xform3(inv, O, 1.0f);
+                d = xform3(inv, D, 0.0f);
+                if (TRI) wt = wt_setup(d);
+                nodes = uni_p(gp(M->nodes));
+                qnodes = uni_p(gp(M->qnodes));
+                tris = uni_p(gp(M->tris));
+                if (sp < PKT_STACK) stk[sp] = STACK_SENTINEL;
+                sp++;
+                blas = true;
+                cur = k;
+                node = broot;
+                pop = false;
+            }
+        } else {
+            leaf_tris<TRI, COUNT>(tris, node, o, d, wt, cur, B, tc);
+        }
+        if (pop) {
+            for (;;) {
+                if (sp == 0) return;
+                sp--;
+                if (sp >= PKT_STACK) continue;   /* a push past the capacity was dropped (not within the bound) */
+                node = uni_i(stk[sp]);
+                if (node != STACK_SENTINEL) break;
+                blas = false;
+                nodes = uni_p(gp(S->tlas));
+                qnodes = uni_p(gp(S->qtlas));
+                o = O;
+                d = D;
+            }
+        }
+    }
+}
+
+/* First depth with per-step refill: each wave reserves consecutive path ids (P.ext_fetch per
+ * queue atomic, shrinking to 64 as the queue drains, as in k_extend), and traces them 64 at a
+ * time: forms their camera rays (camera_ray) and walks one packet per distinct scene among them. */
+template <int TRI, int QUANT, int COUNT>
+__global__ __launch_bounds__(EXT_BLOCK) void k_extend_packet(KParams P, int depth) {
+    __shared__ int stk[PKT_STACK];
+    const int lane = threadIdx.x;
+    GLOBAL DCtl* ctl = gpw(P.ctl);
+    const uint32_t count = ctl->count[depth];
+    const uint32_t fetch = (uint32_t)P.ext_fetch;
+    const GLOBAL DReq* reqs = gp(P.chunk)->reqs;
+    TravCount tc = {0u, 0u};
+    const uint32_t guide = gridDim.x * 2u;
+    uint32_t want = fetch;
+    uint32_t b = 0, end = 0;
+    for (;;) {
+        if (b >= end) {
+            uint32_t base = 0;
+            if (lane == 0) base = atomicAdd(&ctl->fetch[depth], want);
+            base = (uint32_t)uni_i((int)__shfl(base, 0));
+            if (base >= count) break;
+            b = base;
+            end = base + want < count ? base + want : count;
+            const uint32_t left = count - end;
+            const uint32_t g = (left / guide) & ~(uint32_t)(EXT_BLOCK - 1);
+            want = g < (uint32_t)EXT_BLOCK ? (uint32_t)EXT_BLOCK : (g > fetch ? fetch : g);
+        }
+        const uint32_t r = b + (uint32_t)lane;
+        const bool valid = r < end;
+        b += (uint32_t)EXT_BLOCK;
+        float4 o4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), d4 = o4;
+        uint2 rg;
+        uint64_t sbits = 0;
+        Best B;
+        best_init(B);
+        if (valid) {
+            camera_ray(P, r, o4, d4, rg);
+            sbits = (uint64_t)gp(reqs[f2u(o4.w) >> 16].scene);
+        }
+        unsigned long long rem = __ballot(valid);
+        while (rem) {
+            const int lead = (int)__builtin_ctzll(rem);
+            const uint64_t sl = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(sbits >> 32), lead) << 32) |
+                                (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)sbits, lead);
+            const bool mine = valid && sbits == sl;
+            rem &= ~__ballot(mine);
+            if (mine)
+                packet_trace<TRI, QUANT, COUNT>((const GLOBAL DScene*)sl, mk3(o4.x, o4.y, o4.z), mk3(d4.x, d4.y, d4.z),
+                                                B, (LDSAS int*)stk, &tc);
+        }
+        if (valid) write_hit(P, r, B);
+    }
+    if (COUNT) {
+        atomicAdd(&P.trav_stats[0], (unsigned long long)tc.boxes);
+        atomicAdd(&P.trav_stats[1], (unsigned long long)tc.tris);
+    }
+}
+
 /* One ray batch per wave at a time: a wave reserves P.ext_fetch consecutive queue entries with
  * one atomic (same-address atomics serialise, so one per 64-ray batch throttles the queue) and
  * walks them as consecutive 64-ray batches, each to completion. */

@@ -220,6 +220,10 @@ typedef struct ilm_opts {
      * it all, the rest split the chunk), so short requests finish without waiting behind long
      * ones. Results are identical either way. */
     int32_t sched_fair;
+    /* first depth with per-step refill: trace each wave's 64 camera rays as one packet (one
+     * traversal for the wave, every lane testing its own ray; 0 = default: on, 1 = on, 2 = off,
+     * each lane walking alone). Results are identical except at exact distance ties. */
+    int32_t camera_packets;
 } ilm_opts;
 
 typedef struct ilm_render_info {
@@ -278,8 +282,8 @@ void ilm_close(ilm_ctx* ctx);
 const char* ilm_last_error(ilm_ctx* ctx);
 int ilm_stats_get(ilm_ctx* ctx, ilm_stats* out);
 /* Retune an idle context (no counterpart in the reference, whose launch shape is fixed):
- * extend_mode, refill_min, ext_fetch, leaf_wait_pct, shade_sort, sched_fair, sample_group, tile_rows,
- * chunk_passes and paths_in_flight
+ * bvh_layout, extend_mode, refill_min, ext_fetch, leaf_wait_pct, shade_sort, sched_fair,
+ * camera_packets, sample_group, tile_rows, chunk_passes and paths_in_flight
  * take the values in *opts; other fields are ignored. ILM_E_BUSY while served requests are queued or running.
  * The autotuner (illume_amd/autotune.py) drives this between short trial renders. */
 int ilm_tune(ilm_ctx* ctx, const ilm_opts* opts);
