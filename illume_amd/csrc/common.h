@@ -136,6 +136,21 @@ struct DScene {
 };
 
 /* One request inside a launch. */
+/* Exact unsigned division by a divisor fixed on the host: with M = ceil(2^64 / d),
+ * floor(n / d) = floor(n * M / 2^64) for every 32-bit n, because the excess n * (M - 2^64/d) / 2^64
+ * is below 2^-32 and so below 1/d, the smallest gap between n/d and the next integer. d = 1 is
+ * stored as M = 0. Replaces a ~30-instruction integer division by three. */
+struct UDiv {
+    uint32_t lo, hi;
+};
+inline UDiv udiv_make(uint32_t d) {
+    UDiv m;
+    const uint64_t M = d > 1 ? ~0ull / d + 1 : 0ull;
+    m.lo = (uint32_t)M;
+    m.hi = (uint32_t)(M >> 32);
+    return m;
+}
+
 struct DReq {
     const DScene* scene;
     int width, height;
@@ -147,6 +162,9 @@ struct DReq {
     int spp;
     float4* accum;          /* width*height Kahan sums */
     float4* comp;           /* width*height Kahan compensations */
+    /* pixel walk of this chunk (tile_pixel with tile_th rows) and its divisors */
+    uint32_t tile_th, tile_full;
+    UDiv w_div, band_div, th_div;
 };
 
 /* One slice of work in a chunk: passes [pass0, pass0+npass) of pixel slots [px0, px0+npx) of
@@ -161,6 +179,8 @@ struct DWork {
     uint32_t pid_base;      /* first path id of this slice */
     uint32_t acc_base;      /* first accumulate index of this slice (prefix of npx) */
     uint32_t sgroup;        /* consecutive path ids per pixel (power of two dividing npass) */
+    uint32_t sg_log2;       /* log2(sgroup) */
+    UDiv span_div;          /* divides by npx * sgroup */
 };
 
 #define ILM_MAX_JOBS 256

@@ -793,6 +793,16 @@ static void fair_quotas(const std::vector<Job*>& jobs, int64_t cap, std::vector<
     }
 }
 
+/* The divisors a request's pixel walk needs on the device (tile_pixel with th rows). */
+static void req_divs(DReq& R, int th) {
+    const uint32_t w = (uint32_t)R.width, h = (uint32_t)R.height, t = th > 1 ? (uint32_t)th : 1u;
+    R.tile_th = t;
+    R.tile_full = (h / t) * (t * w);
+    R.w_div = udiv_make(w);
+    R.band_div = udiv_make(t * w);
+    R.th_div = udiv_make(t);
+}
+
 static void build_chunk(ilm_ctx* ctx, const std::vector<Job*>& jobs, DChunk& c, std::vector<Job*>& in_chunk) {
     c.total_paths = 0;
     c.total_px = 0;
@@ -833,12 +843,15 @@ static void build_chunk(ilm_ctx* ctx, const std::vector<Job*>& jobs, DChunk& c, 
             const uint32_t sg_max = ctx->opts.sample_group > 0 ? (uint32_t)ctx->opts.sample_group : 64u;
             while (w.sgroup * 2 <= sg_max && w.npass % (w.sgroup * 2) == 0) w.sgroup *= 2;
         }
+        w.sg_log2 = (uint32_t)__builtin_ctz(w.sgroup);
+        w.span_div = udiv_make(w.npx * w.sgroup);
         w.pid_base = c.total_paths;
         w.acc_base = c.total_px;
         c.total_paths += (uint32_t)((int64_t)w.npass * w.npx);
         c.total_px += w.npx;
         left -= (int64_t)w.npass * w.npx;
         c.reqs[nreq] = j->dreq;
+        req_divs(c.reqs[nreq], ctx->opts.tile_rows > 0 ? ctx->opts.tile_rows : 4);
         c.work[c.n_work++] = w;
         c.depth_steps = std::max(c.depth_steps, j->p.max_depth);
         nreq++;
@@ -1085,6 +1098,7 @@ int ilm_debug_path_radiance(ilm_ctx* ctx, ilm_scene* scene, const ilm_render_par
     R.tonemap = params->tonemap;
     R.spp = params->spp;
     hm::render_info(scene->camera, params->width, params->height, R.image_width, R.pixel_size, R.left, R.top);
+    req_divs(R, 1);
     std::vector<float4> c4;
     std::vector<int> sc;
     for (int64_t base = 0; base < n; base += ctx->cap) {

@@ -537,7 +537,7 @@ __global__ __launch_bounds__(256) void k_accumulate(KParams P) {
         const GLOBAL DWork* W = C->work + lo;
         const GLOBAL DReq* R = C->reqs + W->req;
         const uint32_t q = g - W->acc_base;
-        const uint32_t p = tile_pixel(W->px0 + q, (uint32_t)R->width, (uint32_t)R->height, P.tile_rows);
+        const uint32_t p = tile_pixel_r(W->px0 + q, R);
         GLOBAL float4* acc = gpw(R->accum);
         GLOBAL float4* cmp = gpw(R->comp);
         float4 s = acc[p], c = cmp[p];

@@ -40,6 +40,28 @@
 __device__ inline uint32_t f2u(float f) { return __float_as_uint(f); }
 __device__ inline float u2f(uint32_t u) { return __uint_as_float(u); }
 
+__device__ inline uint32_t udiv(uint32_t n, UDiv m) {
+    const uint32_t t = __umulhi(n, m.lo);
+    const uint32_t q = (uint32_t)(((uint64_t)n * m.hi + t) >> 32);
+    return (m.lo | m.hi) ? q : n;
+}
+/* slice_decode for a work item, with its precomputed divisor (sgroup is a power of two) */
+__device__ inline void slice_decode_w(uint32_t l, const GLOBAL DWork* W, uint32_t& k, uint32_t& q) {
+    const uint32_t sh = W->sg_log2, span = W->npx << sh;
+    const uint32_t kb = udiv(l, W->span_div), r = l - kb * span;
+    q = r >> sh;
+    k = (kb << sh) + (r - (q << sh));
+}
+/* tile_pixel for a request of the chunk, with its precomputed divisors */
+__device__ inline uint32_t tile_pixel_r(uint32_t i, const GLOBAL DReq* R) {
+    const uint32_t th = R->tile_th;
+    if (th <= 1 || i >= R->tile_full) return i;
+    const uint32_t w = (uint32_t)R->width, band_px = th * w;
+    const uint32_t band = udiv(i, R->band_div), r = i - band * band_px;
+    const uint32_t x = udiv(r, R->th_div), y = band * th + (r - x * th);
+    return y * w + x;
+}
+
 /* The work item of the chunk that holds path id i (work items sorted by pid_base). */
 __device__ inline int find_work_pid(const GLOBAL DWork* work, int n, uint32_t i) {
     int lo = 0, hi = n - 1;
@@ -195,19 +217,19 @@ __device__ inline void camera_ray(const KParams& P, uint32_t i, float4& o4, floa
     } else {
         const GLOBAL DWork* W = C->work + find_work_pid(C->work, C->n_work, i);
         uint32_t k, q;
-        slice_decode(i - W->pid_base, W->npx, W->sgroup, k, q);
+        slice_decode_w(i - W->pid_base, W, k, q);
         req = W->req;
         pass = (uint32_t)W->pass0 + k;
         pixel = W->px0 + q;
     }
     const GLOBAL DReq* R = C->reqs + req;
-    if (!P.list) pixel = tile_pixel(pixel, (uint32_t)R->width, (uint32_t)R->height, P.tile_rows);
+    if (!P.list) pixel = tile_pixel_r(pixel, R);
     const GLOBAL DScene* S = gp(R->scene);
     uint64_t rs = ilm_path_seed(pass, pixel);
     const uint64_t inc = ilm_pcg32_inc(pixel);
     const float pixel_size = R->pixel_size;
-    int Wd = (int)R->image_width;
-    int x = (int)pixel % Wd, y = (int)pixel / Wd;
+    const uint32_t yq = udiv(pixel, R->w_div);
+    int x = (int)(pixel - yq * (uint32_t)R->width), y = (int)yq;
     float left_edge = R->left + pixel_size * (float)x;
     float top_edge = R->top - pixel_size * (float)y;
     float rx = left_edge + pixel_size * ilm_pcg32_uniform(&rs, inc);
