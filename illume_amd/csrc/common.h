@@ -241,6 +241,7 @@ struct KParams {
     int shade_sort;         /* shade each block's paths in shading-key order (staged in LDS) */
     int cam0;               /* the first depth forms camera rays in extend and shade (no raygen) */
     int packet0;            /* ... and its extend traces each wave's camera rays as one packet */
+    int uni_scene;          /* every request of the chunk renders the same scene */
 };
 
 /* Path id order of a slice: local index l <-> (sample k of the slice, pixel slot q). Samples

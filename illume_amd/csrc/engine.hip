@@ -697,10 +697,16 @@ static int launch_chunk(ilm_ctx* ctx, const DChunk& hc, bool ovf, ProfileSink* p
     if (rc) return rc;
     KParams P = base_params(ctx);
     P.use_ovf = ovf ? 1 : 0;
+    /* one scene for the whole chunk: the extend kernel reads it once from the chunk descriptor
+     * (so a graph captured for it serves any later single-scene chunk) */
+    bool uni = hc.n_work > 0;
+    for (int k = 1; k < hc.n_work && uni; k++) uni = hc.reqs[hc.work[k].req].scene == hc.reqs[hc.work[0].req].scene;
+    uni = uni && hc.work[0].req == 0;
+    P.uni_scene = uni ? 1 : 0;
     const int D = hc.depth_steps;
     const bool graphs = ctx->opts.use_graphs && !prof && !ctx->opts.count_tests;
     if (graphs) {
-        const int key = 2 * D + (ovf ? 1 : 0);
+        const int key = 4 * D + (uni ? 2 : 0) + (ovf ? 1 : 0);
         auto it = ctx->graphs.find(key);
         if (it == ctx->graphs.end()) {
             hipGraph_t g = nullptr;

@@ -39,6 +39,15 @@
 
 __device__ inline uint32_t f2u(float f) { return __float_as_uint(f); }
 __device__ inline float u2f(uint32_t u) { return __uint_as_float(u); }
+/* values the caller knows to be equal in every lane, made uniform (SGPRs) for the compiler */
+__device__ inline int uni_i(int v) { return __builtin_amdgcn_readfirstlane(v); }
+template <class T>
+__device__ inline const GLOBAL T* uni_p(const GLOBAL T* p) {
+    const uint64_t b = (uint64_t)p;
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)b);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(b >> 32));
+    return (const GLOBAL T*)(((uint64_t)hi << 32) | lo);
+}
 
 __device__ inline uint32_t udiv(uint32_t n, UDiv m) {
     const uint32_t t = __umulhi(n, m.lo);
@@ -464,14 +473,6 @@ __device__ inline void trace_ray(const GLOBAL DScene* S, V3 O, V3 D, float tmax0
  * of two triangles at exactly the same distance is kept). */
 #define PKT_STACK 128   /* = LDS_STACK + OVF_STACK: the largest stack bound a scene is accepted with */
 
-__device__ inline int uni_i(int v) { return __builtin_amdgcn_readfirstlane(v); }
-template <class T>
-__device__ inline const GLOBAL T* uni_p(const GLOBAL T* p) {
-    const uint64_t b = (uint64_t)p;
-    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)b);
-    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(b >> 32));
-    return (const GLOBAL T*)(((uint64_t)hi << 32) | lo);
-}
 
 /* The packet walk over the lanes that call it (all of them on scene S). */
 template <int TRI, int QUANT, int COUNT>
@@ -903,9 +904,11 @@ __device__ inline void pstep_init(PState& T, Best& B, Stack<OVF>& st, const GLOB
     st.sp = 0;
 }
 
-/* One action of the walk; true when the ray is finished. */
-template <int TRI, int QUANT, int COUNT, int OVF>
-__device__ inline bool pstep(PState& T, Best& B, Stack<OVF>& st, TravCount* tc) {
+/* One action of the walk; true when the ray is finished. UNI: every lane's ray belongs to scene
+ * Su (a value uniform over the wave), which then replaces the per-lane T.S. */
+template <int TRI, int QUANT, int COUNT, int OVF, int UNI>
+__device__ inline bool pstep(PState& T, Best& B, Stack<OVF>& st, TravCount* tc, const GLOBAL DScene* Su) {
+    const GLOBAL DScene* S = UNI ? Su : T.S;
     int node = T.node;
     bool pop = true;
     if (node == POP_MARK) {
@@ -917,9 +920,9 @@ __device__ inline bool pstep(PState& T, Best& B, Stack<OVF>& st, TravCount* tc) 
         pop = !inner_step<QUANT, OVF, COUNT>(T.nodes, T.qnodes, node, T.id, T.noid, B.t, st, tc);
 #endif
     } else if (!T.blas) {
-        const int k = gp(T.S->tlas_prims)[ilm_leaf_first(node)];
-        const GLOBAL DInst* I = gp(T.S->inst) + k;
-        const GLOBAL DMesh* M = gp(T.S->meshes) + I->mesh;
+        const int k = gp(S->tlas_prims)[ilm_leaf_first(node)];
+        const GLOBAL DInst* I = gp(S->inst) + k;
+        const GLOBAL DMesh* M = gp(S->meshes) + I->mesh;
         const int broot = QUANT ? M->qroot : M->root;
         if (broot != ILM_NO_CHILD) {
             float inv[12];
@@ -955,15 +958,17 @@ __device__ inline bool pstep(PState& T, Best& B, Stack<OVF>& st, TravCount* tc) 
             T.noid = neg_o_over_d(T.O, T.id);
 #endif
             T.blas = 0;
-            T.nodes = gp(T.S->tlas);
-            T.qnodes = gp(T.S->qtlas);
+            T.nodes = gp(S->tlas);
+            T.qnodes = gp(S->qtlas);
         }
     }
     T.node = node;
     return false;
 }
 
-template <int TRI, int QUANT, int COUNT, int OVF, int CAM>
+/* UNI: every request of the chunk renders the same scene (the host checks), so the scene
+ * pointer is read once, uniform over the wave, instead of per lane at every refill. */
+template <int TRI, int QUANT, int COUNT, int OVF, int CAM, int UNI>
 __global__ EXT_BOUNDS void k_extend_pers(KParams P, int depth, int* ovf_base) {
     __shared__ int stk[LDS_STACK * EXT_BLOCK];
     const int lane = threadIdx.x;
@@ -978,6 +983,7 @@ __global__ EXT_BOUNDS void k_extend_pers(KParams P, int depth, int* ovf_base) {
     const GLOBAL float4* ind = gp(P.in.d);
     const GLOBAL uint2* inr = gp(P.in.rng);
     const GLOBAL DReq* reqs = gp(P.chunk)->reqs;
+    const GLOBAL DScene* Su = UNI ? uni_p(gp(reqs[0].scene)) : nullptr;
     const unsigned long long lt_mask = (1ull << lane) - 1ull;
     const uint32_t refill_min = (uint32_t)P.refill_min;
     const uint32_t leaf_pct = (uint32_t)P.leaf_pct;
@@ -1012,11 +1018,11 @@ __global__ EXT_BOUNDS void k_extend_pers(KParams P, int depth, int* ovf_base) {
                         float4 o4, d4;
                         uint2 rg;
                         camera_ray(P, r, o4, d4, rg);
-                        const GLOBAL DScene* S = gp(reqs[f2u(o4.w) >> 16].scene);
+                        const GLOBAL DScene* S = UNI ? Su : gp(reqs[f2u(o4.w) >> 16].scene);
                         pstep_init<QUANT, OVF>(T, B, st, S, mk3(o4.x, o4.y, o4.z), mk3(d4.x, d4.y, d4.z), ILM_FLT_MAX);
                     } else {
                         float4 o4 = ino[r], d4 = ind[r];
-                        const GLOBAL DScene* S = gp(reqs[f2u(o4.w) >> 16].scene);
+                        const GLOBAL DScene* S = UNI ? Su : gp(reqs[f2u(o4.w) >> 16].scene);
                         pstep_init<QUANT, OVF>(T, B, st, S, mk3(o4.x, o4.y, o4.z), mk3(d4.x, d4.y, d4.z),
                                                medium_tmax(S, f2u(o4.w), inr[r], f2u(d4.w)));
                     }
@@ -1036,7 +1042,7 @@ __global__ EXT_BOUNDS void k_extend_pers(KParams P, int depth, int* ovf_base) {
         const uint32_t n_rest = (uint32_t)__popcll(__ballot(has)) - n_leaf;
         const bool leaf_trip = n_leaf * 100u >= n_rest * leaf_pct;
         if (has && at_leaf == leaf_trip) {
-            if (pstep<TRI, QUANT, COUNT, OVF>(T, B, st, &tc)) {
+            if (pstep<TRI, QUANT, COUNT, OVF, UNI>(T, B, st, &tc, Su)) {
                 write_hit(P, ridx, B);
                 ridx = NO_RAY;
             }

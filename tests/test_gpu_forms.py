@@ -54,6 +54,7 @@ def test_graph_replays_match_direct_launches(scenes):
                                    dict(extend_mode=3, refill_min=48, sample_group=16),
                                    dict(extend_mode=3, leaf_wait_pct=10), dict(extend_mode=3, leaf_wait_pct=5000),
                                    dict(shade_sort=2), dict(shade_sort=1, paths_in_flight=5000),
+                                   dict(camera_packets=2), dict(camera_packets=1, sample_group=1, tile_rows=1),
                                    dict(sched_fair=1, paths_in_flight=3000)])
 def test_path_order_leaves_results_unchanged(ctx, scenes, order):
     """Samples grouped per pixel and tiled pixel walks only reorder path ids: every pixel still
