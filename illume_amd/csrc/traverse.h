@@ -292,7 +292,11 @@ __device__ inline float medium_tmax(const GLOBAL DScene* S, uint32_t tag, uint2 
     return __uint_as_float(__float_as_uint(s0) + 1u);
 }
 
-/* Intersect the triangles of BLAS leaf `node`. */
+/* Intersect the triangles of BLAS leaf `node`. A triangle at exactly the distance of the best
+ * triangle hit so far replaces it when its (instance, original index) is smaller, so the closest
+ * hit of a ray does not depend on the order its walk visits triangles in (the walk order differs
+ * between node layouts, and a packet's between mixes of requests); a sphere keeps a tie, as it
+ * would under the strict < of src/renderer.cu:117-136 with spheres tested first. */
 template <int TRI, int COUNT>
 __device__ inline void leaf_tris(const GLOBAL float4* tris, int node, V3 o, V3 d, const WTRay& wt, int cur, Best& B,
                                  TravCount* tc) {
@@ -304,7 +308,9 @@ __device__ inline void leaf_tris(const GLOBAL float4* tris, int node, V3 o, V3 d
         float u = 0.0f, v = 0.0f, t;
         if (TRI) t = tri_wt(wt, mk3(A.x, A.y, A.z), mk3(Bq.x, Bq.y, Bq.z), mk3(C.x, C.y, C.z), o, u, v);
         else t = tri_mt(mk3(A.x, A.y, A.z), mk3(Bq.x, Bq.y, Bq.z), mk3(C.x, C.y, C.z), o, d, u, v);
-        if (t > 0.0f && t < B.t) best_tri(B, t, u, v, cur, A);
+        const int prim = __float_as_int(A.w);
+        const bool tie = t == B.t && B.inst >= 0 && (cur < B.inst || (cur == B.inst && prim < B.prim));
+        if (t > 0.0f && (t < B.t || tie)) best_tri(B, t, u, v, cur, A);
     }
 }
 

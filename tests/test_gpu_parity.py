@@ -194,6 +194,29 @@ def test_subsurface_differing_paths_diverge_at_ties(ctx, subsurface):
         assert gh[0] == oh[0] and gh[2] == oh[2] and gh[1] != oh[1]
 
 
+def test_subsurface_ties_resolved_independently_of_the_walk(ctx, subsurface):
+    """Exact distance ties are resolved by (instance, primitive), not by the order a walk meets
+    the triangles, so the ten tie-bearing pixels of the config-2 image give bit-identical paths
+    under both node layouts and with camera rays walked per lane or as wave packets (whose
+    visiting order depends on which rays share the wave)."""
+    import illume_amd
+    spec, st, sc, _ = subsurface
+    p = _params(st)
+    W = st["width"]
+    pix = [(465, 53), (510, 71), (606, 82), (625, 115), (752, 148), (226, 182), (497, 202), (276, 209),
+           (215, 219), (247, 231)]
+    pairs = np.array([[y * W + x, s] for x, y in pix for s in range(st["spp"])], np.uint32)
+    ref, rseg = ctx.path_radiance(sc, p, pairs)
+    for opts in [dict(camera_packets=2), dict(bvh_layout="quant8"), dict(bvh_layout="quant8", camera_packets=2),
+                 dict(extend_mode=1)]:
+        with illume_amd.Context(0, **opts) as c:
+            s = c.scene(spec)
+            got, seg = c.path_radiance(s, p, pairs)
+            s.free()
+        assert np.array_equal(got, ref), opts
+        assert np.array_equal(seg, rseg), opts
+
+
 def test_subsurface_render_independent_of_launch_shape(subsurface):
     """Same image and segment count whatever the extend scheduling and path order."""
     import illume_amd
