@@ -1,12 +1,12 @@
 """Autotuner for the launch shape of the wavefront pipeline.
 
-The tunables change how paths are laid out and queued, never what is computed: every choice
-gives bit-identical images (samples are summed per pixel in pass order whatever the path order
-or chunking), except bvh_layout, whose two trees can resolve a distance tie between two
-triangles differently. The tuner only measures time.
+The tunables change how paths are laid out, queued and walked, never what is computed: every
+choice gives bit-identical images (samples are summed per pixel in pass order whatever the path
+order or chunking, and a ray's closest hit does not depend on the order its walk meets
+triangles: exact distance ties go to the smaller instance and primitive). The tuner only
+measures time.
 
-    bvh_layout       0: FP32 BVH2 nodes, 1: 8-bit BVH4 nodes (both are resident for every scene;
-                     closest hits agree except at distance ties)
+    bvh_layout       0: FP32 BVH2 nodes, 1: 8-bit BVH4 nodes (both are resident for every scene)
     extend_mode      1: each wave traces one batch of 64 rays to completion
                      2: idle lanes refill from a wave-local queue at BLAS leaves
                      3 (and 0, the default): idle lanes refill after every traversal step

@@ -222,7 +222,7 @@ typedef struct ilm_opts {
     int32_t sched_fair;
     /* first depth with per-step refill: trace each wave's 64 camera rays as one packet (one
      * traversal for the wave, every lane testing its own ray; 0 = default: on, 1 = on, 2 = off,
-     * each lane walking alone). Results are identical except at exact distance ties. */
+     * each lane walking alone). Results are identical either way. */
     int32_t camera_packets;
 } ilm_opts;
 
