@@ -162,3 +162,36 @@ def test_bvh_cache_upload_hits(ctx, tmp_path):
         assert np.array_equal(a, b)
         s1.free()
         s2.free()
+
+
+def test_scene_extent_beyond_8bit_quantum_refused():
+    """8-bit child boxes have a quantum of at most 2^24 units, so that the node-level product
+    quantum x 1/d stays finite for any direction: a mesh spread over 2e10 units (root quantum
+    2^27) is refused in that layout at upload and after a live switch to it, and renders in the
+    FP32 layout before and after."""
+    xs = np.linspace(-1e10, 1e10, 16)
+    pos, idx = [], []
+    for i, x in enumerate(xs):
+        pos += [[x, -1.0, 5.0], [x + 1e6, -1.0, 5.0], [x, 1.0, 5.0]]
+        idx.append([3 * i, 3 * i + 1, 3 * i + 2])
+    mesh = dict(positions=np.array(pos, np.float32), indices=np.array(idx, np.int32), face_normals=True)
+    spec = dict(meshes=[mesh], instances=[dict(mesh=0, material=0, transform=dict(translation=[0, 0, 0]))],
+                spheres=[], materials=[dict(type="lambert", albedo=[0.5, 0.5, 0.5])], mediums=[],
+                environment=[1.0, 1.0, 1.0],
+                camera=dict(fov=40.0, dof=1.0, aperture=0.0, transform=dict(translation=[0, 0, -5])))
+    p = render_params(width=16, height=16, spp=1, max_depth=2, ray_bias=1e-4, tonemap="linear")
+    with illume_amd.Context(0, bvh_layout="quant8") as c:
+        with pytest.raises(illume_amd.IlmError) as e:
+            c.scene(spec)
+        assert e.value.code == -9
+    with illume_amd.Context(0) as c:
+        s = c.scene(spec)
+        a, _, _ = c.render(s, p)
+        c.tune(bvh_layout="quant8")
+        with pytest.raises(illume_amd.IlmError) as e:
+            c.render(s, p)
+        assert e.value.code == -9
+        c.tune(bvh_layout="fp32")
+        b, _, _ = c.render(s, p)
+        s.free()
+    assert np.array_equal(a, b)
