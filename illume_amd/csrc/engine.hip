@@ -657,11 +657,13 @@ static int stage_chunk(ilm_ctx* ctx, const DChunk& hc) {
 }
 
 /* A chunk's first launch after k_ctl_init. With per-step refill the first depth's extend and
- * shade form the camera rays where they use them, so only the per-path results are cleared;
- * the other extend modes read the rays k_raygen writes out. */
+ * shade form the camera rays where they use them, so only the per-path results are cleared
+ * (by the packet extend itself when the first depth runs as packets); the other extend modes
+ * read the rays k_raygen writes out. */
 static hipError_t start_paths(ilm_ctx* ctx, KParams& P, hipStream_t s) {
     P.in = ctx->A;
     P.cam0 = P.extend_mode == ILM_EXTEND_STEP_REFILL ? 1 : 0;
+    if (P.cam0 && P.packet0) return hipSuccess;
     return P.cam0 ? ilm_k_path_clear(P, ctx->gen_grid, s) : ilm_k_raygen(P, ctx->gen_grid, s);
 }
 

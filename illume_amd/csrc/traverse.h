@@ -591,7 +591,9 @@ __device__ inline void packet_trace(const GLOBAL DScene* S, V3 O, V3 D, Best& B,
 
 /* First depth with per-step refill: each wave reserves consecutive path ids (P.ext_fetch per
  * queue atomic, shrinking to 64 as the queue drains, as in k_extend), and traces them 64 at a
- * time: forms their camera rays (camera_ray) and walks one packet per distinct scene among them. */
+ * time: forms their camera rays (camera_ray) and walks one packet per distinct scene among them.
+ * Every path id of the chunk passes through here once, so the kernel also clears the per-path
+ * results (the chunk needs no separate clear). */
 template <int TRI, int QUANT, int COUNT>
 __global__ __launch_bounds__(EXT_BLOCK) void k_extend_packet(KParams P, int depth) {
     __shared__ int stk[PKT_STACK];
@@ -619,6 +621,10 @@ __global__ __launch_bounds__(EXT_BLOCK) void k_extend_packet(KParams P, int dept
         const uint32_t r = b + (uint32_t)lane;
         const bool valid = r < end;
         b += (uint32_t)EXT_BLOCK;
+        if (valid) {
+            gpw(P.contrib)[r] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (P.seg_count) gpw(P.seg_count)[r] = 0;
+        }
         float4 o4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), d4 = o4;
         uint2 rg;
         uint64_t sbits = 0;
