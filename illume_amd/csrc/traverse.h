@@ -181,21 +181,23 @@ __device__ inline void box4q(const float4 w0, const float4 w1, const float4 w2, 
 template <int OVF>
 struct Stack {
     LDSAS int* lds;     /* &stk[lane]; entry s at lds[s * EXT_BLOCK] */
-    GLOBAL int* ovf;    /* this lane's spill area (OVF = 1) */
+    GLOBAL int* ovf;    /* the workgroup's spill area (OVF = 1), OVF_STACK entries per lane; uniform,
+                           so the lane's own slot is formed where it is used (the spill path is rare) */
     int sp;
+    __device__ inline GLOBAL int* spill(int s) const { return ovf + __lane_id() * OVF_STACK + (s - LDS_STACK); }
     __device__ inline void push(int v) {
         if (!OVF) {
             if (sp < LDS_STACK) { lds[sp * EXT_BLOCK] = v; sp++; }
         } else {
             if (sp < LDS_STACK) lds[sp * EXT_BLOCK] = v;
-            else if (sp < LDS_STACK + OVF_STACK) ovf[sp - LDS_STACK] = v;
+            else if (sp < LDS_STACK + OVF_STACK) *spill(sp) = v;
             sp = sp < LDS_STACK + OVF_STACK ? sp + 1 : sp;
         }
     }
     __device__ inline int pop() {
         --sp;
         if (!OVF || sp < LDS_STACK) return lds[sp * EXT_BLOCK];
-        return ovf[sp - LDS_STACK];
+        return *spill(sp);
     }
 };
 
@@ -662,7 +664,7 @@ __global__ EXT_BOUNDS void k_extend(KParams P, int depth, int* ovf_base) {
     const int lane = threadIdx.x;
     Stack<OVF> st;
     st.lds = (LDSAS int*)stk + lane;
-    st.ovf = gpw(ovf_base) + ((size_t)blockIdx.x * EXT_BLOCK + lane) * OVF_STACK;
+    st.ovf = gpw(ovf_base) + (size_t)blockIdx.x * EXT_BLOCK * OVF_STACK;
     st.sp = 0;
     GLOBAL DCtl* ctl = gpw(P.ctl);
     const uint32_t count = ctl->count[depth];
@@ -806,7 +808,7 @@ __global__ EXT_BOUNDS void k_extend_refill(KParams P, int depth, int* ovf_base) 
     const int lane = threadIdx.x;
     Stack<OVF> st;
     st.lds = (LDSAS int*)stk + lane;
-    st.ovf = gpw(ovf_base) + ((size_t)blockIdx.x * EXT_BLOCK + lane) * OVF_STACK;
+    st.ovf = gpw(ovf_base) + (size_t)blockIdx.x * EXT_BLOCK * OVF_STACK;
     st.sp = 0;
     GLOBAL DCtl* ctl = gpw(P.ctl);
     const uint32_t count = ctl->count[depth];
@@ -815,7 +817,6 @@ __global__ EXT_BOUNDS void k_extend_refill(KParams P, int depth, int* ovf_base) 
     const GLOBAL float4* ind = gp(P.in.d);
     const GLOBAL uint2* inr = gp(P.in.rng);
     const GLOBAL DReq* reqs = gp(P.chunk)->reqs;
-    const unsigned long long lt_mask = (1ull << lane) - 1ull;
     uint32_t ridx = NO_RAY;
     bool drained = false;
     /* wave-local pool [pool, pool_end) of ray indices taken from the queue 64 at a time */
@@ -837,7 +838,9 @@ __global__ EXT_BOUNDS void k_extend_refill(KParams P, int depth, int* ovf_base) 
             }
             const uint32_t take = n_idle < pool_end - pool ? n_idle : pool_end - pool;
             if (idle) {
-                const uint32_t rank = (uint32_t)__popcll(idle_mask & lt_mask);
+                /* idle lanes below this one */
+                const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(idle_mask >> 32),
+                                                                __builtin_amdgcn_mbcnt_lo((uint32_t)idle_mask, 0u));
                 if (rank < take) {
                     const uint32_t r = pool + rank;
                     ridx = r;
@@ -986,7 +989,7 @@ __global__ EXT_BOUNDS void k_extend_pers(KParams P, int depth, int* ovf_base) {
     const int lane = threadIdx.x;
     Stack<OVF> st;
     st.lds = (LDSAS int*)stk + lane;
-    st.ovf = gpw(ovf_base) + ((size_t)blockIdx.x * EXT_BLOCK + lane) * OVF_STACK;
+    st.ovf = gpw(ovf_base) + (size_t)blockIdx.x * EXT_BLOCK * OVF_STACK;
     st.sp = 0;
     GLOBAL DCtl* ctl = gpw(P.ctl);
     const uint32_t count = ctl->count[depth];
@@ -996,7 +999,6 @@ __global__ EXT_BOUNDS void k_extend_pers(KParams P, int depth, int* ovf_base) {
     const GLOBAL uint2* inr = gp(P.in.rng);
     const GLOBAL DReq* reqs = gp(P.chunk)->reqs;
     const GLOBAL DScene* Su = UNI ? uni_p(gp(reqs[0].scene)) : nullptr;
-    const unsigned long long lt_mask = (1ull << lane) - 1ull;
     const uint32_t refill_min = (uint32_t)P.refill_min;
     const uint32_t leaf_pct = (uint32_t)P.leaf_pct;
     uint32_t ridx = NO_RAY;
@@ -1021,7 +1023,9 @@ __global__ EXT_BOUNDS void k_extend_pers(KParams P, int depth, int* ovf_base) {
             }
             const uint32_t take = n_idle < pool_end - pool ? n_idle : pool_end - pool;
             if (idle) {
-                const uint32_t rank = (uint32_t)__popcll(idle_mask & lt_mask);
+                /* idle lanes below this one */
+                const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(idle_mask >> 32),
+                                                                __builtin_amdgcn_mbcnt_lo((uint32_t)idle_mask, 0u));
                 if (rank < take) {
                     const uint32_t r = pool + rank;
                     ridx = r;
