@@ -150,7 +150,7 @@ def subsurface(ctx):
 
 
 def test_path_radiance_subsurface_medium(ctx, subsurface):
-    """Config 2's scene: most segments are scatter events inside the 525k-triangle medium, where
+    """The subsurface scene (960x720, 525k triangles): most segments are scatter events inside the medium, where
     the extend kernel stops each query at the sampled free-flight distance."""
     spec, st, sc, orc = subsurface
     p = _params(st)
@@ -171,7 +171,7 @@ def _first_divergence(ctx, sc, orc, p, pixel, pass_):
 
 
 def test_subsurface_differing_paths_diverge_at_ties(ctx, subsurface):
-    """Pixels of the config-2 image where the device and the oracle differ (found by a full
+    """Pixels of the subsurface image (960x720, 32 spp) where the device and the oracle differ (found by a full
     render): every differing path first parts from the oracle at a closest hit whose distance
     equals the oracle's bit for bit on another triangle (an edge both triangles share), never in
     shading or in the free-flight cap."""
@@ -196,7 +196,7 @@ def test_subsurface_differing_paths_diverge_at_ties(ctx, subsurface):
 
 def test_subsurface_ties_resolved_independently_of_the_walk(ctx, subsurface):
     """Exact distance ties are resolved by (instance, primitive), not by the order a walk meets
-    the triangles, so the ten tie-bearing pixels of the config-2 image give bit-identical paths
+    the triangles, so the ten tie-bearing pixels of the subsurface image give bit-identical paths
     under both node layouts and with camera rays walked per lane or as wave packets (whose
     visiting order depends on which rays share the wave)."""
     import illume_amd
