@@ -9,6 +9,9 @@
  * mesh_from_json (src/primitives/mesh.cu:430-443), Tonemapper (src/tonemapper.cu:5-30).
  * Fallbacks keep the reference's: unknown material type -> black diffuse, unknown conductor ->
  * "Cu", unknown mesh name -> mesh 0, missing resolution -> 512x512.
+ * A camera without "depth_of_field" focuses at distance 1 (camera_from_json leaves the field
+ * uninitialised; with aperture 0 the focus distance does not change ray directions, while 0
+ * would make every camera ray direction 0/0).
  * Additive keys (no reference counterpart): material "eta"/"k" (explicit conductor IOR) and
  * "distribution": "ggx"; bitmap textures read PFM (float) and binary PPM (8-bit) files.
  * The OBJ reader accepts faces with more than three vertices (fan triangulation) and long
@@ -439,7 +442,7 @@ int ilm_desc_load_json(const char* path, ilm_desc** out) {
     cam.transform = transform_from_json(nullptr);
     if (const JV* c = root.get("camera")) {
         cam.fov = get_num(*c, "fov", 0);
-        cam.dof = get_num(*c, "depth_of_field", 0);
+        cam.dof = get_num(*c, "depth_of_field", 1);
         cam.aperture = get_num(*c, "aperture_radius", 0);
         cam.transform = transform_from_json(c->get("transform"));
     }

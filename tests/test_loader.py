@@ -122,6 +122,7 @@ def test_loader_quads_and_negative_indices(tmp_path):
     (tmp_path / "s.json").write_text(json.dumps({"meshes": [{"name": "q", "file": "q.obj"}], "camera": {"fov": 40}}))
     d = illume_amd.load_json(str(tmp_path / "s.json")).desc
     assert d.meshes[0].num_triangles == 2 and d.meshes[0].num_vertices == 4
+    assert d.camera.dof == 1.0 and d.camera.aperture == 0.0   # no depth_of_field: focus at 1
 
 
 REF_CORNELL = "/root/reference/res/scenes/cornell_box.json"
