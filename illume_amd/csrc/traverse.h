@@ -606,13 +606,23 @@ __global__ __launch_bounds__(EXT_BLOCK) void k_extend_packet(KParams P, int dept
     const GLOBAL DReq* reqs = gp(P.chunk)->reqs;
     TravCount tc = {0u, 0u};
     const uint32_t guide = gridDim.x * 2u;
+    /* the first reservation is fixed by the wave's index; the cursor hands out the rest (one
+     * address's atomics serialise, so a wave with nothing to do should not touch it) */
+    const uint32_t first = gridDim.x * fetch;
     uint32_t want = fetch;
     uint32_t b = 0, end = 0;
+    bool first_trip = true;
     for (;;) {
         if (b >= end) {
             uint32_t base = 0;
-            if (lane == 0) base = atomicAdd(&ctl->fetch[depth], want);
-            base = (uint32_t)uni_i((int)__shfl(base, 0));
+            if (first_trip) {
+                base = blockIdx.x * fetch;
+                first_trip = false;
+            } else {
+                if (first >= count) break;
+                if (lane == 0) base = atomicAdd(&ctl->fetch[depth], want);
+                base = first + (uint32_t)uni_i((int)__shfl(base, 0));
+            }
             if (base >= count) break;
             b = base;
             end = base + want < count ? base + want : count;
@@ -677,11 +687,19 @@ __global__ EXT_BOUNDS void k_extend(KParams P, int depth, int* ovf_base) {
     /* guided self-scheduling: reservations shrink with the rays left in the queue (down to one
      * batch), so the last waves of a launch finish together; ext_fetch is the upper bound */
     const uint32_t guide = gridDim.x * 2u;
+    /* the first reservation is fixed by the wave's index; the cursor hands out the rest (one
+     * address's atomics serialise, so a wave with nothing to do should not touch it) */
+    const uint32_t first = gridDim.x * fetch;
     uint32_t want = fetch;
-    for (;;) {
+    for (bool first_trip = true;; first_trip = false) {
         uint32_t base = 0;
-        if (lane == 0) base = atomicAdd(&ctl->fetch[depth], want);
-        base = __shfl(base, 0);
+        if (first_trip) {
+            base = blockIdx.x * fetch;
+        } else {
+            if (first >= count) break;
+            if (lane == 0) base = atomicAdd(&ctl->fetch[depth], want);
+            base = first + __shfl(base, 0);
+        }
         if (base >= count) break;
         const uint32_t end = base + want < count ? base + want : count;
         const uint32_t left = count - end;
@@ -802,6 +820,32 @@ __device__ inline bool seg_run(TState& T, Best& B, Stack<OVF>& st, const GLOBAL 
     }
 }
 
+/* Work queue of the per-lane extend kernels: wave w first takes rays [w*fetch, (w+1)*fetch)
+ * without an atomic, and the shared cursor then hands out what lies past the grid's first
+ * reservations, `fetch` rays per atomic. A wave with no rays leaves without touching the cursor:
+ * atomics on one address serialise, and a cursor that every one of the grid's ~7k waves had to
+ * touch cost a launch ~120 us however short its queue (the last depths of a render hold few
+ * rays). Neighbouring waves keep neighbouring rays (coherent in the queue): first spans scaled to
+ * an even split of a long queue measured slower. */
+__device__ inline void pool_first(uint32_t count, uint32_t fetch, uint32_t& pool, uint32_t& pool_end) {
+    const uint32_t b = blockIdx.x * fetch;
+    pool = b < count ? b : count;
+    pool_end = b + fetch < count ? b + fetch : count;
+}
+__device__ inline void pool_next(GLOBAL uint32_t* cursor, uint32_t count, uint32_t fetch, int lane, uint32_t& pool,
+                                 uint32_t& pool_end) {
+    const uint32_t first = gridDim.x * fetch;
+    if (first >= count) {
+        pool = pool_end = count;
+        return;
+    }
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(cursor, fetch);
+    base = first + __shfl(base, 0);
+    pool = base < count ? base : count;
+    pool_end = base + fetch < count ? base + fetch : count;
+}
+
 template <int TRI, int QUANT, int COUNT, int OVF>
 __global__ EXT_BOUNDS void k_extend_refill(KParams P, int depth, int* ovf_base) {
     __shared__ int stk[LDS_STACK * EXT_BLOCK];
@@ -818,9 +862,10 @@ __global__ EXT_BOUNDS void k_extend_refill(KParams P, int depth, int* ovf_base) 
     const GLOBAL uint2* inr = gp(P.in.rng);
     const GLOBAL DReq* reqs = gp(P.chunk)->reqs;
     uint32_t ridx = NO_RAY;
-    bool drained = false;
-    /* wave-local pool [pool, pool_end) of ray indices taken from the queue 64 at a time */
-    uint32_t pool = 0, pool_end = 0;
+    /* wave-local pool [pool, pool_end) of ray indices taken from the queue */
+    uint32_t pool, pool_end;
+    pool_first(count, fetch, pool, pool_end);
+    bool drained = pool == pool_end;
     TState T;
     Best B;
     TravCount tc = {0u, 0u};
@@ -829,13 +874,7 @@ __global__ EXT_BOUNDS void k_extend_refill(KParams P, int depth, int* ovf_base) 
         const unsigned long long idle_mask = __ballot(idle);
         const uint32_t n_idle = (uint32_t)__popcll(idle_mask);
         if (!drained && n_idle >= (uint32_t)P.refill_min) {
-            if (pool == pool_end) {
-                uint32_t base = 0;
-                if (lane == 0) base = atomicAdd(&ctl->fetch[depth], fetch);
-                base = __shfl(base, 0);
-                pool = base < count ? base : count;
-                pool_end = base + fetch < count ? base + fetch : count;
-            }
+            if (pool == pool_end) pool_next(&ctl->fetch[depth], count, fetch, lane, pool, pool_end);
             const uint32_t take = n_idle < pool_end - pool ? n_idle : pool_end - pool;
             if (idle) {
                 /* idle lanes below this one */
@@ -1002,8 +1041,9 @@ __global__ EXT_BOUNDS void k_extend_pers(KParams P, int depth, int* ovf_base) {
     const uint32_t refill_min = (uint32_t)P.refill_min;
     const uint32_t leaf_pct = (uint32_t)P.leaf_pct;
     uint32_t ridx = NO_RAY;
-    bool drained = false;
-    uint32_t pool = 0, pool_end = 0;
+    uint32_t pool, pool_end;
+    pool_first(count, fetch, pool, pool_end);
+    bool drained = pool == pool_end;
     PState T;
     Best B;
     TravCount tc = {0u, 0u};
@@ -1014,13 +1054,7 @@ __global__ EXT_BOUNDS void k_extend_pers(KParams P, int depth, int* ovf_base) {
         if (drained) {
             if (n_idle == EXT_BLOCK) break;
         } else if (n_idle >= refill_min) {
-            if (pool == pool_end) {
-                uint32_t base = 0;
-                if (lane == 0) base = atomicAdd(&ctl->fetch[depth], fetch);
-                base = __shfl(base, 0);
-                pool = base < count ? base : count;
-                pool_end = base + fetch < count ? base + fetch : count;
-            }
+            if (pool == pool_end) pool_next(&ctl->fetch[depth], count, fetch, lane, pool, pool_end);
             const uint32_t take = n_idle < pool_end - pool ? n_idle : pool_end - pool;
             if (idle) {
                 /* idle lanes below this one */
