@@ -827,6 +827,15 @@ __device__ inline bool seg_run(TState& T, Best& B, Stack<OVF>& st, const GLOBAL 
  * touch cost a launch ~120 us however short its queue (the last depths of a render hold few
  * rays). Neighbouring waves keep neighbouring rays (coherent in the queue): first spans scaled to
  * an even split of a long queue measured slower. */
+/* Rays per reservation: `fetch`, or for a queue shorter than the grid's first reservations,
+ * about an even share of it in whole batches of 64, so that the rays spread over the waves
+ * instead of queueing behind one another in a few. */
+__device__ inline uint32_t pool_fetch(uint32_t count, uint32_t fetch) {
+    const uint32_t per = (count + gridDim.x - 1u) / gridDim.x;
+    const uint32_t share = per <= (uint32_t)EXT_BLOCK ? (uint32_t)EXT_BLOCK
+                                                      : (per + (uint32_t)(EXT_BLOCK - 1)) & ~(uint32_t)(EXT_BLOCK - 1);
+    return share < fetch ? share : fetch;
+}
 __device__ inline void pool_first(uint32_t count, uint32_t fetch, uint32_t& pool, uint32_t& pool_end) {
     const uint32_t b = blockIdx.x * fetch;
     pool = b < count ? b : count;
@@ -856,7 +865,7 @@ __global__ EXT_BOUNDS void k_extend_refill(KParams P, int depth, int* ovf_base) 
     st.sp = 0;
     GLOBAL DCtl* ctl = gpw(P.ctl);
     const uint32_t count = ctl->count[depth];
-    const uint32_t fetch = (uint32_t)P.ext_fetch;
+    const uint32_t fetch = pool_fetch(count, (uint32_t)P.ext_fetch);
     const GLOBAL float4* ino = gp(P.in.o);
     const GLOBAL float4* ind = gp(P.in.d);
     const GLOBAL uint2* inr = gp(P.in.rng);
@@ -1032,7 +1041,7 @@ __global__ EXT_BOUNDS void k_extend_pers(KParams P, int depth, int* ovf_base) {
     st.sp = 0;
     GLOBAL DCtl* ctl = gpw(P.ctl);
     const uint32_t count = ctl->count[depth];
-    const uint32_t fetch = (uint32_t)P.ext_fetch;
+    const uint32_t fetch = pool_fetch(count, (uint32_t)P.ext_fetch);
     const GLOBAL float4* ino = gp(P.in.o);
     const GLOBAL float4* ind = gp(P.in.d);
     const GLOBAL uint2* inr = gp(P.in.rng);
