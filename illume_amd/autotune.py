@@ -11,7 +11,7 @@ measures time.
                      2: idle lanes refill from a wave-local queue at BLAS leaves
                      3 (and 0, the default): idle lanes refill after every traversal step
     refill_min       idle lanes that trigger a refill (modes 1 and 2)
-    ext_fetch        rays an extend wave reserves per queue atomic (work-queue granularity)
+    ext_fetch        rays an extend wave reserves from the queue at a time (work-queue granularity)
     leaf_wait_pct    per-step refill: lanes waiting at BLAS leaves (as a percentage of the other
                      busy lanes) before a wave spends a trip on triangle tests
     shade_sort       1: each 512-path shading block runs its paths grouped by material stand-in

@@ -207,7 +207,9 @@ typedef struct ilm_opts {
     const char* bvh_cache_dir;
     /* refill modes: number of idle lanes of a wave that triggers a refill (0 = 32) */
     int32_t refill_min;
-    /* rays an extend wave reserves per queue atomic (multiple of 64; 0 = 256) */
+    /* rays an extend wave reserves from the queue at a time (multiple of 64; 0 = 256): its first
+     * reservation by its index, later ones from the shared cursor; a queue shorter than the
+     * grid's first reservations is split evenly over the waves instead */
     int32_t ext_fetch;
     /* per-step refill: a wave runs its BLAS-leaf triangle tests only on trips where the lanes
      * waiting at leaves are at least this percentage of its other busy lanes (0 = 100) */
