@@ -11,6 +11,7 @@
  */
 #include "bvh_cache.h"
 
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <unistd.h>
@@ -62,6 +63,105 @@ uint64_t payload_hash(const MeshBVH& b) {
     H.bytes(b.qnodes.data(), b.qnodes.size() * sizeof(QNode));
     H.bytes(b.order.data(), b.order.size() * sizeof(int));
     return H.h;
+}
+
+/* A child code of either layout: an inner node index below n_nodes, or a leaf of 1..16
+ * triangles inside [0, n_order) whose slots are marked in cover (each slot once). */
+bool leaf_ok(int32_t c, size_t n_order, std::vector<unsigned char>& cover) {
+    const uint64_t first = ilm_leaf_first(c), count = ilm_leaf_count(c);
+    if (first + count > n_order) return false;
+    for (uint64_t k = first; k < first + count; k++) {
+        if (cover[k]) return false;
+        cover[k] = 1;
+    }
+    return true;
+}
+
+/* Walks both layouts from their roots (the file's own child indices are never trusted): every
+ * inner index in range and reached exactly once (a tree: no cycle, no sharing, no orphan), every
+ * leaf range inside the triangle order and the leaves of each layout covering every slot once.
+ * Sets depth and qdepth to what the walk finds: the FP32 tree's depth counts the leaf level
+ * (root 0), the 8-bit tree's the depth of its deepest node. */
+bool layouts_valid(MeshBVH& b) {
+    const size_t n_order = b.order.size();
+    struct Item { int32_t node; int depth; };
+    std::vector<Item> st;
+    /* FP32 BVH2 */
+    {
+        std::vector<unsigned char> seen(b.nodes.size(), 0), cover(n_order, 0);
+        int depth = 0;
+        size_t visited = 0;
+        if (b.root == ILM_NO_CHILD) {
+            if (!b.nodes.empty() || n_order != 0) return false;
+        } else if (b.root < 0) {
+            if (!b.nodes.empty() || !leaf_ok(b.root, n_order, cover)) return false;
+        } else {
+            if (b.root != 0 || b.nodes.empty()) return false;
+            st.push_back({0, 0});
+        }
+        while (!st.empty()) {
+            const Item it = st.back();
+            st.pop_back();
+            if (it.node < 0 || (size_t)it.node >= b.nodes.size() || seen[it.node]) return false;
+            seen[it.node] = 1;
+            visited++;
+            const int32_t kids[2] = {b.nodes[it.node].d.x, b.nodes[it.node].d.y};
+            for (int32_t c : kids) {
+                if (c == ILM_NO_CHILD) return false;
+                if (c >= 0) {
+                    st.push_back({c, it.depth + 1});
+                } else {
+                    if (!leaf_ok(c, n_order, cover)) return false;
+                    depth = std::max(depth, it.depth + 1);
+                }
+            }
+        }
+        if (visited != b.nodes.size()) return false;
+        for (unsigned char v : cover)
+            if (!v) return false;
+        b.depth = depth;
+    }
+    /* 8-bit BVH4 */
+    {
+        std::vector<unsigned char> seen(b.qnodes.size(), 0), cover(n_order, 0);
+        int depth = 0;
+        size_t visited = 0;
+        if (b.qroot == ILM_NO_CHILD) {
+            if (!b.qnodes.empty() || n_order != 0) return false;
+        } else if (b.qroot < 0) {
+            if (!b.qnodes.empty() || !leaf_ok(b.qroot, n_order, cover)) return false;
+        } else {
+            if (b.qroot != 0 || b.qnodes.empty()) return false;
+            st.push_back({0, 0});
+        }
+        while (!st.empty()) {
+            const Item it = st.back();
+            st.pop_back();
+            if (it.node < 0 || (size_t)it.node >= b.qnodes.size() || seen[it.node]) return false;
+            seen[it.node] = 1;
+            visited++;
+            depth = std::max(depth, it.depth);
+            const QNode& q = b.qnodes[it.node];
+            if (q.nchild < 1 || q.nchild > 4) return false;
+            for (int k = 0; k < 4; k++) {
+                const int32_t c = q.child[k];
+                if (k >= q.nchild) {
+                    if (c != ILM_NO_CHILD) return false;
+                } else if (c == ILM_NO_CHILD) {
+                    return false;
+                } else if (c >= 0) {
+                    st.push_back({c, it.depth + 1});
+                } else if (!leaf_ok(c, n_order, cover)) {
+                    return false;
+                }
+            }
+        }
+        if (visited != b.qnodes.size()) return false;
+        for (unsigned char v : cover)
+            if (!v) return false;
+        b.qdepth = depth;
+    }
+    return true;
 }
 
 std::string entry_path(const std::string& dir, uint64_t key) {
@@ -134,6 +234,9 @@ bool cache_load(const std::string& dir, uint64_t key, int num_triangles, MeshBVH
                     seen[t] = 1;
                 }
             }
+            /* the tree itself: indices in range, a proper tree, depths recomputed (a stale or
+             * crafted entry whose header understates them would overflow the traversal stack) */
+            if (ok) ok = layouts_valid(b) && b.depth == h.depth && b.qdepth == h.qdepth;
             if (ok) out = std::move(b);
         }
     }

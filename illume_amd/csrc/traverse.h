@@ -820,13 +820,6 @@ __device__ inline bool seg_run(TState& T, Best& B, Stack<OVF>& st, const GLOBAL 
     }
 }
 
-/* Work queue of the per-lane extend kernels: wave w first takes rays [w*fetch, (w+1)*fetch)
- * without an atomic, and the shared cursor then hands out what lies past the grid's first
- * reservations, `fetch` rays per atomic. A wave with no rays leaves without touching the cursor:
- * atomics on one address serialise, and a cursor that every one of the grid's ~7k waves had to
- * touch cost a launch ~120 us however short its queue (the last depths of a render hold few
- * rays). Neighbouring waves keep neighbouring rays (coherent in the queue): first spans scaled to
- * an even split of a long queue measured slower. */
 /* Rays per reservation: `fetch`, or for a queue shorter than the grid's first reservations,
  * about an even share of it in whole batches of 64, so that the rays spread over the waves
  * instead of queueing behind one another in a few. */
@@ -836,6 +829,13 @@ __device__ inline uint32_t pool_fetch(uint32_t count, uint32_t fetch) {
                                                       : (per + (uint32_t)(EXT_BLOCK - 1)) & ~(uint32_t)(EXT_BLOCK - 1);
     return share < fetch ? share : fetch;
 }
+/* Work queue of the per-lane extend kernels: wave w first takes rays [w*fetch, (w+1)*fetch)
+ * without an atomic, and the shared cursor then hands out what lies past the grid's first
+ * reservations, `fetch` rays per atomic. A wave with no rays leaves without touching the cursor:
+ * atomics on one address serialise, and a cursor that every one of the grid's ~7k waves had to
+ * touch cost a launch ~120 us however short its queue (the last depths of a render hold few
+ * rays). Neighbouring waves keep neighbouring rays (coherent in the queue): first spans scaled to
+ * an even split of a long queue measured slower. */
 __device__ inline void pool_first(uint32_t count, uint32_t fetch, uint32_t& pool, uint32_t& pool_end) {
     const uint32_t b = blockIdx.x * fetch;
     pool = b < count ? b : count;
@@ -920,7 +920,8 @@ __global__ EXT_BOUNDS void k_extend_refill(KParams P, int depth, int* ovf_base) 
  * inner-node visit, one instance entry or one BLAS leaf, plus the pops that follow), and the
  * wave re-checks for idle lanes on every trip, so a lane whose ray finished takes a new one
  * after at most one trip instead of waiting for the rest of the wave to reach a leaf. The world
- * ray stays in registers for the return from an instance. Selected with extend_mode = 2. */
+ * ray stays in registers for the return from an instance. Selected with extend_mode = 3
+ * (ILM_EXTEND_STEP_REFILL) and by the default, 0. */
 /* PSTATE_LEAN: the reciprocal direction and -o/d are recomputed at each node visit from the
  * current ray instead of being carried across loop trips (fewer live registers and fewer
  * register copies on the hot path, for a few extra VALU operations per visit). */

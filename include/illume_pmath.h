@@ -13,7 +13,7 @@
  *   log     : x = 2^e m, m in (sqrt2/2, sqrt2], log m = 2 atanh(s), s = (m-1)/(m+1), series to s^13
  *   atan    : 1/x for |x|>1, pi/4 shift for |x|>tan(pi/8), Taylor to t^19
  *   atan2, asin (via atan2 with sqrt((1-x)(1+x))), pow = exp(y log x)
- * Accuracy is a few ulp over the ranges the renderer uses; tests/test_pmath.py states the bounds
+ * Accuracy is a few ulp over the ranges the renderer uses; tests/test_oracle.py states the bounds
  * measured against double-precision libm.
  *
  * PCG32 is O'Neill's permuted congruential generator (pcg-random.org, "pcg32" XSH-RR, 64-bit

@@ -89,6 +89,16 @@ class OracleScene:
         self.h = lib().orc_scene_create2(C.byref(self.desc), int(fp64), int(threads), int(median_bvh))
         self.fp64 = fp64
 
+    @classmethod
+    def from_desc(cls, desc, owner=None, fp64=False, threads=0, median_bvh=False):
+        """The oracle's copy of an ilm_scene_desc (e.g. one the scene-JSON loader produced);
+        owner keeps the memory the descriptor points into alive."""
+        self = cls.__new__(cls)
+        self.desc, self._keep = desc, owner
+        self.h = lib().orc_scene_create2(C.byref(desc), int(fp64), int(threads), int(median_bvh))
+        self.fp64 = fp64
+        return self
+
     def free(self):
         if self.h:
             lib().orc_scene_free(self.h)

@@ -134,3 +134,96 @@ def test_bvh_cache_round_trip_and_rejection(tmp_path):
     other = bvh_device_layouts(p2, m["indices"], cache_dir=str(tmp_path))
     assert not other[4]
     assert len(list(tmp_path.glob("*.ilmbvh"))) == 2
+
+
+class _CacheHash:
+    """The cache's payload hash (illume_amd/csrc/bvh_cache.cpp Hasher), to forge entries whose
+    checksum matches while their tree is wrong."""
+    M = (1 << 64) - 1
+
+    def __init__(self):
+        self.h = 0x243f6a8885a308d3
+
+    def mix(self, w):
+        h = self.h
+        h ^= (w + 0x9e3779b97f4a7c15 + ((h << 6) & self.M) + (h >> 2)) & self.M
+        h = (h * 0xff51afd7ed558ccd) & self.M
+        h ^= h >> 33
+        self.h = h
+
+    def bytes(self, b):
+        n, i = len(b), 0
+        while i + 8 <= n:
+            self.mix(int.from_bytes(b[i:i + 8], "little"))
+            i += 8
+        self.mix(int.from_bytes(b[i:] + bytes(8 - (n - i)), "little") ^ (((n - i) << 56) & self.M))
+        self.mix(n)
+
+
+def _forge(path, edit):
+    """Rewrite a cache entry after edit(header dict, nodes, qnodes, order), with the payload hash
+    recomputed so that only the tree checks can reject it."""
+    import struct
+    raw = bytearray(path.read_bytes())
+    fmt = "<8sIIQQiiiiQQQII"
+    hsz = struct.calcsize(fmt)
+    f = list(struct.unpack(fmt, raw[:hsz]))
+    hdr = dict(zip(["magic", "version", "hb", "key", "hash", "root", "qroot", "depth", "qdepth", "nn", "nq", "no",
+                    "sb", "sq"], f))
+    nb, qb = hdr["nn"] * 64, hdr["nq"] * 64
+    nodes = np.frombuffer(bytes(raw[hsz:hsz + nb]), np.int32).reshape(-1, 16).copy()
+    qnodes = np.frombuffer(bytes(raw[hsz + nb:hsz + nb + qb]), np.int32).reshape(-1, 16).copy()
+    order = np.frombuffer(bytes(raw[hsz + nb + qb:]), np.int32).copy()
+    edit(hdr, nodes, qnodes, order)
+    H = _CacheHash()
+    for a in (nodes, qnodes, order):
+        H.bytes(a.tobytes())
+    hdr["hash"] = H.h
+    head = struct.pack(fmt, *[hdr[k] for k in ["magic", "version", "hb", "key", "hash", "root", "qroot", "depth",
+                                               "qdepth", "nn", "nq", "no", "sb", "sq"]])
+    path.write_bytes(head + nodes.tobytes() + qnodes.tobytes() + order.tobytes())
+
+
+def test_bvh_cache_rejects_entries_with_a_matching_hash_but_a_bad_tree(tmp_path):
+    """The payload hash only catches accidents: an entry whose checksum is recomputed after its
+    tree was changed must still be refused (and rebuilt) when a child index is out of range, a
+    child points back up the tree, a leaf range runs past the triangles, or the header
+    understates a depth that bounds the device traversal stack."""
+    from illume_amd import bvh_device_layouts
+    m = S.displaced_icosphere(4, 3)
+    fresh = bvh_device_layouts(m["positions"], m["indices"])
+    assert not bvh_device_layouts(m["positions"], m["indices"], cache_dir=str(tmp_path))[4]
+    path = next(tmp_path.glob("*.ilmbvh"))
+    good = path.read_bytes()
+    _forge(path, lambda h, n, q, o: None)    # unchanged tree, recomputed hash: still accepted
+    assert bvh_device_layouts(m["positions"], m["indices"], cache_dir=str(tmp_path))[4]
+
+    def bad_child(h, n, q, o):
+        n[1, 12] = h["nn"] + 5              # BNode child0 (bytes 48..51) out of range
+
+    def cycle(h, n, q, o):
+        n[1, 13] = 0                         # child1 back to the root
+
+    def leaf_past_end(h, n, q, o):
+        for i in range(q.shape[0]):         # first 8-bit leaf child: push its range past the end
+            for k in range(4):
+                c = int(q[i, 12 + k])
+                if c < 0 and c != 0x7fffffff:
+                    q[i, 12 + k] = np.int32(np.uint32(0x80000000 | (3 << 27) | (h["no"] - 2)).view(np.int32))
+                    return
+
+    def shallow(h, n, q, o):
+        h["depth"] = 1                       # understated FP32 depth
+
+    def qshallow(h, n, q, o):
+        h["qdepth"] = 0
+
+    for edit in (bad_child, cycle, leaf_past_end, shallow, qshallow):
+        path.write_bytes(good)
+        _forge(path, edit)
+        got = bvh_device_layouts(m["positions"], m["indices"], cache_dir=str(tmp_path))
+        assert not got[4], edit.__name__     # refused, rebuilt and rewritten
+        for a, b in zip(fresh[:3], got[:3]):
+            assert np.array_equal(a, b), edit.__name__
+        assert got[3] == fresh[3]
+        assert bvh_device_layouts(m["positions"], m["indices"], cache_dir=str(tmp_path))[4]
