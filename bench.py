@@ -20,18 +20,26 @@ tonemap and pixel readback), with the scene and its BVH already resident in HBM.
 Prints ONE JSON line on stdout; progress goes to stderr.
 
 value       segments/s / 1e6 over the timed steps, timed on the device (HIP events on the render
-            stream from the first kernel of each render to its tonemap: scene and path state are
-            resident, the PCIe copy of the image to the caller is excluded and reported beside it
-            as value_pcie_inclusive, the wall clock around the synchronous calls); a segment is one
-            closest-hit query (src/renderer.cu:554 counts the same)
-roofline    extend kernel: algorithmic bytes B_seg = 28*N_box + 36*N_tri + 128 per segment,
-            N_box / N_tri the mean child-box / triangle tests per query of the FP32 BVH counted
-            by the kernel's own counters on the same workload; achieved = B_seg * segments /
-            summed extend time (HIP events around every extend launch, on the render stream)
-            per launch; peak 8 TB/s HBM3E
+            stream from the first kernel of each render to its tonemap, each render one replay of
+            captured hipGraphs: scene and path state are resident, the PCIe copy of the image to
+            the caller is excluded and reported beside it as value_pcie_inclusive, the wall clock
+            around the synchronous calls); a segment is one closest-hit query
+            (src/renderer.cu:554 counts the same)
+roofline    extend kernel, layout-neutral convention: algorithmic bytes B_seg = 28*N_box +
+            36*N_tri + 128 per segment, N_box / N_tri the mean child-box / triangle tests per
+            closest-hit query of per-ray ordered, t-pruned walks over the FP32 BVH2, counted by
+            the kernel's own counters on the same workload (a separate, untimed render);
+            achieved = B_seg * segments / summed extend-launch time, i.e. bytes per launch over
+            the average launch duration (HIP events on the render stream around every extend
+            launch, in a profiled pass of the same renders right after the timed steps); peak
+            8 TB/s HBM3E. roofline.layout repeats it with the benchmarked node layout's own bytes
+            (16 B per 8-bit BVH4 child test) and counts; traffic / hbm_measured come from the
+            rocprofv3 counter summaries in profiles/
 cpu_baseline  the oracle (CPU restatement of the reference per-path routine: exhaustive
             traversal of the reference's sweep-built BVH, linear instance loop) on a band of
-            the same image, OpenMP over pixels on the host's cores; a one-core figure beside it
+            the same image, OpenMP over pixels on the host's cores; beside it a one-core figure
+            and value_tpruned, the same band with the build's ordered, t-pruned traversal over
+            the same trees (algorithmic vs hardware speedup), and the host CPU model
 """
 import argparse
 import json
@@ -92,6 +100,29 @@ def traffic_from_profiles(cfg, layout):
 BOX_BYTES = {"fp32": 28.0, "quant8": 16.0}
 
 
+def profile_value(cfg, key):
+    """A `key: value` line of profiles/extend_config<cfg>_counters.txt (rocprofv3 summaries), or None."""
+    path = os.path.join(ROOT, "profiles", "extend_config%d_counters.txt" % cfg)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith(key + ":"):
+            return float(line.split(":")[1])
+    return None
+
+
+def host_cpu():
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return model, os.cpu_count()
+
+
 def cpu_baseline(spec, st, target_s=15.0, target_1core_s=5.0):
     """Oracle render (FP32, reference BVH and traversal) of a band of rows of the same image at
     full spp, sized to about target_s of work on all cores; then one core on a smaller band."""
@@ -125,6 +156,17 @@ def cpu_baseline(spec, st, target_s=15.0, target_1core_s=5.0):
     _, _, segs1, secs1 = orc.render(p, pb, pb + n1, kahan=True, threads=1, pixels=False)
     out["value_1core"] = segs1 / secs1 / 1e6
     out["sample_1core"] = "%d px from row %d x %d spp: %d segments in %.2f s" % (n1, H // 2, st["spp"], segs1, secs1)
+    # the same band with the build's ordered, t-pruned traversal over the same trees
+    orc.set_tpruned(True)
+    r0, rows, pb, pe = band(rows)
+    _, _, segs_t, secs_t = orc.render(p, pb, pe, kahan=True, threads=threads, pixels=False)
+    orc.set_tpruned(False)
+    out["value_tpruned"] = segs_t / secs_t / 1e6
+    out["sample_tpruned"] = ("same %d px x %d spp, ordered t-pruned traversal of the same sweep-SAH trees: %d segments "
+                             "in %.2f s" % (pe - pb, st["spp"], segs_t, secs_t))
+    model, ncpu = host_cpu()
+    out["cpu_model"] = model
+    out["host_logical_cpus"] = ncpu
     orc.free()
     return out
 
@@ -175,6 +217,24 @@ def ctx_opts(args):
     return o
 
 
+def count_tests(spec, p, shape, args, layout, packets):
+    """Mean child-box and triangle tests per closest-hit query, from the extend kernel's own
+    counters on a separate render of the same workload (node layout and camera-packet setting
+    given; the launch shape otherwise the benchmarked one)."""
+    import illume_amd
+    opts = ctx_opts(args)
+    opts["bvh_layout"] = layout
+    with illume_amd.Context(0, count_tests=1, **opts) as cctx:
+        sh = dict(shape)
+        sh["bvh_layout"] = 1 if layout == "quant8" else 0
+        sh["camera_packets"] = 1 if packets else 2
+        cctx.tune(**sh)
+        cscene = cctx.scene(spec)
+        _, _, cinfo = cctx.render(cscene, p, want_accum=False, want_pixels=False)
+        cscene.free()
+    return cinfo["box_tests"] / max(1, cinfo["segments"]), cinfo["tri_tests"] / max(1, cinfo["segments"])
+
+
 def bench_plain(args):
     import illume_amd
     from illume_amd import scenes as S
@@ -183,7 +243,7 @@ def bench_plain(args):
     t0 = time.time()
     spec, st = S.CONFIGS[args.config]()
     log("scene generated in %.2f s" % (time.time() - t0))
-    ctx = illume_amd.Context(0, profile=1, **ctx_opts(args))
+    ctx = illume_amd.Context(0, **ctx_opts(args))
     t0 = time.time()
     scene = ctx.scene(spec)
     upload_s = time.time() - t0
@@ -205,9 +265,8 @@ def bench_plain(args):
         _, _, info = ctx.render(scene, p, want_accum=False)
         log("warmup %d: %.1f ms, %d segments" % (i, info["device_ms"], info["segments"]))
 
+    # timed steps: each render one replay of its captured graphs
     segs = 0
-    ext_ms = 0.0
-    ext_launches = 0
     lat = []
     dev_ms = 0.0
     t_start = time.perf_counter()
@@ -216,52 +275,82 @@ def bench_plain(args):
         _, _, info = ctx.render(scene, p, want_accum=False)
         lat.append((time.perf_counter() - ts) * 1e3)
         segs += info["segments"]
-        ext_ms += info["extend_ms"]
-        ext_launches += info["extend_launches"]
         dev_ms += info["device_ms"]
-        log("step %d: %.1f ms wall, %.1f ms device, %d segments, extend %.1f ms" % (i, lat[-1], info["device_ms"],
-                                                                                    info["segments"], info["extend_ms"]))
+        log("step %d: %.1f ms wall, %.2f ms device, %d segments" % (i, lat[-1], info["device_ms"], info["segments"]))
     wall = time.perf_counter() - t_start
     value = segs / (dev_ms / 1e3) / 1e6
+    shape = ctx.settings()
+    layout = "quant8" if shape["bvh_layout"] == 1 else "fp32"
 
-    layout = "quant8" if ctx.settings()["bvh_layout"] == 1 else "fp32"
-    # traversal test counts on the same workload and node layout (a separate, untimed render)
     roof = None
     if not args.no_count:
-        cctx = illume_amd.Context(0, count_tests=1, **ctx_opts(args))
-        cctx.tune(**ctx.settings())
-        cscene = cctx.scene(spec)
-        _, _, cinfo = cctx.render(cscene, p, want_accum=False, want_pixels=False)
-        n_box = cinfo["box_tests"] / max(1, cinfo["segments"])
-        n_tri = cinfo["tri_tests"] / max(1, cinfo["segments"])
-        cscene.free()
-        cctx.close()
-        b_seg = BOX_BYTES[layout] * n_box + 36.0 * n_tri + 128.0
-        per_launch_bytes = b_seg * segs / max(1, ext_launches)
+        # profiled pass: the same renders with HIP events around every extend launch (direct
+        # launches on the render stream)
+        pctx = illume_amd.Context(0, profile=1, **ctx_opts(args))
+        pctx.tune(**shape)
+        pscene = pctx.scene(spec)
+        pctx.render(pscene, p, want_accum=False, want_pixels=False)
+        ext_ms, ext_launches, psegs, pdev = 0.0, 0, 0, 0.0
+        for i in range(max(1, args.steps)):
+            _, _, info = pctx.render(pscene, p, want_accum=False, want_pixels=False)
+            ext_ms += info["extend_ms"]
+            ext_launches += info["extend_launches"]
+            psegs += info["segments"]
+            pdev += info["device_ms"]
+        pscene.free()
+        pctx.close()
+        n_box, n_tri = count_tests(spec, p, shape, args, "fp32", packets=False)
+        l_box, l_tri = count_tests(spec, p, shape, args, layout, packets=shape["camera_packets"] != 2)
+        b_seg = BOX_BYTES["fp32"] * n_box + 36.0 * n_tri + 128.0
+        b_lay = BOX_BYTES[layout] * l_box + 36.0 * l_tri + 128.0
         avg_launch_s = ext_ms / 1e3 / max(1, ext_launches)
+        per_launch_bytes = b_seg * psegs / max(1, ext_launches)
         achieved = per_launch_bytes / avg_launch_s / 1e9
+        ach_lay = b_lay * psegs / max(1, ext_launches) / avg_launch_s / 1e9
         traffic = traffic_from_profiles(args.config, layout)
+        lanes = profile_value(args.config, "active_lanes_extend")
         roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": "k_extend", "b_seg": round(b_seg, 1), "n_box": round(n_box, 2), "n_tri": round(n_tri, 2),
+                "kernel": "k_extend (every extend launch of the render)",
+                "convention": "B_seg = 28*N_box + 36*N_tri + 128; N_box, N_tri per closest-hit query of per-ray "
+                              "ordered t-pruned walks over the FP32 BVH2, counted on the same workload",
+                "b_seg": round(b_seg, 1), "n_box": round(n_box, 3), "n_tri": round(n_tri, 3),
                 "avg_launch_ms": round(avg_launch_s * 1e3, 4), "launches": ext_launches,
                 "algorithmic_bytes_per_launch": round(per_launch_bytes, 1),
-                "segments_per_s_at_roofline_M": round(HBM_PEAK_GBS * 1e9 / b_seg / 1e6, 1)}
-        log("roofline: N_box %.2f N_tri %.2f B_seg %.1f B, extend %.1f ms over %d launches -> %.1f GB/s"
-            % (n_box, n_tri, b_seg, ext_ms, ext_launches, achieved))
+                "segments_per_s_at_roofline_M": round(HBM_PEAK_GBS * 1e9 / b_seg / 1e6, 1),
+                "layout": {"bvh_layout": layout, "b_box": BOX_BYTES[layout], "n_box": round(l_box, 3),
+                           "n_tri": round(l_tri, 3), "b_seg": round(b_lay, 1), "achieved": round(ach_lay, 2),
+                           "frac": round(ach_lay / HBM_PEAK_GBS, 4),
+                           "camera_packets": shape["camera_packets"] != 2},
+                "profiled_pass": {"renders": max(1, args.steps), "device_ms_per_render": round(pdev / max(1, args.steps), 3),
+                                  "extend_ms_per_render": round(ext_ms / max(1, args.steps), 3),
+                                  "segments_per_render": psegs // max(1, args.steps)}}
+        if traffic is not None:
+            hbm = traffic / avg_launch_s / 1e9
+            roof["hbm_measured"] = round(hbm, 2)
+            roof["hbm_measured_frac"] = round(hbm / HBM_PEAK_GBS, 4)
+            roof["traffic_source"] = "profiles/extend_config%d_traffic.txt" % args.config
+        if lanes is not None:
+            roof["active_lanes"] = lanes
+            roof["active_lanes_source"] = "profiles/extend_config%d_counters.txt" % args.config
+        log("roofline: FP32 N_box %.2f N_tri %.2f B_seg %.1f B (layout %s: %.2f / %.2f, %.1f B), extend %.2f ms over "
+            "%d launches -> %.1f GB/s (%.3f of peak), layout bytes %.1f GB/s"
+            % (n_box, n_tri, b_seg, layout, l_box, l_tri, b_lay, ext_ms, ext_launches, achieved,
+               achieved / HBM_PEAK_GBS, ach_lay))
 
     cpu = None
     if not args.no_cpu:
         try:
             cpu = cpu_baseline(spec, st)
-            log("cpu baseline: %.2f Mrays/s on %d cores, %.3f on one" % (cpu["value"], cpu["cores"], cpu["value_1core"]))
+            log("cpu baseline: %.2f Mrays/s on %d cores, %.3f on one, %.2f with t-pruned traversal"
+                % (cpu["value"], cpu["cores"], cpu["value_1core"], cpu["value_tpruned"]))
         except Exception as e:  # the GPU number stands without it
             log("cpu baseline failed: %r" % (e,))
 
     config = {"workload": workload_name(args.config, spec, st), "width": st["width"], "height": st["height"],
               "spp": st["spp"], "max_depth": st["max_depth"], "bvh_layout": layout, "tri_test": args.tri,
               "segments_per_step": segs // max(1, args.steps), "parallelism": "1 GPU",
-              "launch_shape": ctx.settings(), "autotuned": tuned is not None}
+              "launch_shape": shape, "autotuned": tuned is not None, "graphs": True}
     out = base_line(args, METRIC, value, "Mrays/s", dev_ms / args.steps, config)
     out["latency_ms"] = {"p50": round(percentile(lat, 0.5), 3), "p99": round(percentile(lat, 0.99), 3),
                          "form": "plain (one request per ilm_render, image copied to the host)"}
@@ -461,6 +550,45 @@ def bench_served(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_obj_load(args):
+    """OBJ loader on the 12-instance scene's 722,000-triangle displaced icosphere (n=190, seed 5)
+    written as an OBJ by the generator: parse and vertex-merge seconds (host only; the reference's
+    reader is src/primitives/mesh.cu:159-340), and that the triangles read back bit for bit."""
+    import tempfile
+    import illume_amd
+    from illume_amd import scenes as S
+    m = S.displaced_icosphere(190, 5)
+    out = {}
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "mesh.obj")
+        t0 = time.time()
+        S.write_obj(m, path)
+        out["write_s"] = round(time.time() - t0, 3)
+        out["obj_bytes"] = os.path.getsize(path)
+        with open(os.path.join(d, "scene.json"), "w") as f:
+            json.dump({"meshes": [{"name": "m", "file": "mesh.obj"}], "camera": {"fov": 40}}, f)
+        times = []
+        for _ in range(max(1, args.steps)):
+            t0 = time.perf_counter()
+            js = illume_amd.load_json(os.path.join(d, "scene.json"))
+            wall = time.perf_counter() - t0
+            parse_s, merge_s = js.load_times()
+            times.append((wall, parse_s, merge_s))
+            dm = js.desc.meshes[0]
+            pos = np.ctypeslib.as_array(dm.positions, shape=(dm.num_vertices * 3,)).reshape(-1, 3)
+            idx = np.ctypeslib.as_array(dm.indices, shape=(dm.num_triangles * 3,)).reshape(-1, 3)
+            exact = bool(np.array_equal(pos[idx], m["positions"][m["indices"]]))
+            js.free()
+    best = min(times)
+    model, ncpu = host_cpu()
+    line = {"metric": "OBJ load seconds (722,000 triangles)", "value": round(best[0], 3), "unit": "s",
+            "higher_is_better": False, "parse_s": round(best[1], 3), "merge_s": round(best[2], 3),
+            "triangles": int(len(m["indices"])), "vertices": int(len(m["positions"])),
+            "triangles_bit_exact": exact, "runs": len(times), "cpu_model": model, "threads": 1}
+    line.update(out)
+    print(json.dumps(line), flush=True)
+
+
 def calibrate_copy():
     """A plain device copy of known size (1 GiB read + 1 GiB written, three times) for scaling
     the FETCH_SIZE / WRITE_SIZE counters of rocprofv3 --pmc to bytes."""
@@ -499,9 +627,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-count", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true", help="only run the 1 GiB copy used to scale HBM counters")
+    ap.add_argument("--obj-load", action="store_true", help="only time the OBJ loader on the 722k-triangle mesh (host)")
     args = ap.parse_args()
     if args.calibrate_copy:
         return calibrate_copy()
+    if args.obj_load:
+        return bench_obj_load(args)
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (the path shards by request or sample range; "
               "multi-GPU runs are not part of it). Use --gpus 1.", file=sys.stderr)

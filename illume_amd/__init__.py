@@ -50,6 +50,7 @@ def lib():
         "ilm_desc_scene": (P(SceneDesc), [vp]),
         "ilm_desc_settings": (C.c_int, [vp, P(RenderParams)]),
         "ilm_desc_free": (None, [vp]),
+        "ilm_desc_load_times": (C.c_int, [vp, P(C.c_double), P(C.c_double)]),
         "ilm_loader_error": (C.c_char_p, []),
         "ilm_render": (C.c_int, [vp, vp, P(RenderParams), P(C.c_float), P(Pixel), P(RenderInfo)]),
         "ilm_render_batch": (C.c_int, [vp, C.c_int32, P(vp), P(RenderParams), P(P(C.c_float)), P(P(Pixel)), P(RenderInfo)]),
@@ -373,6 +374,12 @@ class JsonScene:
 
     def upload(self, ctx):
         return Scene(ctx, desc=self.desc)
+
+    def load_times(self):
+        """(parse seconds, merge seconds) the load spent on OBJ files."""
+        a, b = C.c_double(), C.c_double()
+        _check(lib().ilm_desc_load_times(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
 
     def free(self):
         if self.h:

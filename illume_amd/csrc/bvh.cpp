@@ -161,6 +161,38 @@ Tree build_tree(const std::vector<Box>& prim_boxes, const BuildOpts& opts) {
     return t;
 }
 
+/* Moller-Trumbore is not watertight: near an edge it can report a hit slightly outside the
+ * triangle (its barycentrics carry rounding of about 1e-7 of the distance from the ray origin to
+ * the triangle). With boxes tight on the vertices, such a hit is found or not depending on which
+ * boxes a walk enters, so two node layouts, or a packet walk and a lone ray, could settle on
+ * different triangles at nearly the same distance. Padding every triangle box by 2^-20 of the
+ * mesh extent keeps those hits inside every box on their way for rays that start within a few
+ * mesh extents of the triangle, so every walk over either layout finds them. */
+std::vector<Box> triangle_boxes(const float* pos, const int* idx, int num_triangles) {
+    std::vector<Box> pb(num_triangles > 0 ? num_triangles : 0);
+    Box all;
+    all.reset();
+    for (int t = 0; t < num_triangles; t++) {
+        pb[t].reset();
+        for (int j = 0; j < 3; j++) pb[t].grow(&pos[3 * idx[3 * t + j]]);
+        all.grow(pb[t]);
+    }
+    if (num_triangles == 0) return pb;
+    double ext = 0.0;
+    for (int a = 0; a < 3; a++) ext = std::max(ext, (double)all.hi[a] - (double)all.lo[a]);
+    const double pad = ext / 1048576.0;
+    if (!(pad > 0.0)) return pb;
+    for (Box& b : pb)
+        for (int a = 0; a < 3; a++) {
+            float lo = (float)((double)b.lo[a] - pad), hi = (float)((double)b.hi[a] + pad);
+            if ((double)lo > (double)b.lo[a] - pad) lo = std::nextafter(lo, -INFINITY);
+            if ((double)hi < (double)b.hi[a] + pad) hi = std::nextafter(hi, INFINITY);
+            b.lo[a] = lo;
+            b.hi[a] = hi;
+        }
+    return pb;
+}
+
 static int child_code(const Tree& t, const std::vector<int>& inner_id, int c) {
     const Node2& n = t.nodes[c];
     if (n.leaf()) return ilm_leaf_code((uint32_t)n.first, (uint32_t)std::max(1, n.count));
@@ -211,12 +243,32 @@ int qnodes_max_exponent(const std::vector<QNode>& nodes) {
     return e - 127;
 }
 
+/* Outward pad of every quantised child box, as a fraction of its node's extent per axis. The
+ * traversal forms a plane's slab distance from B = (o_n - o)/d, whose rounding leaves an
+ * absolute error of at most 2^-23 * 255 * s / |d| (s the axis quantum, 255 s <= 2.03 x the
+ * node extent); moving every plane out by 2^-20 x the node extent widens each slab interval by
+ * four times that, so the rounded test still contains the exact box whatever the ray origin. */
+#define QPAD_REL (1.0 / 1048576.0)
+
 static void quantize_node(QNode& q, const Box& nb, const Box* cb, int nc) {
-    const float org[3] = {nb.lo[0], nb.lo[1], nb.lo[2]};
+    /* per axis: the pad, the padded child bounds (double) and an origin below the padded node */
+    double pad[3], plo[4][3], phi[4][3];
+    float org[3];
+    for (int a = 0; a < 3; a++) {
+        const double ext = (double)nb.hi[a] - (double)nb.lo[a];
+        pad[a] = ext > 0.0 ? ext * QPAD_REL : 0.0;
+        for (int k = 0; k < nc; k++) {
+            plo[k][a] = (double)cb[k].lo[a] - pad[a];
+            phi[k][a] = (double)cb[k].hi[a] + pad[a];
+        }
+        float o = (float)((double)nb.lo[a] - pad[a]);
+        if ((double)o > (double)nb.lo[a] - pad[a]) o = std::nextafter(o, -INFINITY);
+        org[a] = o;
+    }
     q.ox = org[0]; q.oy = org[1]; q.oz = org[2];
     uint8_t* ex[3] = {&q.ex, &q.ey, &q.ez};
     for (int a = 0; a < 3; a++) {
-        double ext = (double)nb.hi[a] - (double)nb.lo[a];
+        double ext = (double)nb.hi[a] + pad[a] - (double)org[a];
         int e;
         if (!(ext > 0.0)) e = -126;
         else {
@@ -228,13 +280,15 @@ static void quantize_node(QNode& q, const Box& nb, const Box* cb, int nc) {
         for (;;) {
             bool ok = true;
             for (int k = 0; k < nc && ok; k++) {
-                double fl = std::floor(((double)cb[k].lo[a] - org[a]) / s);
-                double ch = std::ceil(((double)cb[k].hi[a] - org[a]) / s);
+                double fl = std::floor((plo[k][a] - org[a]) / s);
+                double ch = std::ceil((phi[k][a] - org[a]) / s);
                 int ql = (int)std::max(0.0, std::min(255.0, fl));
                 int qh = (int)std::max(0.0, std::min(255.0, ch));
-                while (ql > 0 && q_deq(org[a], ql, s) > cb[k].lo[a]) ql--;
-                while (qh < 255 && q_deq(org[a], qh, s) < cb[k].hi[a]) qh++;
-                if (q_deq(org[a], ql, s) > cb[k].lo[a] || q_deq(org[a], qh, s) < cb[k].hi[a]) { ok = false; break; }
+                /* the float plane the kernel forms (org + q*s, q*s exact) must lie outside the
+                 * padded bound */
+                while (ql > 0 && (double)q_deq(org[a], ql, s) > plo[k][a]) ql--;
+                while (qh < 255 && (double)q_deq(org[a], qh, s) < phi[k][a]) qh++;
+                if ((double)q_deq(org[a], ql, s) > plo[k][a] || (double)q_deq(org[a], qh, s) < phi[k][a]) { ok = false; break; }
                 q.qlo[a][k] = (uint8_t)ql;
                 q.qhi[a][k] = (uint8_t)qh;
             }

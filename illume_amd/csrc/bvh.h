@@ -50,6 +50,10 @@ struct BuildOpts {
 /* prim_boxes: one box per primitive */
 Tree build_tree(const std::vector<Box>& prim_boxes, const BuildOpts& opts);
 
+/* One box per triangle for build_tree: the vertices' bounds padded outward by 2^-20 of the
+ * mesh's largest extent (see bvh.cpp). */
+std::vector<Box> triangle_boxes(const float* pos, const int* idx, int num_triangles);
+
 /* Device layouts */
 struct Layout2 {
     std::vector<BNode> nodes;

@@ -21,7 +21,7 @@ namespace ilm {
 namespace {
 
 const char kMagic[8] = {'I', 'L', 'M', 'B', 'V', 'H', 0, 1};
-const uint32_t kVersion = 1;
+const uint32_t kVersion = 2;   /* 2: padded triangle boxes and 8-bit child boxes (bvh.cpp) */
 
 struct CacheHeader {
     char magic[8];
@@ -185,12 +185,7 @@ uint64_t mesh_key(const float* pos, int num_vertices, const int* idx, int num_tr
 }
 
 MeshBVH build_mesh_bvh(const float* pos, const int* idx, int num_triangles, const BuildOpts& o) {
-    std::vector<Box> pb(num_triangles);
-    for (int t = 0; t < num_triangles; t++) {
-        pb[t].reset();
-        for (int j = 0; j < 3; j++) pb[t].grow(&pos[3 * idx[3 * t + j]]);
-    }
-    Tree tree = build_tree(pb, o);
+    Tree tree = build_tree(triangle_boxes(pos, idx, num_triangles), o);
     Layout2 L2 = make_layout2(tree);
     LayoutQ LQ = make_layout_q(tree);
     MeshBVH b;

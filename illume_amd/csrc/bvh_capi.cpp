@@ -20,14 +20,9 @@ extern "C" int ilm_bvh_build(const ilm_mesh_desc* m, ilm_bvh** out) {
     if (!m || !out || m->num_triangles < 0 || (m->num_triangles > 0 && (!m->positions || !m->indices))) return ILM_E_INVALID;
     for (int64_t k = 0; k < 3LL * m->num_triangles; k++)
         if (m->indices[k] < 0 || m->indices[k] >= m->num_vertices) return ILM_E_INVALID;
-    std::vector<Box> pb(m->num_triangles);
-    for (int t = 0; t < m->num_triangles; t++) {
-        pb[t].reset();
-        for (int j = 0; j < 3; j++) pb[t].grow(&m->positions[3 * m->indices[3 * t + j]]);
-    }
     ilm_bvh* b = new ilm_bvh();
     BuildOpts o;
-    b->tree = build_tree(pb, o);
+    b->tree = build_tree(triangle_boxes(m->positions, m->indices, m->num_triangles), o);
     b->q = make_layout_q(b->tree);
     *out = b;
     return ILM_OK;

@@ -95,6 +95,21 @@ struct ProfileSink {
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
 };
 
+/* Device buffers one chunk pipeline runs on (the context's own, allocated at ilm_open). */
+struct Pipe {
+    DPaths A, B;                    /* path state, ping-ponged across depths */
+    DHits hits;
+    float4* contrib = nullptr;      /* per-path radiance */
+    DCtl* ctl = nullptr;
+    DChunk* chunk = nullptr;        /* device copy of the chunk descriptor */
+    int* ovf = nullptr;             /* traversal-stack spill area, sized for ext_grid */
+    int ext_grid = 0;
+    unsigned long long* seg_total = nullptr;
+};
+
+/* Latencies of the most recent completed requests (p50/p99 are taken over this window). */
+#define ILM_LATENCY_WINDOW 4096
+
 struct ilm_ctx {
     /* held by the caller's handle and by every live scene and request handle: the context's
      * device resources are released when the last of them goes */
@@ -143,15 +158,22 @@ struct ilm_ctx {
     int64_t in_use = 0;
     int64_t chunk_seq = 0;
     int64_t chunks_retired = 0;
-    std::vector<double> latencies;
-    uint64_t stat_segments = 0, stat_done = 0, stat_rejected = 0, stat_cancelled = 0;
+    std::vector<double> latencies;  /* ring of ILM_LATENCY_WINDOW entries */
+    size_t lat_next = 0;
+    uint64_t stat_done = 0, stat_rejected = 0, stat_cancelled = 0;
+    /* segment counter as of the last completed chunk or render (stat_mu); read by ilm_stats_get
+     * without touching the device. h_seg_ring: pinned slots the device counter is copied into
+     * behind each served chunk, read when that chunk's event completes. */
+    uint64_t stat_segments = 0;
+    unsigned long long* h_seg_ring = nullptr;   /* ILM_RING slots */
+    int seg_ring_pos = 0;
     uint64_t stat_bvh_hits = 0, stat_bvh_misses = 0;
     std::string bvh_cache_dir;      /* empty: no BVH cache */
     double stat_busy_ms = 0.0;
 
     /* Lock order: sched_mu before dev_mu; stat_mu is a leaf (no other lock is taken while it
      * is held). Nothing takes sched_mu while holding dev_mu. */
-    std::mutex stat_mu;             /* stat_busy_ms, stat_bvh_* */
+    std::mutex stat_mu;             /* stat_busy_ms, stat_bvh_*, stat_segments */
     std::mutex err_mu;
     std::string err, err_copy;
 };

@@ -430,7 +430,7 @@ static int ctx_alloc_paths(ilm_ctx* ctx, int64_t cap) {
 static bool opts_valid(const ilm_opts& o) {
     const bool sg_ok = o.sample_group >= 0 && o.sample_group <= 4096 && (o.sample_group & (o.sample_group - 1)) == 0;
     return !(o.bvh_layout < 0 || o.bvh_layout > ILM_BVH_QUANT8 || o.tri_test < 0 || o.tri_test > ILM_TRI_WATERTIGHT ||
-             o.extend_mode < 0 || o.extend_mode > ILM_EXTEND_STEP_REFILL || !sg_ok || o.tile_rows < 0 || o.tile_rows > 256 ||
+             o.extend_mode < 0 || o.extend_mode > ILM_EXTEND_SPECULATIVE || !sg_ok || o.tile_rows < 0 || o.tile_rows > 256 ||
              o.refill_min < 0 || o.refill_min > 64 || o.ext_fetch < 0 || o.ext_fetch > 65536 || (o.ext_fetch & 63) != 0 ||
              o.leaf_wait_pct < 0 || o.leaf_wait_pct > 10000 || o.shade_sort < 0 || o.shade_sort > 2 ||
              o.sched_fair < 0 || o.sched_fair > 1 || o.camera_packets < 0 || o.camera_packets > 2 ||
@@ -502,6 +502,7 @@ int ilm_open(int device, const ilm_opts* opts, ilm_ctx** out) {
         HIPTRY(ctx, hipEventCreateWithFlags(&ctx->ring_ev[i], hipEventDisableTiming));
     }
     HIPTRY(ctx, hipHostMalloc((void**)&ctx->h_u64, 4 * sizeof(unsigned long long), hipHostMallocDefault));
+    HIPTRY(ctx, hipHostMalloc((void**)&ctx->h_seg_ring, ILM_RING * sizeof(unsigned long long), hipHostMallocDefault));
     size_t free_b = 0, total_b = 0;
     HIPTRY(ctx, hipMemGetInfo(&free_b, &total_b));
     ctx->budget = ctx->opts.hbm_budget_bytes > 0 ? ctx->opts.hbm_budget_bytes : (int64_t)(0.9 * (double)free_b);
@@ -526,6 +527,7 @@ static void ctx_destroy(ilm_ctx* ctx) {
         if (ctx->ring_ev[i]) (void)hipEventDestroy(ctx->ring_ev[i]);
     }
     if (ctx->h_u64) (void)hipHostFree(ctx->h_u64);
+    if (ctx->h_seg_ring) (void)hipHostFree(ctx->h_seg_ring);
     (void)hipStreamDestroy(ctx->stream);
     (void)hipGetLastError();
     delete ctx;
@@ -624,15 +626,30 @@ static bool jobs_need_ovf(const ilm_ctx* ctx, const std::vector<Job*>& jobs) {
     return false;
 }
 
-static KParams base_params(ilm_ctx* ctx) {
+/* The context's own pipeline buffers. */
+static Pipe ctx_pipe(const ilm_ctx* ctx) {
+    Pipe q;
+    q.A = ctx->A;
+    q.B = ctx->B;
+    q.hits = ctx->hits;
+    q.contrib = ctx->contrib;
+    q.ctl = ctx->ctl;
+    q.chunk = ctx->d_chunk;
+    q.ovf = ctx->ovf;
+    q.ext_grid = ctx->ext_grid;
+    q.seg_total = ctx->d_seg_total;
+    return q;
+}
+
+static KParams base_params(ilm_ctx* ctx, const Pipe& q) {
     KParams P;
     std::memset(&P, 0, sizeof(P));
-    P.chunk = ctx->d_chunk;
-    P.ctl = ctx->ctl;
-    P.in = ctx->A;
-    P.out = ctx->B;
-    P.hits = ctx->hits;
-    P.contrib = ctx->contrib;
+    P.chunk = q.chunk;
+    P.ctl = q.ctl;
+    P.in = q.A;
+    P.out = q.B;
+    P.hits = q.hits;
+    P.contrib = q.contrib;
     P.bvh_layout = ctx->opts.bvh_layout;
     P.tri_test = ctx->opts.tri_test;
     P.extend_mode = ext_mode_of(ctx->opts);
@@ -645,65 +662,72 @@ static KParams base_params(ilm_ctx* ctx) {
     P.trav_stats = ctx->opts.count_tests ? ctx->d_trav : nullptr;
     return P;
 }
+static KParams base_params(ilm_ctx* ctx) { return base_params(ctx, ctx_pipe(ctx)); }
 
-/* Stage a chunk descriptor: pinned ring slot -> the fixed device DChunk (stream ordered). */
-static int stage_chunk(ilm_ctx* ctx, const DChunk& hc) {
+/* Stage a chunk descriptor: pinned ring slot -> the device DChunk dst (stream ordered). */
+static int stage_chunk(ilm_ctx* ctx, const DChunk& hc, DChunk* dst) {
     int slot = ctx->ring_pos++ % ILM_RING;
     HIPTRY(ctx, hipEventSynchronize(ctx->ring_ev[slot]));
     std::memcpy(ctx->h_chunk[slot], &hc, sizeof(DChunk));
-    HIPTRY(ctx, hipMemcpyAsync(ctx->d_chunk, ctx->h_chunk[slot], sizeof(DChunk), hipMemcpyHostToDevice, ctx->stream));
+    HIPTRY(ctx, hipMemcpyAsync(dst, ctx->h_chunk[slot], sizeof(DChunk), hipMemcpyHostToDevice, ctx->stream));
     HIPTRY(ctx, hipEventRecord(ctx->ring_ev[slot], ctx->stream));
     return ILM_OK;
 }
+static int stage_chunk(ilm_ctx* ctx, const DChunk& hc) { return stage_chunk(ctx, hc, ctx->d_chunk); }
 
 /* A chunk's first launch after k_ctl_init. With per-step refill the first depth's extend and
  * shade form the camera rays where they use them, so only the per-path results are cleared
  * (by the packet extend itself when the first depth runs as packets); the other extend modes
  * read the rays k_raygen writes out. */
-static hipError_t start_paths(ilm_ctx* ctx, KParams& P, hipStream_t s) {
-    P.in = ctx->A;
-    P.cam0 = P.extend_mode == ILM_EXTEND_STEP_REFILL ? 1 : 0;
+static hipError_t start_paths(ilm_ctx* ctx, const Pipe& q, KParams& P, hipStream_t s) {
+    P.in = q.A;
+    P.cam0 = (P.extend_mode == ILM_EXTEND_STEP_REFILL || P.extend_mode == ILM_EXTEND_SPECULATIVE) ? 1 : 0;
     if (P.cam0 && P.packet0) return hipSuccess;
     return P.cam0 ? ilm_k_path_clear(P, ctx->gen_grid, s) : ilm_k_raygen(P, ctx->gen_grid, s);
 }
 
 /* The launch sequence of one chunk with depth_steps depth iterations. */
-static int enqueue_pipeline(ilm_ctx* ctx, const KParams& base, int depth_steps, ProfileSink* prof) {
+static int enqueue_pipeline(ilm_ctx* ctx, const Pipe& q, const KParams& base, int depth_steps, ProfileSink* prof) {
     hipStream_t s = ctx->stream;
     KParams P = base;
     HIPTRY(ctx, ilm_k_ctl_init(P, s));
-    HIPTRY(ctx, start_paths(ctx, P, s));
+    HIPTRY(ctx, start_paths(ctx, q, P, s));
     for (int d = 0; d < depth_steps; d++) {
-        P.in = (d & 1) ? ctx->B : ctx->A;
-        P.out = (d & 1) ? ctx->A : ctx->B;
+        P.in = (d & 1) ? q.B : q.A;
+        P.out = (d & 1) ? q.A : q.B;
         if (prof) {
             hipEvent_t a, b;
             HIPTRY(ctx, hipEventCreate(&a));
             HIPTRY(ctx, hipEventCreate(&b));
             HIPTRY(ctx, hipEventRecord(a, s));
-            HIPTRY(ctx, ilm_k_extend(P, d, ctx->ovf, ctx->ext_grid, s));
+            HIPTRY(ctx, ilm_k_extend(P, d, q.ovf, q.ext_grid, s));
             HIPTRY(ctx, hipEventRecord(b, s));
             prof->ev.push_back({a, b});
         } else {
-            HIPTRY(ctx, ilm_k_extend(P, d, ctx->ovf, ctx->ext_grid, s));
+            HIPTRY(ctx, ilm_k_extend(P, d, q.ovf, q.ext_grid, s));
         }
         HIPTRY(ctx, ilm_k_shade(P, d, ctx->shade_grid, s));
     }
     HIPTRY(ctx, ilm_k_accumulate(P, ctx->gen_grid, s));
-    HIPTRY(ctx, ilm_k_chunk_end(P, ctx->d_seg_total, s));
+    HIPTRY(ctx, ilm_k_chunk_end(P, q.seg_total, s));
     return ILM_OK;
+}
+
+/* whether every request of the chunk renders the same scene (the extend kernel then reads it
+ * once from the chunk descriptor, so a graph captured for it serves any later single-scene chunk) */
+static bool chunk_uni_scene(const DChunk& hc) {
+    bool uni = hc.n_work > 0;
+    for (int k = 1; k < hc.n_work && uni; k++) uni = hc.reqs[hc.work[k].req].scene == hc.reqs[hc.work[0].req].scene;
+    return uni && hc.work[0].req == 0;
 }
 
 static int launch_chunk(ilm_ctx* ctx, const DChunk& hc, bool ovf, ProfileSink* prof) {
     int rc = stage_chunk(ctx, hc);
     if (rc) return rc;
-    KParams P = base_params(ctx);
+    const Pipe q = ctx_pipe(ctx);
+    KParams P = base_params(ctx, q);
     P.use_ovf = ovf ? 1 : 0;
-    /* one scene for the whole chunk: the extend kernel reads it once from the chunk descriptor
-     * (so a graph captured for it serves any later single-scene chunk) */
-    bool uni = hc.n_work > 0;
-    for (int k = 1; k < hc.n_work && uni; k++) uni = hc.reqs[hc.work[k].req].scene == hc.reqs[hc.work[0].req].scene;
-    uni = uni && hc.work[0].req == 0;
+    const bool uni = chunk_uni_scene(hc);
     P.uni_scene = uni ? 1 : 0;
     const int D = hc.depth_steps;
     const bool graphs = ctx->opts.use_graphs && !prof && !ctx->opts.count_tests;
@@ -714,7 +738,7 @@ static int launch_chunk(ilm_ctx* ctx, const DChunk& hc, bool ovf, ProfileSink* p
             hipGraph_t g = nullptr;
             hipGraphExec_t ge = nullptr;
             HIPTRY(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-            int rc2 = enqueue_pipeline(ctx, P, D, nullptr);
+            int rc2 = enqueue_pipeline(ctx, q, P, D, nullptr);
             hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
             if (rc2) return rc2;
             if (ec != hipSuccess) return fail(ctx, ILM_E_DEVICE, std::string("graph capture: ") + hipGetErrorString(ec));
@@ -725,10 +749,30 @@ static int launch_chunk(ilm_ctx* ctx, const DChunk& hc, bool ovf, ProfileSink* p
         HIPTRY(ctx, hipGraphLaunch(it->second, ctx->stream));
         return ILM_OK;
     }
-    return enqueue_pipeline(ctx, P, D, prof);
+    return enqueue_pipeline(ctx, q, P, D, prof);
 }
 
 /* ======================= jobs ======================= */
+
+/* The request's device descriptor, over its Kahan buffer d_accum (2*W*H float4: sums, then
+ * compensations); the issue cursor starts at pass 0. */
+static void job_fill_req(Job* j) {
+    const ilm_render_params& p = j->p;
+    const int64_t wh = (int64_t)p.width * p.height;
+    std::memset(&j->dreq, 0, sizeof(j->dreq));
+    j->dreq.scene = j->scene->d_scene;
+    j->dreq.width = p.width;
+    j->dreq.height = p.height;
+    j->dreq.max_depth = p.max_depth;
+    j->dreq.ray_bias = p.ray_bias;
+    j->dreq.tonemap = p.tonemap;
+    j->dreq.spp = p.spp;
+    hm::render_info(j->scene->camera, p.width, p.height, j->dreq.image_width, j->dreq.pixel_size, j->dreq.left, j->dreq.top);
+    j->dreq.accum = j->d_accum;
+    j->dreq.comp = j->d_accum + wh;
+    j->next_pass = 0;
+    j->next_px = 0;
+}
 
 static int job_prepare(ilm_ctx* ctx, Job* j) {
     const ilm_render_params& p = j->p;
@@ -746,19 +790,7 @@ static int job_prepare(ilm_ctx* ctx, Job* j) {
         j->px_cap = pb;
     }
     HIPTRY(ctx, ilm_k_fill_zero4(j->d_accum, 2 * wh, ctx->stream));
-    std::memset(&j->dreq, 0, sizeof(j->dreq));
-    j->dreq.scene = j->scene->d_scene;
-    j->dreq.width = p.width;
-    j->dreq.height = p.height;
-    j->dreq.max_depth = p.max_depth;
-    j->dreq.ray_bias = p.ray_bias;
-    j->dreq.tonemap = p.tonemap;
-    j->dreq.spp = p.spp;
-    hm::render_info(j->scene->camera, p.width, p.height, j->dreq.image_width, j->dreq.pixel_size, j->dreq.left, j->dreq.top);
-    j->dreq.accum = j->d_accum;
-    j->dreq.comp = j->d_accum + wh;
-    j->next_pass = 0;
-    j->next_px = 0;
+    job_fill_req(j);
     return ILM_OK;
 }
 
@@ -811,11 +843,55 @@ static void req_divs(DReq& R, int th) {
     R.th_div = udiv_make(t);
 }
 
-static void build_chunk(ilm_ctx* ctx, const std::vector<Job*>& jobs, DChunk& c, std::vector<Job*>& in_chunk) {
+static void chunk_reset(DChunk& c) {
     c.total_paths = 0;
     c.total_px = 0;
     c.n_work = 0;
     c.depth_steps = 0;
+}
+
+/* Append job j's next slice, at most lim paths, as request nreq of chunk c: all pixels x as
+ * many whole passes as fit, or a pixel range of one pass. Advances j's issue cursor. */
+static void add_slice(const ilm_ctx* ctx, Job* j, int64_t lim, int nreq, DChunk& c) {
+    const int64_t wh = (int64_t)j->p.width * j->p.height;
+    DWork w;
+    std::memset(&w, 0, sizeof(w));
+    w.req = nreq;
+    w.pass0 = j->next_pass;
+    if (j->next_px == 0 && wh <= lim) {
+        int64_t np = std::min<int64_t>(j->p.spp - j->next_pass, lim / wh);
+        if (ctx->opts.chunk_passes > 0) np = std::min<int64_t>(np, ctx->opts.chunk_passes);
+        w.npass = (int)np;
+        w.px0 = 0;
+        w.npx = (uint32_t)wh;
+        j->next_pass += (int)np;
+    } else {
+        int64_t npx = std::min<int64_t>(wh - j->next_px, lim);
+        w.npass = 1;
+        w.px0 = (uint32_t)j->next_px;
+        w.npx = (uint32_t)npx;
+        j->next_px += npx;
+        if (j->next_px >= wh) { j->next_px = 0; j->next_pass += 1; }
+    }
+    w.sgroup = 1;
+    if (w.px0 == 0) {
+        const uint32_t sg_max = ctx->opts.sample_group > 0 ? (uint32_t)ctx->opts.sample_group : 64u;
+        while (w.sgroup * 2 <= sg_max && w.npass % (w.sgroup * 2) == 0) w.sgroup *= 2;
+    }
+    w.sg_log2 = (uint32_t)__builtin_ctz(w.sgroup);
+    w.span_div = udiv_make(w.npx * w.sgroup);
+    w.pid_base = c.total_paths;
+    w.acc_base = c.total_px;
+    c.total_paths += (uint32_t)((int64_t)w.npass * w.npx);
+    c.total_px += w.npx;
+    c.reqs[nreq] = j->dreq;
+    req_divs(c.reqs[nreq], ctx->opts.tile_rows > 0 ? ctx->opts.tile_rows : 4);
+    c.work[c.n_work++] = w;
+    c.depth_steps = std::max(c.depth_steps, j->p.max_depth);
+}
+
+static void build_chunk(ilm_ctx* ctx, const std::vector<Job*>& jobs, DChunk& c, std::vector<Job*>& in_chunk) {
+    chunk_reset(c);
     in_chunk.clear();
     int64_t left = ctx->cap;
     int nreq = 0;
@@ -825,43 +901,10 @@ static void build_chunk(ilm_ctx* ctx, const std::vector<Job*>& jobs, DChunk& c, 
         Job* j = jobs[jk];
         if (!job_has_work(j) || j->cancel.load()) continue;
         if (nreq >= ILM_MAX_JOBS || c.n_work >= ILM_MAX_WORK || left <= 0) break;
-        const int64_t wh = (int64_t)j->p.width * j->p.height;
         const int64_t lim = std::min(left, std::max<int64_t>(quota[jk], 1));
-        DWork w;
-        std::memset(&w, 0, sizeof(w));
-        w.req = nreq;
-        w.pass0 = j->next_pass;
-        if (j->next_px == 0 && wh <= lim) {
-            int64_t np = std::min<int64_t>(j->p.spp - j->next_pass, lim / wh);
-            if (ctx->opts.chunk_passes > 0) np = std::min<int64_t>(np, ctx->opts.chunk_passes);
-            w.npass = (int)np;
-            w.px0 = 0;
-            w.npx = (uint32_t)wh;
-            j->next_pass += (int)np;
-        } else {
-            int64_t npx = std::min<int64_t>(wh - j->next_px, lim);
-            w.npass = 1;
-            w.px0 = (uint32_t)j->next_px;
-            w.npx = (uint32_t)npx;
-            j->next_px += npx;
-            if (j->next_px >= wh) { j->next_px = 0; j->next_pass += 1; }
-        }
-        w.sgroup = 1;
-        if (w.px0 == 0) {
-            const uint32_t sg_max = ctx->opts.sample_group > 0 ? (uint32_t)ctx->opts.sample_group : 64u;
-            while (w.sgroup * 2 <= sg_max && w.npass % (w.sgroup * 2) == 0) w.sgroup *= 2;
-        }
-        w.sg_log2 = (uint32_t)__builtin_ctz(w.sgroup);
-        w.span_div = udiv_make(w.npx * w.sgroup);
-        w.pid_base = c.total_paths;
-        w.acc_base = c.total_px;
-        c.total_paths += (uint32_t)((int64_t)w.npass * w.npx);
-        c.total_px += w.npx;
-        left -= (int64_t)w.npass * w.npx;
-        c.reqs[nreq] = j->dreq;
-        req_divs(c.reqs[nreq], ctx->opts.tile_rows > 0 ? ctx->opts.tile_rows : 4);
-        c.work[c.n_work++] = w;
-        c.depth_steps = std::max(c.depth_steps, j->p.max_depth);
+        const uint32_t before = c.total_paths;
+        add_slice(ctx, j, lim, nreq, c);
+        left -= (int64_t)(c.total_paths - before);
         nreq++;
         in_chunk.push_back(j);
     }
@@ -961,6 +1004,7 @@ static int run_jobs_sync(ilm_ctx* ctx, std::vector<Job*>& jobs, ilm_render_info*
     {
         std::lock_guard<std::mutex> lk(ctx->stat_mu);   /* dev_mu is held: never sched_mu here */
         ctx->stat_busy_ms += ms;
+        ctx->stat_segments += segs;
     }
     return ILM_OK;
 }
@@ -1124,7 +1168,7 @@ int ilm_debug_path_radiance(ilm_ctx* ctx, ilm_scene* scene, const ilm_render_par
         P.use_ovf = scene_needs_ovf(ctx, scene) ? 1 : 0;
         P.list = d_list;
         P.seg_count = ctx->seg_count;
-        if (ilm_k_ctl_init(P, ctx->stream) != hipSuccess || start_paths(ctx, P, ctx->stream) != hipSuccess) {
+        if (ilm_k_ctl_init(P, ctx->stream) != hipSuccess || start_paths(ctx, ctx_pipe(ctx), P, ctx->stream) != hipSuccess) {
             rc = fail(ctx, ILM_E_DEVICE, "raygen launch");
             break;
         }
@@ -1162,11 +1206,45 @@ int ilm_debug_path_radiance(ilm_ctx* ctx, ilm_scene* scene, const ilm_render_par
 static void admit_select_locked(ilm_ctx* ctx, std::vector<Job*>& picked);
 static void admit_prepare(ilm_ctx* ctx, std::vector<Job*>& picked);
 
+static void latency_record_locked(ilm_ctx* ctx, double ms) {
+    if (ctx->latencies.size() < ILM_LATENCY_WINDOW) {
+        ctx->latencies.push_back(ms);
+    } else {
+        ctx->latencies[ctx->lat_next] = ms;
+        ctx->lat_next = (ctx->lat_next + 1) % ILM_LATENCY_WINDOW;
+    }
+}
+
+/* Ends a job the scheduler holds (finished, failed or cancelled with nothing in flight). Its
+ * device buffers go back first, outside sched_mu (a hipFree, when the pool is full, waits for the
+ * device); then, under sched_mu, it leaves the active set, its state changes and the scheduler's
+ * reference drops. The caller holds no lock. */
+static void sched_finish(ilm_ctx* ctx, Job* j, int state, double t) {
+    job_release_device(j);
+    std::lock_guard<std::mutex> lk(ctx->sched_mu);
+    ctx->in_use -= j->bytes;
+    auto it = std::find(ctx->active.begin(), ctx->active.end(), j);
+    if (it != ctx->active.end()) ctx->active.erase(it);
+    j->end_ms = t;
+    if (state == ILM_REQ_CANCELLED) {
+        ctx->stat_cancelled++;
+    } else {
+        latency_record_locked(ctx, t - j->submit_ms);
+        ctx->stat_done++;
+        j->samples_done = j->p.spp;
+    }
+    j->state.store(state);
+    j->cv.notify_all();
+    if (--j->refs == 0) { scene_release(j->scene, true); delete j; }
+}
+
 static void server_loop(ilm_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     std::unique_ptr<DChunk> hc(new DChunk());
     std::vector<Job*> in_chunk;
-    struct Inflight { hipEvent_t ev; std::vector<Job*> finishing; };
+    /* a chunk in flight: its completion event, the jobs it finishes and the pinned slot its
+     * segment count is copied into */
+    struct Inflight { hipEvent_t ev; std::vector<Job*> finishing; int seg_slot; };
     std::deque<Inflight> inflight;
     for (;;) {
         std::vector<Job*> active, picked;
@@ -1184,7 +1262,6 @@ static void server_loop(ilm_ctx* ctx) {
             std::lock_guard<std::mutex> lk(ctx->sched_mu);
             active = ctx->active;
         }
-        /* retire cancelled jobs that have nothing in flight */
         bool launched = false;
         {
             std::lock_guard<std::mutex> dev(ctx->dev_mu);
@@ -1196,6 +1273,10 @@ static void server_loop(ilm_ctx* ctx) {
                 for (int w = 0; w < hc->n_work; w++) hc->work[w].pass0 += in_chunk[hc->work[w].req]->p.first_sample;
                 int rc = launch_chunk(ctx, *hc, jobs_need_ovf(ctx, in_chunk), nullptr);
                 Inflight f;
+                f.seg_slot = ctx->seg_ring_pos++ % ILM_RING;   /* at most two chunks in flight */
+                ctx->h_seg_ring[f.seg_slot] = 0;
+                (void)hipMemcpyAsync(ctx->h_seg_ring + f.seg_slot, &ctx->ctl->segments, sizeof(unsigned long long),
+                                     hipMemcpyDeviceToHost, ctx->stream);
                 (void)hipEventCreateWithFlags(&f.ev, hipEventDisableTiming);
                 for (Job* j : in_chunk) {
                     if (!job_has_work(j)) {
@@ -1224,45 +1305,27 @@ static void server_loop(ilm_ctx* ctx) {
             if (q == hipErrorNotReady) break;
             double t = now_ms(ctx);
             std::vector<Job*> done = f.finishing;
+            {
+                std::lock_guard<std::mutex> lk(ctx->stat_mu);
+                ctx->stat_segments += ctx->h_seg_ring[f.seg_slot];
+            }
             (void)hipEventDestroy(f.ev);
             inflight.pop_front();
             ctx->chunks_retired++;
-            std::lock_guard<std::mutex> lk(ctx->sched_mu);
-            for (Job* j : done) {
-                j->end_ms = t;
-                job_release_device(j);
-                ctx->in_use -= j->bytes;
-                ctx->active.erase(std::find(ctx->active.begin(), ctx->active.end(), j));
-                ctx->latencies.push_back(t - j->submit_ms);
-                ctx->stat_done++;
-                j->state.store(j->failed ? ILM_REQ_FAILED : ILM_REQ_DONE);
-                j->samples_done = j->p.spp;
-                j->cv.notify_all();
-                if (--j->refs == 0) { scene_release(j->scene, true); delete j; }
-            }
+            for (Job* j : done) sched_finish(ctx, j, j->failed ? ILM_REQ_FAILED : ILM_REQ_DONE, t);
             launched = false;
         }
-        /* cancelled jobs: release once no chunk that used them is still in flight */
+        /* cancelled jobs: release once no chunk that used them is still in flight (stream order
+         * keeps any later user of their pooled buffers behind the kernels already queued) */
+        std::vector<Job*> cancelled;
         {
             std::lock_guard<std::mutex> lk(ctx->sched_mu);
-            for (size_t k = 0; k < ctx->active.size();) {
-                Job* j = ctx->active[k];
+            for (Job* j : ctx->active) {
                 bool busy = j->last_ev_seq >= ctx->chunk_seq - (int64_t)inflight.size();
-                if (j->cancel.load() && !busy) {
-                    (void)hipStreamSynchronize(ctx->stream);
-                    job_release_device(j);
-                    ctx->in_use -= j->bytes;
-                    ctx->active.erase(ctx->active.begin() + k);
-                    j->end_ms = now_ms(ctx);
-                    ctx->stat_cancelled++;
-                    j->state.store(ILM_REQ_CANCELLED);
-                    j->cv.notify_all();
-                    if (--j->refs == 0) { scene_release(j->scene, true); delete j; }
-                    continue;
-                }
-                k++;
+                if (j->cancel.load() && !busy) cancelled.push_back(j);
             }
         }
+        for (Job* j : cancelled) sched_finish(ctx, j, ILM_REQ_CANCELLED, now_ms(ctx));
     }
 }
 
@@ -1448,37 +1511,39 @@ Job::~Job() {
     if (h_px) (void)hipHostFree(h_px);
 }
 
+/* Reads host-side counters only (no device call, so a poll never waits behind a chunk in
+ * flight): segments as of the last completed chunk or render, p50/p99 over the last
+ * ILM_LATENCY_WINDOW completed requests. */
 int ilm_stats_get(ilm_ctx* ctx, ilm_stats* st) {
     if (!ctx || !st) return fail(ctx, ILM_E_INVALID, "null argument");
-    unsigned long long segs = 0;
-    (void)hipSetDevice(ctx->device);
-    (void)hipStreamSynchronize(ctx->stream);
-    (void)hipMemcpy(&segs, ctx->d_seg_total, sizeof(segs), hipMemcpyDeviceToHost);
+    double lat[ILM_LATENCY_WINDOW];
+    size_t n = 0;
     std::lock_guard<std::mutex> lk(ctx->sched_mu);
     std::memset(st, 0, sizeof(*st));
-    st->segments = segs;
     st->requests_done = ctx->stat_done;
     st->requests_rejected = ctx->stat_rejected;
     st->requests_cancelled = ctx->stat_cancelled;
     {
         std::lock_guard<std::mutex> sl(ctx->stat_mu);
+        st->segments = ctx->stat_segments;
         st->busy_ms = ctx->stat_busy_ms;
         st->bvh_cache_hits = ctx->stat_bvh_hits;
         st->bvh_cache_misses = ctx->stat_bvh_misses;
     }
-    std::vector<double> l = ctx->latencies;
-    if (!l.empty()) {
-        std::sort(l.begin(), l.end());
+    n = ctx->latencies.size();
+    std::copy(ctx->latencies.begin(), ctx->latencies.end(), lat);
+    st->hbm_budget_bytes = ctx->budget;
+    st->hbm_in_use_bytes = ctx->in_use;
+    if (n) {
+        std::sort(lat, lat + n);
         auto q = [&](double f) {
-            double pos = f * (double)(l.size() - 1);
+            double pos = f * (double)(n - 1);
             size_t i = (size_t)pos;
             double fr = pos - (double)i;
-            return i + 1 < l.size() ? l[i] * (1 - fr) + l[i + 1] * fr : l[i];
+            return i + 1 < n ? lat[i] * (1 - fr) + lat[i + 1] * fr : lat[i];
         };
         st->p50_ms = q(0.50);
         st->p99_ms = q(0.99);
     }
-    st->hbm_budget_bytes = ctx->budget;
-    st->hbm_in_use_bytes = ctx->in_use;
     return ILM_OK;
 }

@@ -1,9 +1,11 @@
 /*
  * illume_cli.cpp - command line of the reference (src/main.cu:12-43):
  *     illume <save> <scene.json> <spp> <maxdepth>
- * renders the scene and writes <save>-<w>x<h>-<spp>spp-<maxdepth>md.png. The PNG writer is
- * self-contained (zlib stored blocks, CRC-32 and Adler-32 computed here), replacing the
- * reference's libpng output (src/bitmap.c:67-151); 8-bit RGB as there.
+ * renders the scene and writes <save>-<w>x<h>-<spp>spp-<maxdepth>md.png, and beside it the
+ * same name with .pfm: the mean radiance per pixel (sum of the samples / spp) as float RGB,
+ * before tonemapping. The PNG writer is self-contained (zlib stored blocks, CRC-32 and Adler-32
+ * computed here), replacing the reference's libpng output (src/bitmap.c:67-151); 8-bit RGB as
+ * there. PFM: "PF" header, little-endian (negative scale), rows bottom to top.
  */
 #include <cstdio>
 #include <cstdlib>
@@ -75,6 +77,20 @@ bool write_png(const char* path, const illume::Bitmap* b) {
     return fclose(f) == 0;
 }
 
+/* float RGB PFM of the mean radiance (rgb_sum / spp), little-endian, rows bottom to top */
+bool write_pfm(const char* path, int w, int h, const std::vector<float>& rgb_sum, int spp) {
+    FILE* f = fopen(path, "wb");
+    if (!f) return false;
+    fprintf(f, "PF\n%d %d\n-1.0\n", w, h);
+    std::vector<float> row(3 * (size_t)w);
+    bool ok = true;
+    for (int y = h - 1; y >= 0 && ok; y--) {
+        for (size_t i = 0; i < 3 * (size_t)w; i++) row[i] = rgb_sum[3 * (size_t)y * w + i] / (float)spp;
+        ok = fwrite(row.data(), sizeof(float), row.size(), f) == row.size();
+    }
+    return (fclose(f) == 0) && ok;
+}
+
 int main(int argc, char** argv) {
     if (argc < 5) {
         printf("illume: usage - <save> <scene> <spp> <maxdepth>\n");
@@ -87,7 +103,8 @@ int main(int argc, char** argv) {
         illume::HostScene scene(dev, argv[2]);
         illume::Renderer renderer(scene, spp, max_depth);
         illume::Bitmap* image = illume::bitmap_new(renderer.get_width(), renderer.get_height());
-        renderer.render_to_bitmap(image);
+        std::vector<float> radiance(3 * (size_t)renderer.get_width() * renderer.get_height());
+        renderer.render_to_bitmap(image, radiance.data());
         printf("Rays per sec: %f rays/s\n", renderer.rays_per_sec());
         printf("Render time: %f seconds\n", renderer.last_info().device_ms * 1e-3);
         std::string name = std::string(argv[1]) + "-" + std::to_string(renderer.get_width()) + "x" +
@@ -99,6 +116,13 @@ int main(int argc, char** argv) {
             return 1;
         }
         printf("Saved to: %s\n", name.c_str());
+        const std::string pfm = name.substr(0, name.size() - 4) + ".pfm";
+        if (!write_pfm(pfm.c_str(), renderer.get_width(), renderer.get_height(), radiance, renderer.get_spp())) {
+            fprintf(stderr, "illume: cannot write %s\n", pfm.c_str());
+            illume::bitmap_free(image);
+            return 1;
+        }
+        printf("Saved to: %s\n", pfm.c_str());
         illume::bitmap_free(image);
     } catch (const illume::Error& e) {
         fprintf(stderr, "illume: %s (%d)\n", e.what(), e.code);

@@ -18,6 +18,7 @@
  * lines; vertex merging assigns indices in first-occurrence order as src/primitives/mesh.cu:264-340.
  */
 #include <cctype>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -145,6 +146,7 @@ struct ilm_desc {
     std::vector<ilm_medium_desc> mediums;
     std::vector<std::unique_ptr<TexOwn>> tex;
     std::string err;
+    double obj_parse_s = 0.0, obj_merge_s = 0.0;   /* wall time of the OBJ reads, summed over meshes */
 };
 
 static thread_local std::string g_loader_err;
@@ -323,51 +325,106 @@ bool conductor_get(const std::string& name, float eta[3], float k[3]) {
 }
 
 /* OBJ: src/primitives/mesh.cu:159-340, 363-388 */
+/* OBJ face token "p", "p/t", "p//n" or "p/t/n" (1-based, negative = relative to the end);
+ * false when it does not start with an index. */
+bool obj_face_token(const char*& s, long& p, long& t, long& n) {
+    char* e;
+    p = std::strtol(s, &e, 10);
+    if (e == s) return false;
+    s = e;
+    t = n = 0;
+    if (*s == '/') {
+        s++;
+        if (*s != '/') { t = std::strtol(s, &e, 10); s = e; }
+        if (*s == '/') { s++; n = std::strtol(s, &e, 10); s = e; }
+    }
+    while (*s && *s != ' ' && *s != '\t' && *s != '\n' && *s != '\r') s++;
+    return true;
+}
+
 bool load_obj(const std::string& path, bool z_up, bool face_normals, bool has_tex, bool flip, ilm_desc* D, int mi) {
-    std::ifstream in(path);
-    if (!in.good()) return false;
+    const auto t_start = std::chrono::steady_clock::now();
+    /* the whole file in memory, parsed in place (any line length) */
+    std::string buf;
+    {
+        FILE* f = fopen(path.c_str(), "rb");
+        if (!f) return false;
+        if (fseek(f, 0, SEEK_END) != 0) { fclose(f); return false; }
+        const long size = ftell(f);
+        if (size < 0 || fseek(f, 0, SEEK_SET) != 0) { fclose(f); return false; }
+        buf.resize((size_t)size);
+        const bool ok = fread(&buf[0], 1, (size_t)size, f) == (size_t)size;
+        fclose(f);
+        if (!ok) return false;
+    }
     std::vector<float> v, vt, vn;
     struct FV { int p, t, n; };
-    std::vector<FV> tris;
-    std::string line;
-    auto fix = [](long i, size_t n) -> int { return i > 0 ? (int)(i - 1) : (int)((long)n + i); };
-    while (std::getline(in, line)) {
-        const char* s = line.c_str();
+    std::vector<FV> tris, poly;
+    auto fix = [](long i, size_t n) -> int { return i > 0 ? (int)(i - 1) : (i < 0 ? (int)((long)n + i) : -1); };
+    const char* s = buf.c_str();
+    while (*s) {
         while (*s == ' ' || *s == '\t') s++;
-        if (s[0] == 'v' && s[1] == ' ') {
-            float a, b, c;
-            if (sscanf(s + 2, "%f %f %f", &a, &b, &c) == 3) {
-                if (z_up) { v.push_back(a); v.push_back(c); v.push_back(b); }
-                else { v.push_back(a); v.push_back(b); v.push_back(c); }
+        const char* line = s;
+        while (*s && *s != '\n') s++;          /* s: end of this line */
+        const char* q = line;
+        char* e;
+        if (q[0] == 'v' && (q[1] == ' ' || q[1] == '\t')) {
+            float c[3];
+            int k = 0;
+            q += 2;
+            for (; k < 3; k++) {
+                c[k] = std::strtof(q, &e);
+                if (e == q || e > s) break;
+                q = e;
             }
-        } else if (s[0] == 'v' && s[1] == 't') {
-            float a = 0, b = 0;
-            sscanf(s + 2, "%f %f", &a, &b);
-            vt.push_back(a); vt.push_back(b);
-        } else if (s[0] == 'v' && s[1] == 'n') {
-            float a, b, c;
-            if (sscanf(s + 2, "%f %f %f", &a, &b, &c) == 3) { vn.push_back(a); vn.push_back(b); vn.push_back(c); }
-        } else if (s[0] == 'f' && (s[1] == ' ' || s[1] == '\t')) {
-            std::istringstream ss(s + 2);
-            std::string tok;
-            std::vector<FV> poly;
-            while (ss >> tok) {
-                FV f = {-1, -1, -1};
-                long a = 0, b = 0, c = 0;
-                if (sscanf(tok.c_str(), "%ld/%ld/%ld", &a, &b, &c) == 3) f = {fix(a, v.size() / 3), fix(b, vt.size() / 2), fix(c, vn.size() / 3)};
-                else if (sscanf(tok.c_str(), "%ld//%ld", &a, &c) == 2) f = {fix(a, v.size() / 3), -1, fix(c, vn.size() / 3)};
-                else if (sscanf(tok.c_str(), "%ld/%ld", &a, &b) == 2) f = {fix(a, v.size() / 3), fix(b, vt.size() / 2), -1};
-                else if (sscanf(tok.c_str(), "%ld", &a) == 1) f = {fix(a, v.size() / 3), -1, -1};
-                else continue;
-                poly.push_back(f);
+            if (k == 3) {
+                if (z_up) { v.push_back(c[0]); v.push_back(c[2]); v.push_back(c[1]); }
+                else { v.push_back(c[0]); v.push_back(c[1]); v.push_back(c[2]); }
+            }
+        } else if (q[0] == 'v' && q[1] == 't') {
+            float c[2] = {0, 0};
+            q += 2;
+            for (int k = 0; k < 2; k++) {
+                const float x = std::strtof(q, &e);
+                if (e == q || e > s) break;
+                c[k] = x;
+                q = e;
+            }
+            vt.push_back(c[0]); vt.push_back(c[1]);
+        } else if (q[0] == 'v' && q[1] == 'n') {
+            float c[3];
+            int k = 0;
+            q += 2;
+            for (; k < 3; k++) {
+                c[k] = std::strtof(q, &e);
+                if (e == q || e > s) break;
+                q = e;
+            }
+            if (k == 3) { vn.push_back(c[0]); vn.push_back(c[1]); vn.push_back(c[2]); }
+        } else if (q[0] == 'f' && (q[1] == ' ' || q[1] == '\t')) {
+            poly.clear();
+            q += 2;
+            for (;;) {
+                while (q < s && (*q == ' ' || *q == '\t' || *q == '\r')) q++;
+                if (q >= s) break;
+                long a, b, c;
+                if (!obj_face_token(q, a, b, c)) {   /* not an index: skip the token */
+                    while (q < s && *q != ' ' && *q != '\t') q++;
+                    continue;
+                }
+                poly.push_back({fix(a, v.size() / 3), b ? fix(b, vt.size() / 2) : -1, c ? fix(c, vn.size() / 3) : -1});
             }
             for (size_t k = 2; k < poly.size(); k++) { tris.push_back(poly[0]); tris.push_back(poly[k - 1]); tris.push_back(poly[k]); }
         }
+        if (*s == '\n') s++;
     }
+    const auto t_parsed = std::chrono::steady_clock::now();
+    D->obj_parse_s += std::chrono::duration<double>(t_parsed - t_start).count();
     /* merge identical (position, normal, uv) vertices in first-occurrence order */
     struct Key { uint32_t w[8]; bool operator==(const Key& o) const { return !memcmp(w, o.w, sizeof(w)); } };
     struct KH { size_t operator()(const Key& k) const { size_t h = 1469598103934665603ull; for (uint32_t x : k.w) h = (h ^ x) * 1099511628211ull; return h; } };
     std::unordered_map<Key, int, KH> map;
+    map.reserve(tris.size() / 2 + 16);
     std::vector<float>& P = D->pos[mi];
     std::vector<float>& N = D->nrm[mi];
     std::vector<float>& T = D->uv[mi];
@@ -408,12 +465,20 @@ bool load_obj(const std::string& path, bool z_up, bool face_normals, bool has_te
         if (face_normals) for (size_t t = 0; t + 2 < I.size(); t += 3) std::swap(I[t + 1], I[t + 2]);
         else for (float& x : N) x = -x;
     }
+    D->obj_merge_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - t_parsed).count();
     return true;
 }
 
 }  // namespace
 
 extern "C" {
+
+int ilm_desc_load_times(const ilm_desc* d, double* parse_s, double* merge_s) {
+    if (!d) return ILM_E_INVALID;
+    if (parse_s) *parse_s = d->obj_parse_s;
+    if (merge_s) *merge_s = d->obj_merge_s;
+    return ILM_OK;
+}
 
 int ilm_desc_load_json(const char* path, ilm_desc** out) {
     if (!path || !out) return ILM_E_INVALID;

@@ -95,13 +95,16 @@ class Renderer {
         params_.max_depth = max_depth;
         params_.first_sample = 0;
     }
-    void render_to_bitmap(Bitmap* bitmap) {
+    void render_to_bitmap(Bitmap* bitmap) { render_to_bitmap(bitmap, nullptr); }
+    /* Also returns the Kahan-summed radiance of the spp samples, 3 floats per pixel (nullable). */
+    void render_to_bitmap(Bitmap* bitmap, float* accum_rgb) {
         if (!bitmap || bitmap->width != params_.width || bitmap->height != params_.height)
             throw Error(ILM_E_INVALID, "bitmap size differs from the scene resolution");
-        check(ilm_render(scene_.device().get(), scene_.device_scene(), &params_, nullptr,
+        check(ilm_render(scene_.device().get(), scene_.device_scene(), &params_, accum_rgb,
                          reinterpret_cast<ilm_pixel*>(bitmap->pixels), &info_),
               scene_.device().get());
     }
+    int get_spp() const { return params_.spp; }
     int get_width() const { return params_.width; }
     int get_height() const { return params_.height; }
     const ilm_render_info& last_info() const { return info_; }

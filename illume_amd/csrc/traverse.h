@@ -101,23 +101,26 @@ __device__ inline void write_hit(const KParams& P, uint32_t i, const Best& B) {
 }
 
 /* Reciprocal direction for the slab test. Components below 1e-30 in magnitude (axis-parallel
- * rays) become +-1e-30 first: with a true 1/0 the slab bounds lo/d - o/d turn into inf - inf =
- * NaN and a box the ray crosses could be rejected; with 1e-30 they are +-huge (or exactly 0),
- * which keeps parallel slabs exact and the fma products finite for |o| < 3e8. Only the box test
- * uses this; triangles and spheres see the true direction. */
+ * rays) become +-1e-30 first: with a true 1/0 a plane through the origin gives 0 * inf = NaN and
+ * a box the ray crosses could be rejected; with 1e-30 the slab distances (p - o) / d are +-huge
+ * (or exactly 0), which keeps parallel slabs exact and the products finite for |p - o| < 3e8.
+ * Only the box test uses this; triangles and spheres see the true direction. */
 __device__ inline float safe_dir(float x) { return __builtin_fabsf(x) < 1e-30f ? __builtin_copysignf(1e-30f, x) : x; }
 __device__ inline V3 rcp3(V3 d) {
     return mk3(__builtin_amdgcn_rcpf(safe_dir(d.x)), __builtin_amdgcn_rcpf(safe_dir(d.y)),
                __builtin_amdgcn_rcpf(safe_dir(d.z)));
 }
-__device__ inline V3 neg_o_over_d(V3 o, V3 id) { return mk3(-o.x * id.x, -o.y * id.y, -o.z * id.z); }
 
-/* Two child boxes of an FP32 BVH2 node against the ray (o, 1/d); noid = -o/d. */
-__device__ inline void box2(const float4 a, const float4 b, const float4 c, V3 id, V3 noid, float tmax,
+/* Two child boxes of an FP32 BVH2 node against the ray (o, 1/d). A slab distance is formed as
+ * (p - o) * (1/d): each of the two roundings and v_rcp_f32's error is relative to the distance
+ * itself, so BOX_LO / BOX_HI cover them wherever the ray starts (the fma form p/d - o/d carries
+ * the rounding of o/d, an absolute error of |o/d| * 2^-24 that a ray far from the world origin
+ * meeting a box at small distance would not be covered for). */
+__device__ inline void box2(const float4 a, const float4 b, const float4 c, V3 id, V3 o, float tmax,
                             float& t0, float& t1, bool& h0, bool& h1) {
-    float x0 = __builtin_fmaf(a.x, id.x, noid.x), x1 = __builtin_fmaf(a.y, id.x, noid.x);
-    float y0 = __builtin_fmaf(a.z, id.y, noid.y), y1 = __builtin_fmaf(a.w, id.y, noid.y);
-    float z0 = __builtin_fmaf(b.x, id.z, noid.z), z1 = __builtin_fmaf(b.y, id.z, noid.z);
+    float x0 = (a.x - o.x) * id.x, x1 = (a.y - o.x) * id.x;
+    float y0 = (a.z - o.y) * id.y, y1 = (a.w - o.y) * id.y;
+    float z0 = (b.x - o.z) * id.z, z1 = (b.y - o.z) * id.z;
     float tn = __builtin_fmaxf(__builtin_fmaxf(__builtin_fminf(x0, x1), __builtin_fminf(y0, y1)),
                                __builtin_fmaxf(__builtin_fminf(z0, z1), 0.0f));
     float tf = __builtin_fminf(__builtin_fminf(__builtin_fmaxf(x0, x1), __builtin_fmaxf(y0, y1)),
@@ -125,9 +128,9 @@ __device__ inline void box2(const float4 a, const float4 b, const float4 c, V3 i
     t0 = tn * BOX_LO;
     h0 = (t0 <= tf * BOX_HI) && (t0 <= tmax);
 
-    float X0 = __builtin_fmaf(b.z, id.x, noid.x), X1 = __builtin_fmaf(b.w, id.x, noid.x);
-    float Y0 = __builtin_fmaf(c.x, id.y, noid.y), Y1 = __builtin_fmaf(c.y, id.y, noid.y);
-    float Z0 = __builtin_fmaf(c.z, id.z, noid.z), Z1 = __builtin_fmaf(c.w, id.z, noid.z);
+    float X0 = (b.z - o.x) * id.x, X1 = (b.w - o.x) * id.x;
+    float Y0 = (c.x - o.y) * id.y, Y1 = (c.y - o.y) * id.y;
+    float Z0 = (c.z - o.z) * id.z, Z1 = (c.w - o.z) * id.z;
     float Tn = __builtin_fmaxf(__builtin_fmaxf(__builtin_fminf(X0, X1), __builtin_fminf(Y0, Y1)),
                                __builtin_fmaxf(__builtin_fminf(Z0, Z1), 0.0f));
     float Tf = __builtin_fminf(__builtin_fminf(__builtin_fmaxf(X0, X1), __builtin_fmaxf(Y0, Y1)),
@@ -138,7 +141,7 @@ __device__ inline void box2(const float4 a, const float4 b, const float4 c, V3 i
 
 /* Up to four dequantised child boxes of a QNode (dequantisation as bvh.cpp's qnode_child_box). */
 /* w0 = ox oy oz | ex ey ez nchild, w1 = qlo[x][0..3] qlo[y] qlo[z] qhi[x], w2 = qhi[y] qhi[z] */
-__device__ inline void box4q(const float4 w0, const float4 w1, const float4 w2, V3 id, V3 noid, float tmax,
+__device__ inline void box4q(const float4 w0, const float4 w1, const float4 w2, V3 id, V3 o, float tmax,
                              float* tn_out, uint32_t& hitmask, int& nchild) {
     const uint32_t e = f2u(w0.w);
     const float sx = u2f((e & 0xffu) << 23), sy = u2f(((e >> 8) & 0xffu) << 23), sz = u2f(((e >> 16) & 0xffu) << 23);
@@ -146,16 +149,17 @@ __device__ inline void box4q(const float4 w0, const float4 w1, const float4 w2, 
     const uint32_t lx = f2u(w1.x), ly = f2u(w1.y), lz = f2u(w1.z);
     const uint32_t hx = f2u(w1.w), hy = f2u(w2.x), hz = f2u(w2.y);
     hitmask = 0;
-    /* slab distance of plane q: (o_n + q*s - o) / d = q * (s/d) + (o_n/d - o/d). s is a power of
+    /* slab distance of plane q: (o_n + q*s - o) / d = q * (s/d) + (o_n - o)/d. s is a power of
      * two, so A = s * (1/d) is exact while it is a normal float, and a plane costs one fma on top
-     * of the byte conversion; its rounding error (that of B = fma(o_n, 1/d, -o/d), then of the
-     * fma) is no larger than the direct form's, whose plane o_n + q*s is rounded at the
-     * magnitude of o_n. |1/d| <= 1e30 (safe_dir) and s <= 2^ILM_QUANT_MAX_EXP (scenes that
-     * need a larger quantum do not get the 8-bit layout) keep A below 2^124; a subnormal A only
-     * adds an absolute error below 2^-140. */
+     * of the byte conversion. The node origin and the ray origin are differenced first, so no
+     * error grows with the distance of either from the world origin: with B = (o_n - o) * (1/d)
+     * rounded twice, a plane's error is relative to its distance (covered by BOX_LO / BOX_HI)
+     * plus at most 2^-23 * 255 * |A|, which the builder's outward pad of every child box
+     * (bvh.cpp QPAD_REL, 2^-20 of the node extent) covers four times over. |1/d| <= 1e30
+     * (safe_dir) and s <= 2^ILM_QUANT_MAX_EXP (scenes that need a larger quantum do not get the
+     * 8-bit layout) keep A below 2^124; a subnormal A only adds an absolute error below 2^-140. */
     const float ax = sx * id.x, ay = sy * id.y, az = sz * id.z;
-    const float bx = __builtin_fmaf(w0.x, id.x, noid.x), by = __builtin_fmaf(w0.y, id.y, noid.y);
-    const float bz = __builtin_fmaf(w0.z, id.z, noid.z);
+    const float bx = (w0.x - o.x) * id.x, by = (w0.y - o.y) * id.y, bz = (w0.z - o.z) * id.z;
 #pragma unroll
     for (int k = 0; k < 4; k++) {
         const int sh = 8 * k;
@@ -198,6 +202,12 @@ struct Stack {
         --sp;
         if (!OVF || sp < LDS_STACK) return lds[sp * EXT_BLOCK];
         return *spill(sp);
+    }
+    /* the entry pop() would return (sp > 0) */
+    __device__ inline int top() const {
+        const int s = sp - 1;
+        if (!OVF || s < LDS_STACK) return lds[s * EXT_BLOCK];
+        return *spill(s);
     }
 };
 
@@ -321,12 +331,12 @@ __device__ inline void leaf_tris(const GLOBAL float4* tris, int node, V3 o, V3 d
  * first three), ch its child codes. */
 template <int QUANT, int OVF, int COUNT>
 __device__ inline bool inner_step_words(const float4 n0, const float4 n1, const float4 n2, const int4 ch, int& node,
-                                        V3 id, V3 noid, float tmax, Stack<OVF>& st, TravCount* tc) {
+                                        V3 id, V3 o, float tmax, Stack<OVF>& st, TravCount* tc) {
     if (QUANT) {
         float tn[4];
         uint32_t hm;
         int nc;
-        box4q(n0, n1, n2, id, noid, tmax, tn, hm, nc);
+        box4q(n0, n1, n2, id, o, tmax, tn, hm, nc);
         if (COUNT) tc->boxes += nc;
         if (hm == 0) return false;
         /* order the hit children near to far with a 4-input sorting network (missed children
@@ -356,7 +366,7 @@ __device__ inline bool inner_step_words(const float4 n0, const float4 n1, const 
     } else {
         float t0, t1;
         bool h0, h1;
-        box2(n0, n1, n2, id, noid, tmax, t0, t1, h0, h1);
+        box2(n0, n1, n2, id, o, tmax, t0, t1, h0, h1);
         if (COUNT) tc->boxes += 2;
         if (h0 && h1) {
             int nearc = ch.x, farc = ch.y;
@@ -383,12 +393,12 @@ __device__ inline void load_node(const GLOBAL BNode* nodes, const GLOBAL QNode* 
 }
 
 template <int QUANT, int OVF, int COUNT>
-__device__ inline bool inner_step(const GLOBAL BNode* nodes, const GLOBAL QNode* qnodes, int& node, V3 id, V3 noid,
+__device__ inline bool inner_step(const GLOBAL BNode* nodes, const GLOBAL QNode* qnodes, int& node, V3 id, V3 o,
                                   float tmax, Stack<OVF>& st, TravCount* tc) {
     float4 n0, n1, n2;
     int4 ch;
     load_node<QUANT>(nodes, qnodes, node, n0, n1, n2, ch);
-    return inner_step_words<QUANT, OVF, COUNT>(n0, n1, n2, ch, node, id, noid, tmax, st, tc);
+    return inner_step_words<QUANT, OVF, COUNT>(n0, n1, n2, ch, node, id, o, tmax, st, tc);
 }
 
 /* The same visit for a ray kept as (o, d): the node's loads are issued first and 1/d, -o/d are
@@ -402,9 +412,8 @@ __device__ inline bool inner_step_od(const GLOBAL BNode* nodes, const GLOBAL QNo
     int4 ch;
     load_node<QUANT>(nodes, qnodes, node, n0, n1, n2, ch);
     const V3 id = rcp3(d);
-    const V3 noid = neg_o_over_d(o, id);
     asm volatile("" ::"v"(ch.x), "v"(ch.y), "v"(ch.z), "v"(ch.w));
-    return inner_step_words<QUANT, OVF, COUNT>(n0, n1, n2, ch, node, id, noid, tmax, st, tc);
+    return inner_step_words<QUANT, OVF, COUNT>(n0, n1, n2, ch, node, id, o, tmax, st, tc);
 }
 
 template <int TRI, int QUANT, int COUNT, int OVF>
@@ -422,12 +431,11 @@ __device__ inline void trace_ray(const GLOBAL DScene* S, V3 O, V3 D, float tmax0
     int cur = -1;
     V3 o = O, d = D;
     V3 id = rcp3(d);
-    V3 noid = neg_o_over_d(o, id);
     WTRay wt;
     st.sp = 0;
     for (;;) {
         if (node >= 0) {
-            if (inner_step<QUANT, OVF, COUNT>(nodes, qnodes, node, id, noid, B.t, st, tc)) continue;
+            if (inner_step<QUANT, OVF, COUNT>(nodes, qnodes, node, id, o, B.t, st, tc)) continue;
         } else if (!blas) {
             /* TLAS leaf: one instance; enter its BLAS in object space */
             const int k = gp(S->tlas_prims)[ilm_leaf_first(node)];
@@ -440,7 +448,6 @@ __device__ inline void trace_ray(const GLOBAL DScene* S, V3 O, V3 D, float tmax0
                 o = xform3(inv, O, 1.0f);
                 d = xform3(inv, D, 0.0f);
                 id = rcp3(d);
-                noid = neg_o_over_d(o, id);
                 if (TRI) wt = wt_setup(d);
                 nodes = gp(M->nodes);
                 qnodes = gp(M->qnodes);
@@ -465,7 +472,6 @@ __device__ inline void trace_ray(const GLOBAL DScene* S, V3 O, V3 D, float tmax0
             o = O;
             d = D;
             id = rcp3(d);
-            noid = neg_o_over_d(o, id);
         }
     }
 }
@@ -503,16 +509,15 @@ __device__ inline void packet_trace(const GLOBAL DScene* S, V3 O, V3 D, Best& B,
             int4 ch;
             load_node<QUANT>(nodes, qnodes, node, n0, n1, n2, ch);
             const V3 id = rcp3(d);
-            const V3 noid = neg_o_over_d(o, id);
             float tn[4];
             uint32_t hm;
             if (QUANT) {
                 int nc;
-                box4q(n0, n1, n2, id, noid, B.t, tn, hm, nc);
+                box4q(n0, n1, n2, id, o, B.t, tn, hm, nc);
                 if (COUNT) tc->boxes += nc;
             } else {
                 bool h0, h1;
-                box2(n0, n1, n2, id, noid, B.t, tn[0], tn[1], h0, h1);
+                box2(n0, n1, n2, id, o, B.t, tn[0], tn[1], h0, h1);
                 tn[2] = tn[3] = 0.0f;
                 hm = (h0 ? 1u : 0u) | (h1 ? 2u : 0u);
                 if (COUNT) tc->boxes += 2;
@@ -734,7 +739,7 @@ __global__ EXT_BOUNDS void k_extend(KParams P, int depth, int* ovf_base) {
 #define NO_RAY 0xffffffffu
 
 struct TState {
-    V3 o, d, id, noid;      /* ray in the current (world or object) space */
+    V3 o, d, id;           /* ray in the current (world or object) space */
     const GLOBAL BNode* nodes;
     const GLOBAL QNode* qnodes;
     const GLOBAL float4* tris;
@@ -761,7 +766,6 @@ __device__ inline void seg_init(TState& T, Best& B, Stack<OVF>& st, const GLOBAL
     T.o = O;
     T.d = D;
     T.id = rcp3(D);
-    T.noid = neg_o_over_d(O, T.id);
     st.sp = 0;
 }
 
@@ -773,7 +777,7 @@ __device__ inline bool seg_run(TState& T, Best& B, Stack<OVF>& st, const GLOBAL 
     for (;;) {
         if (node == POP_MARK) goto pop;
         if (node >= 0) {
-            if (inner_step<QUANT, OVF, COUNT>(T.nodes, T.qnodes, node, T.id, T.noid, B.t, st, tc)) continue;
+            if (inner_step<QUANT, OVF, COUNT>(T.nodes, T.qnodes, node, T.id, T.o, B.t, st, tc)) continue;
             goto pop;
         } else if (!T.blas) {
             const int k = gp(T.S->tlas_prims)[ilm_leaf_first(node)];
@@ -787,7 +791,6 @@ __device__ inline bool seg_run(TState& T, Best& B, Stack<OVF>& st, const GLOBAL 
             T.o = xform3(inv, O, 1.0f);
             T.d = xform3(inv, D, 0.0f);
             T.id = rcp3(T.d);
-            T.noid = neg_o_over_d(T.o, T.id);
             if (TRI) T.wt = wt_setup(T.d);
             T.nodes = gp(M->nodes);
             T.qnodes = gp(M->qnodes);
@@ -812,7 +815,6 @@ __device__ inline bool seg_run(TState& T, Best& B, Stack<OVF>& st, const GLOBAL 
             T.o = mk3(a.x, a.y, a.z);
             T.d = mk3(b.x, b.y, b.z);
             T.id = rcp3(T.d);
-            T.noid = neg_o_over_d(T.o, T.id);
             T.blas = 0;
             T.nodes = gp(T.S->tlas);
             T.qnodes = gp(T.S->qtlas);
@@ -932,7 +934,7 @@ struct PState {
     V3 O, D;                /* world ray */
     V3 o, d;                /* ray in the current (world or object) space */
 #if !PSTATE_LEAN
-    V3 id, noid;
+    V3 id;
 #endif
     const GLOBAL BNode* nodes;
     const GLOBAL QNode* qnodes;
@@ -941,8 +943,10 @@ struct PState {
     int node;
     int cur;
     int blas;
+    int pend;               /* speculative walk: a parked BLAS leaf, or NO_PEND */
     WTRay wt;
 };
+#define NO_PEND ((int)0x7fffffff)
 
 template <int QUANT, int OVF>
 __device__ inline void pstep_init(PState& T, Best& B, Stack<OVF>& st, const GLOBAL DScene* S, V3 O, V3 D, float tmax0) {
@@ -957,13 +961,13 @@ __device__ inline void pstep_init(PState& T, Best& B, Stack<OVF>& st, const GLOB
     T.tris = nullptr;
     T.blas = 0;
     T.cur = -1;
+    T.pend = NO_PEND;
     T.O = O;
     T.D = D;
     T.o = O;
     T.d = D;
 #if !PSTATE_LEAN
     T.id = rcp3(D);
-    T.noid = neg_o_over_d(O, T.id);
 #endif
     st.sp = 0;
 }
@@ -981,7 +985,7 @@ __device__ inline bool pstep(PState& T, Best& B, Stack<OVF>& st, TravCount* tc, 
 #if PSTATE_LEAN
         pop = !inner_step_od<QUANT, OVF, COUNT>(T.nodes, T.qnodes, node, T.o, T.d, B.t, st, tc);
 #else
-        pop = !inner_step<QUANT, OVF, COUNT>(T.nodes, T.qnodes, node, T.id, T.noid, B.t, st, tc);
+        pop = !inner_step<QUANT, OVF, COUNT>(T.nodes, T.qnodes, node, T.id, T.o, B.t, st, tc);
 #endif
     } else if (!T.blas) {
         const int k = gp(S->tlas_prims)[ilm_leaf_first(node)];
@@ -995,7 +999,6 @@ __device__ inline bool pstep(PState& T, Best& B, Stack<OVF>& st, TravCount* tc, 
             T.d = xform3(inv, T.D, 0.0f);
 #if !PSTATE_LEAN
             T.id = rcp3(T.d);
-            T.noid = neg_o_over_d(T.o, T.id);
 #endif
             if (TRI) T.wt = wt_setup(T.d);
             T.nodes = gp(M->nodes);
@@ -1019,7 +1022,6 @@ __device__ inline bool pstep(PState& T, Best& B, Stack<OVF>& st, TravCount* tc, 
             T.d = T.D;
 #if !PSTATE_LEAN
             T.id = rcp3(T.D);
-            T.noid = neg_o_over_d(T.O, T.id);
 #endif
             T.blas = 0;
             T.nodes = gp(S->tlas);
@@ -1030,9 +1032,64 @@ __device__ inline bool pstep(PState& T, Best& B, Stack<OVF>& st, TravCount* tc, 
     return false;
 }
 
+/* ---------------- speculative walk (extend_mode 4) ----------------
+ * Per-step refill with speculative traversal (Aila and Laine, HPG 2009): on a node trip a lane
+ * whose walk has reached a BLAS leaf parks that leaf and walks on inside the same BLAS, so it
+ * visits nodes instead of idling until the wave's next leaf trip; the parked leaf is intersected
+ * on a leaf trip. A lane holds one parked leaf and is blocked (waits for a leaf trip) at a second
+ * leaf, at the instance boundary (the parked leaf is in this BLAS's object space) and at the end
+ * of its stack. The closest hit is unchanged: only the order of the tests changes, and while a
+ * leaf is parked the boxes are tested against a closest hit that is not yet lowered by it, so a
+ * few more boxes pass. */
+template <int OVF>
+__device__ inline bool spec_blocked(const PState& T, const Stack<OVF>& st) {
+    if (T.pend == NO_PEND) return false;
+    if (T.node < 0 && T.blas) return true;
+    return T.node == POP_MARK && (st.sp == 0 || st.top() == STACK_SENTINEL);
+}
+
+/* A node trip of an unblocked lane; true when the ray is finished. */
+template <int TRI, int QUANT, int COUNT, int OVF, int UNI>
+__device__ inline bool pstep_spec_node(PState& T, Best& B, Stack<OVF>& st, TravCount* tc, const GLOBAL DScene* Su) {
+    int node = T.node;
+    if (node < 0 && T.blas) {            /* reached a leaf with none parked: park it */
+        T.pend = node;
+        node = POP_MARK;
+    }
+    if (T.pend == NO_PEND) return pstep<TRI, QUANT, COUNT, OVF, UNI>(T, B, st, tc, Su);
+    if (node == POP_MARK) {
+        if (st.sp == 0 || st.top() == STACK_SENTINEL) {
+            T.node = POP_MARK;
+            return false;
+        }
+        node = st.pop();
+        if (node < 0) {                  /* a second leaf: wait for a leaf trip */
+            T.node = node;
+            return false;
+        }
+    }
+    /* an inner BLAS node with a leaf parked: one visit; pops stay inside the BLAS */
+    if (!inner_step_od<QUANT, OVF, COUNT>(T.nodes, T.qnodes, node, T.o, T.d, B.t, st, tc)) {
+        if (st.sp == 0 || st.top() == STACK_SENTINEL) node = POP_MARK;
+        else node = st.pop();
+    }
+    T.node = node;
+    return false;
+}
+
+/* A leaf trip: the parked leaf if there is one, else the leaf the walk stands at; true when the
+ * ray is finished. */
+template <int TRI, int QUANT, int COUNT, int OVF, int UNI>
+__device__ inline bool pstep_spec_leaf(PState& T, Best& B, Stack<OVF>& st, TravCount* tc, const GLOBAL DScene* Su) {
+    if (T.pend == NO_PEND) return pstep<TRI, QUANT, COUNT, OVF, UNI>(T, B, st, tc, Su);
+    leaf_tris<TRI, COUNT>(T.tris, T.pend, T.o, T.d, T.wt, T.cur, B, tc);
+    T.pend = NO_PEND;
+    return T.node == POP_MARK && st.sp == 0;
+}
+
 /* UNI: every request of the chunk renders the same scene (the host checks), so the scene
  * pointer is read once, uniform over the wave, instead of per lane at every refill. */
-template <int TRI, int QUANT, int COUNT, int OVF, int CAM, int UNI>
+template <int TRI, int QUANT, int COUNT, int OVF, int CAM, int UNI, int SPEC>
 __global__ EXT_BOUNDS void k_extend_pers(KParams P, int depth, int* ovf_base) {
     __shared__ int stk[LDS_STACK * EXT_BLOCK];
     const int lane = threadIdx.x;
@@ -1097,14 +1154,34 @@ __global__ EXT_BOUNDS void k_extend_pers(KParams P, int depth, int* ovf_base) {
          * of the two code paths instead of both. Either group is non-empty when chosen unless
          * no lane is busy, so every trip makes progress. */
         const bool has = ridx != NO_RAY;
-        const bool at_leaf = has && T.node < 0 && T.blas;
-        const uint32_t n_leaf = (uint32_t)__popcll(__ballot(at_leaf));
-        const uint32_t n_rest = (uint32_t)__popcll(__ballot(has)) - n_leaf;
-        const bool leaf_trip = n_leaf * 100u >= n_rest * leaf_pct;
-        if (has && at_leaf == leaf_trip) {
-            if (pstep<TRI, QUANT, COUNT, OVF, UNI>(T, B, st, &tc, Su)) {
+        if (SPEC) {
+            /* speculative walk: a leaf trip once the blocked lanes reach leaf_pct percent of
+             * the others; a blocked lane always has a parked leaf, so either trip progresses */
+            const bool blocked = has && spec_blocked(T, st);
+            const bool leafy = has && (T.pend != NO_PEND || (T.node < 0 && T.blas));
+            const uint32_t n_blocked = (uint32_t)__popcll(__ballot(blocked));
+            const uint32_t n_free = (uint32_t)__popcll(__ballot(has)) - n_blocked;
+            const bool leaf_trip = n_blocked * 100u >= n_free * leaf_pct;
+            bool done = false;
+            if (leaf_trip) {
+                if (leafy) done = pstep_spec_leaf<TRI, QUANT, COUNT, OVF, UNI>(T, B, st, &tc, Su);
+            } else if (has && !blocked) {
+                done = pstep_spec_node<TRI, QUANT, COUNT, OVF, UNI>(T, B, st, &tc, Su);
+            }
+            if (done) {
                 write_hit(P, ridx, B);
                 ridx = NO_RAY;
+            }
+        } else {
+            const bool at_leaf = has && T.node < 0 && T.blas;
+            const uint32_t n_leaf = (uint32_t)__popcll(__ballot(at_leaf));
+            const uint32_t n_rest = (uint32_t)__popcll(__ballot(has)) - n_leaf;
+            const bool leaf_trip = n_leaf * 100u >= n_rest * leaf_pct;
+            if (has && at_leaf == leaf_trip) {
+                if (pstep<TRI, QUANT, COUNT, OVF, UNI>(T, B, st, &tc, Su)) {
+                    write_hit(P, ridx, B);
+                    ridx = NO_RAY;
+                }
             }
         }
     }

@@ -287,3 +287,19 @@ def serve_trace(seed=4, rate=20.0, duration=60.0):
         out.append(dict(t=t, scene=int(rng.randint(4)), width=w, height=h, spp=int(rng.randint(8, 513)),
                         max_depth=int(rng.randint(8, 33))))
     return out
+
+
+def write_obj(mesh, path, normals=False):
+    """Write a mesh dict (positions, indices; vertex normals when normals=True) as a Wavefront
+    OBJ: `v x y z` lines with shortest round-trip float text, then `f a b c` (or `f a//a ...`)
+    with 1-based indices."""
+    pos = np.asarray(mesh["positions"], np.float32)
+    idx = np.asarray(mesh["indices"], np.int64) + 1
+    with open(path, "w") as f:
+        f.write("".join("v %r %r %r\n" % (float(x), float(y), float(z)) for x, y, z in pos))
+        if normals:
+            nrm = np.asarray(mesh["normals"], np.float32)
+            f.write("".join("vn %r %r %r\n" % (float(x), float(y), float(z)) for x, y, z in nrm))
+            f.write("".join("f %d//%d %d//%d %d//%d\n" % (a, a, b, b, c, c) for a, b, c in idx))
+        else:
+            f.write("".join("f %d %d %d\n" % (a, b, c) for a, b, c in idx))

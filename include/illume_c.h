@@ -177,8 +177,11 @@ typedef struct ilm_pixel {
 
 /* Extend-kernel scheduling (results are identical for every mode). BATCH: each wave traces 64
  * rays to completion; LEAF_REFILL: idle lanes take new rays when the wave reaches BLAS leaves;
- * STEP_REFILL: idle lanes take new rays after every traversal step (the default). */
-enum { ILM_EXTEND_DEFAULT = 0, ILM_EXTEND_BATCH = 1, ILM_EXTEND_LEAF_REFILL = 2, ILM_EXTEND_STEP_REFILL = 3 };
+ * STEP_REFILL: idle lanes take new rays after every traversal step (the default); SPECULATIVE:
+ * STEP_REFILL where a lane that reaches a BLAS leaf parks it and walks on until the wave's next
+ * leaf trip. */
+enum { ILM_EXTEND_DEFAULT = 0, ILM_EXTEND_BATCH = 1, ILM_EXTEND_LEAF_REFILL = 2, ILM_EXTEND_STEP_REFILL = 3,
+       ILM_EXTEND_SPECULATIVE = 4 };
 
 typedef struct ilm_opts {
     int32_t bvh_layout;         /* ILM_BVH_* (default FP32) */
@@ -208,8 +211,10 @@ typedef struct ilm_opts {
     /* refill modes: number of idle lanes of a wave that triggers a refill (0 = 32) */
     int32_t refill_min;
     /* rays an extend wave reserves from the queue at a time (multiple of 64; 0 = 256): its first
-     * reservation by its index, later ones from the shared cursor; a queue shorter than the
-     * grid's first reservations is split evenly over the waves instead */
+     * reservation by its index, later ones from the shared cursor. In the per-step and leaf
+     * refill modes a queue shorter than the grid's first reservations is split evenly over the
+     * waves instead; the batch mode and the first depth's camera packets shrink their later
+     * reservations as the queue drains (down to 64 rays) */
     int32_t ext_fetch;
     /* per-step refill: a wave runs its BLAS-leaf triangle tests only on trips where the lanes
      * waiting at leaves are at least this percentage of its other busy lanes (0 = 100) */
@@ -300,6 +305,9 @@ int ilm_desc_load_json(const char* path, ilm_desc** out);
 const ilm_scene_desc* ilm_desc_scene(const ilm_desc* d);
 int ilm_desc_settings(const ilm_desc* d, ilm_render_params* out);
 void ilm_desc_free(ilm_desc* d);
+/* Wall time the load spent reading OBJ files (parse) and merging their vertices, summed over
+ * the scene's meshes; the reference's reader is src/primitives/mesh.cu:159-340. */
+int ilm_desc_load_times(const ilm_desc* d, double* parse_s, double* merge_s);
 const char* ilm_loader_error(void);
 
 /* ---- plain render (one request, synchronous) ----

@@ -483,6 +483,7 @@ template <class R> struct Scene {
     std::vector<Material<R>> mats;
     std::vector<Medium<R>> mediums;
     Tex<R> env;
+    bool tpruned = false;   /* mesh_intersect_pruned instead of the reference walk */
     R fov, dof, aperture;
     M4<R> cam;
 };
@@ -687,6 +688,72 @@ template <class R> static inline R tri_intersect(V3<R> v0, V3<R> v1, V3<R> v2, c
     return vdot(e2, Q) * det;
 }
 
+template <class R> static void mesh_hit_attributes(const Mesh<R>& mesh, int tri_index, Hit<R>* hit);
+
+/* The build's traversal order over the same tree, for the second CPU baseline figure (not the
+ * reference's walk): a node's two children are box-tested together, the nearer is entered
+ * first, and a subtree is skipped once its entry distance lies beyond the closest hit so far
+ * (tmax: the closest hit of the objects tested before this instance). The hits equal
+ * mesh_intersect's except at exact distance ties, and no subtree is dropped for stack depth. */
+template <class R> static void mesh_intersect_pruned(const Mesh<R>& mesh, const Ray<R>& ray, R tmax, Hit<R>* hit,
+                                                     int* box_tests, int* tri_tests) {
+    struct Entry { uint32_t node; R t; };
+    Entry stack[128];
+    int sp = 0;
+    hit->is = false;
+    hit->d = (R)FLT_MAX;
+    R best = tmax;
+    int tri_index = -1;
+    /* boxes of inner nodes only: the reference walk never tests a leaf's box, and a leaf of
+     * a flat mesh can have a box of zero thickness; a leaf inherits its parent's distance */
+    R parent_t = 0;
+    auto enter = [&](uint32_t n, R* t) {
+        const RefNode& c = mesh.bvh.nodes[n];
+        if (c.a & 0x80000000u) { *t = parent_t; return true; }
+        AABB<R> b{v3((R)c.bounds.lo.x, (R)c.bounds.lo.y, (R)c.bounds.lo.z), v3((R)c.bounds.hi.x, (R)c.bounds.hi.y, (R)c.bounds.hi.z)};
+        if (box_tests) (*box_tests)++;
+        R tb = aabb_ray_intersect(b, ray);
+        if (tb == (R)-FLT_MAX) return false;
+        *t = tb > (R)0 ? tb : (R)0;
+        return *t <= best;
+    };
+    R t0;
+    if (mesh.bvh.nodes.empty() || !enter(0, &t0)) return;
+    stack[sp++] = {0u, t0};
+    while (sp > 0) {
+        const Entry e = stack[--sp];
+        if (e.t > best) continue;
+        const RefNode& node = mesh.bvh.nodes[e.node];
+        if (node.a & 0x80000000u) {
+            for (uint32_t i = node.b; i < node.b + (node.a & 0x7fffffffu); i++) {
+                int t = mesh.bvh.tri_indices[i];
+                R u = 0, v = 0;
+                if (tri_tests) (*tri_tests)++;
+                R d = tri_intersect(mesh.pos[mesh.idx[3 * t]], mesh.pos[mesh.idx[3 * t + 1]], mesh.pos[mesh.idx[3 * t + 2]], ray, u, v);
+                if (d > 0 && d < best) { tri_index = t; hit->u = u; hit->v = v; best = d; }
+            }
+        } else {
+            R ta = 0, tb = 0;
+            parent_t = e.t;
+            const bool ha = enter(node.a, &ta), hb = enter(node.b, &tb);
+            if (ha && hb) {
+                const bool a_first = ta <= tb;
+                stack[sp++] = a_first ? Entry{node.b, tb} : Entry{node.a, ta};
+                stack[sp++] = a_first ? Entry{node.a, ta} : Entry{node.b, tb};
+            } else if (ha) {
+                stack[sp++] = {node.a, ta};
+            } else if (hb) {
+                stack[sp++] = {node.b, tb};
+            }
+        }
+    }
+    if (tri_index != -1) {
+        hit->is = true;
+        hit->d = best;
+        mesh_hit_attributes(mesh, tri_index, hit);
+    }
+}
+
 /* src/primitives/mesh.cu:479-562; returns is_intersect, d, uv, normal (object space), prim */
 template <class R> static void mesh_intersect(const Mesh<R>& mesh, const Ray<R>& ray, Hit<R>* hit, int* box_tests, int* tri_tests) {
     const int BVH_STACK_SIZE = 32;
@@ -718,7 +785,12 @@ template <class R> static void mesh_intersect(const Mesh<R>& mesh, const Ray<R>&
             }
         }
     }
-    if (tri_index != -1) {
+    if (tri_index != -1) mesh_hit_attributes(mesh, tri_index, hit);
+}
+
+/* normal and texture coordinates of a triangle hit, src/primitives/mesh.cu:532-561 */
+template <class R> static void mesh_hit_attributes(const Mesh<R>& mesh, int tri_index, Hit<R>* hit) {
+    {
         const int* ix = &mesh.idx[3 * tri_index];
         if (mesh.face_normals) {
             V3<R> v0 = mesh.pos[ix[0]], v1 = mesh.pos[ix[1]], v2 = mesh.pos[ix[2]];
@@ -780,7 +852,8 @@ template <class R> static Hit<R> get_min_hit(const Scene<R>& sc, const Ray<R>& r
         r2.o = m4_mul_v3(in.t.inv, ray.o, (R)1);
         r2.d = m4_mul_v3(in.t.inv, ray.d, (R)0);
         Hit<R> h;
-        mesh_intersect(sc.meshes[in.mesh], r2, &h, box_tests, tri_tests);
+        if (sc.tpruned) mesh_intersect_pruned(sc.meshes[in.mesh], r2, min.d, &h, box_tests, tri_tests);
+        else mesh_intersect(sc.meshes[in.mesh], r2, &h, box_tests, tri_tests);
         if (h.is) {
             h.normal = m4_mul_v3(in.t.trans_inv, h.normal, (R)0);
             vnormalize(&h.normal);
@@ -1178,6 +1251,14 @@ orc_scene* orc_scene_create2(const ilm_scene_desc* desc, int fp64, int threads, 
 
 orc_scene* orc_scene_create(const ilm_scene_desc* desc, int fp64, int threads) {
     return orc_scene_create2(desc, fp64, threads, 0);
+}
+
+/* 1: closest hits by the build's ordered, t-pruned traversal over the same trees (second CPU
+ * baseline figure); 0: the reference's exhaustive walk (the default) */
+void orc_scene_set_tpruned(orc_scene* s, int on) {
+    if (!s) return;
+    if (s->f) s->f->tpruned = on != 0;
+    if (s->d) s->d->tpruned = on != 0;
 }
 
 void orc_scene_free(orc_scene* s) {

@@ -31,6 +31,7 @@ def lib():
         "orc_scene_create": (vp, [P(SceneDesc), C.c_int, C.c_int]),
         "orc_scene_create2": (vp, [P(SceneDesc), C.c_int, C.c_int, C.c_int]),
         "orc_scene_free": (None, [vp]),
+        "orc_scene_set_tpruned": (None, [vp, C.c_int]),
         "orc_ref_bvh_size": (C.c_int, [vp, C.c_int, ip]),
         "orc_ref_bvh_export": (C.c_int, [vp, C.c_int, fp, P(C.c_uint32), ip]),
         "orc_closest_hit": (C.c_int, [vp, fp, C.c_int, ip, ip, fp, C.c_int, ip, C.c_int]),
@@ -109,6 +110,11 @@ class OracleScene:
             self.free()
         except Exception:
             pass
+
+    def set_tpruned(self, on):
+        """Closest hits by the build's ordered, t-pruned traversal over the same trees (True) or
+        by the reference's exhaustive walk (False, the default)."""
+        lib().orc_scene_set_tpruned(self.h, int(bool(on)))
 
     def closest_hit(self, rays, brute=False, counts=False, threads=0):
         rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 6)

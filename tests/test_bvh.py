@@ -10,7 +10,7 @@ import pytest
 import illume_amd
 import illume_amd.scenes as S
 from oracle.pyoracle import OracleScene
-from conftest import compare_hits, random_rays
+from conftest import camera_rays, compare_hits, random_rays
 
 DEPTH_CAP_PLUS_MEDIAN = 64
 
@@ -227,3 +227,28 @@ def test_bvh_cache_rejects_entries_with_a_matching_hash_but_a_bad_tree(tmp_path)
             assert np.array_equal(a, b), edit.__name__
         assert got[3] == fresh[3]
         assert bvh_device_layouts(m["positions"], m["indices"], cache_dir=str(tmp_path))[4]
+
+
+def test_tpruned_traversal_same_hits_fewer_tests():
+    """The oracle's second traversal (the build's order: children tested together, nearer first,
+    subtrees beyond the closest hit skipped), used for the t-pruned CPU baseline figure, finds
+    the reference walk's closest hits (ties aside) with fewer box and triangle tests, and renders
+    the same image within the parity bar."""
+    from illume_amd._abi import render_params
+    spec, st = S.gold_spheres(n=24)
+    orc = OracleScene(spec)
+    rays = np.concatenate([camera_rays(spec, st, 20000, seed=5), random_rays([-2, 0, -2], [2, 2, 2], 20000, seed=6)])
+    ref = orc.closest_hit(rays, counts=True)
+    orc.set_tpruned(True)
+    got = orc.closest_hit(rays, counts=True)
+    exact, tie, bad = compare_hits(got[:3], ref[:3])
+    assert len(bad) == 0
+    assert got[3][:, 0].sum() < ref[3][:, 0].sum() and got[3][:, 1].sum() < ref[3][:, 1].sum()
+    p = render_params(**dict(st, width=48, height=32, spp=4, max_depth=6))
+    a, _, segs_t, _ = orc.render(p)
+    orc.set_tpruned(False)
+    b, _, segs_r, _ = orc.render(p)
+    orc.free()
+    assert segs_t == segs_r
+    rel = np.abs(a - b) / np.maximum(np.abs(b), 1e-6)
+    assert np.mean(np.all(rel <= 1e-4, axis=1)) >= 0.999

@@ -1,6 +1,7 @@
 """The `illume` command line (src/main.cu:12-43): renders a reference-schema scene JSON and saves
-<save>-<W>x<H>-<spp>spp-<maxdepth>md.png. The PNG's pixels must equal, byte for byte, a render of
-the same scene through the Python API with the scene's own render_settings."""
+<save>-<W>x<H>-<spp>spp-<maxdepth>md.png (and .pfm). The PNG's pixels must equal, byte for byte, a render of
+the same scene through the Python API with the scene's own render_settings; the .pfm written
+beside it must hold the API's accumulated radiance divided by spp, bit for bit."""
 import json
 import os
 import struct
@@ -83,6 +84,17 @@ def _read_png(path):
     return raw[:, 1:].reshape(h, w, 3)
 
 
+def _read_pfm(path):
+    """Float RGB PFM (top row first in the returned array)."""
+    data = open(path, "rb").read()
+    parts = data.split(b"\n", 3)
+    assert parts[0] == b"PF"
+    w, h = (int(x) for x in parts[1].split())
+    scale = float(parts[2])
+    px = np.frombuffer(parts[3], "<f4" if scale < 0 else ">f4").reshape(h, w, 3)
+    return px[::-1].astype(np.float32)
+
+
 @pytest.mark.parametrize("width,height,spp,max_depth", [(48, 32, 4, 5), (37, 23, 1, 2)])
 def test_cli_png_equals_api_render(tmp_path, width, height, spp, max_depth):
     assert os.path.exists(CLI), "illume CLI not built"
@@ -95,16 +107,23 @@ def test_cli_png_equals_api_render(tmp_path, width, height, spp, max_depth):
     assert "Rays per sec:" in r.stdout
     png = _read_png(str(name))
 
+    pfm_name = tmp_path / ("out-%dx%d-%dspp-%dmd.pfm" % (width, height, spp, max_depth))
+    assert "Saved to: %s" % pfm_name.name in r.stdout
+    pfm = _read_pfm(str(pfm_name))
+
     js = illume_amd.load_json(scene)
     p = js.settings()
     p.spp, p.max_depth, p.first_sample = spp, max_depth, 0
     with illume_amd.Context(0) as ctx:
-        _, pix, info = ctx.render(js.upload(ctx), p, want_accum=False)
+        acc, pix, info = ctx.render(js.upload(ctx), p)
     assert info["segments"] > 0
     assert pix.max() <= 255
     assert png.shape == (height, width, 3)
     assert np.array_equal(png, pix.astype(np.uint8))
     assert png.std() > 0   # not a blank frame
+    # the PFM holds the mean radiance: the API's sums over spp, in float
+    assert pfm.shape == (height, width, 3)
+    assert np.array_equal(pfm, acc / np.float32(spp))
 
 
 def test_cli_usage_and_errors(tmp_path):

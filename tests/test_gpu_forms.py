@@ -54,6 +54,9 @@ def test_graph_replays_match_direct_launches(scenes):
                                    dict(extend_mode=1), dict(extend_mode=2, refill_min=4), dict(extend_mode=3, refill_min=1),
                                    dict(extend_mode=3, refill_min=48, sample_group=16),
                                    dict(extend_mode=3, leaf_wait_pct=10), dict(extend_mode=3, leaf_wait_pct=5000),
+                                   dict(extend_mode=4), dict(extend_mode=4, leaf_wait_pct=10, refill_min=1),
+                                   dict(extend_mode=4, leaf_wait_pct=5000, bvh_layout="quant8"),
+                                   dict(extend_mode=4, camera_packets=2),
                                    dict(shade_sort=2), dict(shade_sort=1, paths_in_flight=5000),
                                    dict(camera_packets=2), dict(camera_packets=1, sample_group=1, tile_rows=1),
                                    dict(sched_fair=1, paths_in_flight=3000)])
@@ -168,12 +171,29 @@ def test_cancel_and_admission_limits(scenes):
         s.free()
 
 
+def _kahan_f32(samples):
+    """Kahan sum over axis 0 in float32, in order (the device's accumulation, no fused ops)."""
+    s = np.zeros(samples.shape[1:], np.float32)
+    c = np.zeros_like(s)
+    for x in samples.astype(np.float32):
+        y = (x - c).astype(np.float32)
+        t = (s + y).astype(np.float32)
+        c = ((t - s).astype(np.float32) - y).astype(np.float32)
+        s = t
+    return s
+
+
 def test_first_sample_offsets_compose(ctx, scenes):
+    """Pass k of a render seeds wang_hash(first_sample + k): one-sample renders at first_sample
+    = 0..5 give each pass's radiance exactly (a Kahan sum of one value is that value), and their
+    float32 Kahan sum in pass order equals the 6-sample render bit for bit; a render of passes
+    2..5 equals the Kahan sum of those four."""
     spec, st, s = scenes["cornell"]
     full, _, _ = ctx.render(s, _p(st, width=32, height=32, spp=6))
-    a, _, _ = ctx.render(s, _p(st, width=32, height=32, spp=2))
-    b, _, _ = ctx.render(s, _p(st, width=32, height=32, spp=4, first_sample=2))
-    assert np.allclose(full, a + b, rtol=1e-5, atol=1e-6)
+    one = np.stack([ctx.render(s, _p(st, width=32, height=32, spp=1, first_sample=k))[0] for k in range(6)])
+    assert np.array_equal(full, _kahan_f32(one))
+    tail, _, _ = ctx.render(s, _p(st, width=32, height=32, spp=4, first_sample=2))
+    assert np.array_equal(tail, _kahan_f32(one[2:]))
 
 
 def test_layout_switch_by_tune_matches_a_context_opened_with_it(ctx_quant, scenes):
@@ -217,7 +237,8 @@ def test_long_queues_match_across_reservations_and_oracle(scenes):
     spec, st, _ = scenes["gold"]
     p = _p(st, width=640, height=480, spp=8, max_depth=6)
     shapes = [dict(ext_fetch=64), dict(ext_fetch=64, camera_packets=2), dict(ext_fetch=64, extend_mode=1),
-              dict(ext_fetch=64, extend_mode=2, refill_min=4), dict(ext_fetch=4096)]
+              dict(ext_fetch=64, extend_mode=2, refill_min=4), dict(ext_fetch=4096),
+              dict(ext_fetch=64, extend_mode=4, camera_packets=2)]
     outs = []
     for kw in shapes:
         with illume_amd.Context(0, bvh_layout="quant8", **kw) as c:
@@ -236,3 +257,45 @@ def test_long_queues_match_across_reservations_and_oracle(scenes):
     print("640x480x8: segments gpu %d oracle %d, pixels within 1e-4 %.5f" % (outs[0][2], osegs, np.mean(rel <= 1e-4)))
     assert outs[0][2] == osegs
     assert np.mean(rel <= 1e-4) >= 0.999
+
+
+def test_stats_poll_and_submit_do_not_wait_for_the_chunk_in_flight():
+    """ilm_stats_get reads host-side counters and the scheduler never waits on the device while
+    holding the scheduling lock, so a telemetry poll, a status poll and a submission return
+    promptly while a long request occupies the device (subsurface scene, 1024x1024 x 512 spp:
+    chunks of 2^26 paths that take several hundred ms each)."""
+    spec, st = S.subsurface(env_size=(256, 128))
+    with illume_amd.Context(0) as c:
+        s = c.scene(spec)
+        long = c.submit(s, _p(st, width=1024, height=1024, spp=512))
+        for _ in range(5000):
+            if long.poll()["state"] == "running":
+                break
+            time.sleep(0.001)
+        time.sleep(0.3)   # chunks are in flight now
+        worst = {}
+        small = None
+        for k in range(5):
+            t0 = time.perf_counter()
+            c.stats()
+            worst["stats"] = max(worst.get("stats", 0.0), time.perf_counter() - t0)
+            t0 = time.perf_counter()
+            long.poll()
+            worst["poll"] = max(worst.get("poll", 0.0), time.perf_counter() - t0)
+            if small is None:
+                t0 = time.perf_counter()
+                small = c.submit(s, _p(st, width=32, height=32, spp=1))
+                worst["submit"] = time.perf_counter() - t0
+            time.sleep(0.05)
+        assert long.poll()["state"] == "running"
+        print("while a long request runs: worst stats %.2f ms, poll %.2f ms, submit %.2f ms"
+              % (worst["stats"] * 1e3, worst["poll"] * 1e3, worst["submit"] * 1e3))
+        assert max(worst.values()) < 0.05, worst
+        long.cancel()
+        for r in (long, small):
+            try:
+                r.wait(120000)
+            except illume_amd.IlmError as ex:
+                assert ex.code == -7
+            r.free()
+        s.free()

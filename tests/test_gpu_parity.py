@@ -79,7 +79,7 @@ def test_closest_hit_gold_100k(ctx, gold):
     assert t_exact
 
 
-@pytest.mark.parametrize("mode", [1, 2, 3])
+@pytest.mark.parametrize("mode", [1, 2, 3, 4])
 def test_closest_hit_refill_extend_modes(gold, mode):
     import illume_amd
     spec, st, _, orc = gold
@@ -223,7 +223,8 @@ def test_subsurface_render_independent_of_launch_shape(subsurface):
     spec, st, _, _ = subsurface
     p = _params(st, width=96, height=72, spp=4)
     outs = []
-    for opts in [dict(extend_mode=1, shade_sort=2, sample_group=1, tile_rows=1), dict(), dict(extend_mode=2, ext_fetch=128)]:
+    for opts in [dict(extend_mode=1, shade_sort=2, sample_group=1, tile_rows=1), dict(), dict(extend_mode=2, ext_fetch=128),
+                 dict(extend_mode=4), dict(extend_mode=4, bvh_layout="quant8")]:
         with illume_amd.Context(0, **opts) as c:
             s = c.scene(spec)
             outs.append(c.render(s, p))
@@ -247,3 +248,21 @@ def test_render_cornell_small_matches_oracle(ctx, cornell):
           % (same.mean(), rel.max(), info["segments"], osegs, np.all(px == opx, axis=2).mean()))
     assert info["segments"] == osegs
     assert np.mean(rel <= 1e-4) >= 0.999
+
+
+@pytest.mark.parametrize("op", [0, 1, 2, 3], ids=["linear", "reinhard", "filmic", "uncharted"])
+def test_device_tonemap_matches_oracle(ctx, cornell, op):
+    """k_tonemap (src/renderer.cu:426-441, src/tonemapper.cu:42-76) for every operator: on the
+    device's own accumulated sums its pixels equal the oracle's tonemap of those sums bit for bit;
+    against the whole oracle pipeline >= 99.9 % of pixels are equal."""
+    from oracle.pyoracle import tonemap_image
+    spec, st, sc, orc = cornell
+    p = _params(st, width=48, height=40, spp=32, tonemap=op)
+    acc, px, _ = ctx.render(sc, p)
+    mine = tonemap_image(acc, p.spp, op).reshape(px.shape)
+    assert np.array_equal(px, mine)
+    _, opx, _, _ = orc.render(p, kahan=True)
+    eq = np.all(px.reshape(-1, 3) == opx, axis=1).mean()
+    lit = int((px.reshape(-1, 3).max(axis=1) > 0).sum())
+    print("tonemap op %d: device pixels equal to the oracle pipeline's %.4f (%d lit pixels)" % (op, eq, lit))
+    assert eq >= 0.999 and lit > 0.3 * px.shape[0] * px.shape[1]

@@ -137,3 +137,29 @@ def test_loader_reads_the_reference_cornell_scene():
     assert d.num_meshes == 2 and d.num_instances == 8
     assert d.meshes[0].num_triangles == 12 and d.meshes[1].num_triangles == 2
     assert d.materials[3].type == 0   # "light" emissive
+
+
+def test_loader_reads_generator_obj_exactly(tmp_path):
+    """A generated mesh written as OBJ (shortest round-trip float text) reads back with the same
+    triangles bit for bit; vertices are renumbered in first-occurrence order (as
+    src/primitives/mesh.cu:264-340 merges them), so positions are compared per triangle. With
+    vertex normals (f a//a ...) the smooth-normal path reads normalised normals."""
+    import illume_amd.scenes as S
+    m = S.displaced_icosphere(30, 5)
+    S.write_obj(m, str(tmp_path / "m.obj"))
+    (tmp_path / "s.json").write_text(json.dumps({"meshes": [{"name": "m", "file": "m.obj"}], "camera": {"fov": 40}}))
+    js = illume_amd.load_json(str(tmp_path / "s.json"))
+    d = js.desc.meshes[0]
+    pos = np.ctypeslib.as_array(d.positions, shape=(d.num_vertices * 3,)).reshape(-1, 3)
+    idx = np.ctypeslib.as_array(d.indices, shape=(d.num_triangles * 3,)).reshape(-1, 3)
+    assert d.num_vertices == len(m["positions"]) and d.num_triangles == len(m["indices"])
+    assert np.array_equal(pos[idx], m["positions"][m["indices"]])
+    parse_s, merge_s = js.load_times()
+    assert parse_s > 0 and merge_s > 0
+    nm = dict(m, normals=m["positions"] / np.linalg.norm(m["positions"], axis=1, keepdims=True) * 2.0)
+    S.write_obj(nm, str(tmp_path / "n.obj"), normals=True)
+    (tmp_path / "n.json").write_text(json.dumps({"meshes": [{"name": "n", "file": "n.obj", "face_normals": False}],
+                                                 "camera": {"fov": 40}}))
+    d = illume_amd.load_json(str(tmp_path / "n.json")).desc.meshes[0]
+    nrm = np.ctypeslib.as_array(d.normals, shape=(d.num_vertices * 3,)).reshape(-1, 3)
+    assert d.face_normals == 0 and np.allclose(np.linalg.norm(nrm, axis=1), 1.0, atol=1e-6)
