@@ -53,6 +53,7 @@ def lib():
         "ilm_desc_load_times": (C.c_int, [vp, P(C.c_double), P(C.c_double)]),
         "ilm_loader_error": (C.c_char_p, []),
         "ilm_render": (C.c_int, [vp, vp, P(RenderParams), P(C.c_float), P(Pixel), P(RenderInfo)]),
+        "ilm_render_noalloc": (C.c_int, [vp, vp, P(RenderParams), P(C.c_float), P(Pixel), P(RenderInfo)]),
         "ilm_render_batch": (C.c_int, [vp, C.c_int32, P(vp), P(RenderParams), P(P(C.c_float)), P(P(Pixel)), P(RenderInfo)]),
         "ilm_submit": (C.c_int, [vp, vp, P(RenderParams), P(vp)]),
         "ilm_poll": (C.c_int, [vp, P(ReqStatus)]),
@@ -172,6 +173,18 @@ class Context:
         info = RenderInfo()
         _check(lib().ilm_render(self.h, scene.h, C.byref(params), _fp(acc) if acc is not None else None,
                                 _pxp(pix), C.byref(info)), self.h)
+        return (acc.reshape(H, W, 3) if acc is not None else None), \
+            (pix.reshape(H, W, 3) if pix is not None else None), struct_to_dict(info)
+
+    def render_noalloc(self, scene, params, want_accum=True, want_pixels=True):
+        """ilm_render_noalloc: the plain render with every device buffer carved from the library's
+        static scratch array (no device allocation). Same return values as render()."""
+        W, H = params.width, params.height
+        acc = np.empty((H * W, 3), dtype=np.float32) if want_accum else None
+        pix = np.empty((H * W, 3), dtype=np.uint64) if want_pixels else None
+        info = RenderInfo()
+        _check(lib().ilm_render_noalloc(self.h, scene.h, C.byref(params), _fp(acc) if acc is not None else None,
+                                        _pxp(pix), C.byref(info)), self.h)
         return (acc.reshape(H, W, 3) if acc is not None else None), \
             (pix.reshape(H, W, 3) if pix is not None else None), struct_to_dict(info)
 

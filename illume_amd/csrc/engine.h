@@ -95,7 +95,8 @@ struct ProfileSink {
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
 };
 
-/* Device buffers one chunk pipeline runs on (the context's own, allocated at ilm_open). */
+/* Device buffers one chunk pipeline runs on: the context's own (allocated at ilm_open), or those
+ * of ilm_render_noalloc, carved from its static device array. */
 struct Pipe {
     DPaths A, B;                    /* path state, ping-ponged across depths */
     DHits hits;
@@ -167,6 +168,10 @@ struct ilm_ctx {
     uint64_t stat_segments = 0;
     unsigned long long* h_seg_ring = nullptr;   /* ILM_RING slots */
     int seg_ring_pos = 0;
+    /* ilm_render_noalloc: this device's address of the static scratch array (from
+     * hipGetSymbolAddress at ilm_open) and the timing events of its renders */
+    void* noalloc_scratch = nullptr;
+    hipEvent_t na_ev[2] = {};
     uint64_t stat_bvh_hits = 0, stat_bvh_misses = 0;
     std::string bvh_cache_dir;      /* empty: no BVH cache */
     double stat_busy_ms = 0.0;

@@ -34,6 +34,38 @@ using namespace ilm;
 
 static thread_local std::string g_err;
 
+/* ======================= no-allocation scratch ======================= */
+
+/* Device memory of ilm_render_noalloc: path state, hits, per-path results, control block, chunk
+ * descriptor, traversal-stack spill area, Kahan sums and pixels of one render, all carved from
+ * this array. Each GPU the library runs on holds its own copy until the process ends. In host
+ * code the name is a host-side stand-in, never a device address: kernels and copies use the
+ * address hipGetSymbolAddress gives for the current device (noalloc_scratch_of). */
+static __device__ __align__(256) unsigned char scratch[ILM_NOALLOC_SCRATCH_BYTES];
+
+#define ILM_MAX_DEVICES 64
+static std::mutex g_scratch_mu;                     /* g_scratch_ptr */
+static void* g_scratch_ptr[ILM_MAX_DEVICES];
+/* held for the whole of an ilm_render_noalloc call: calls on one GPU share its array */
+static std::mutex g_noalloc_mu[ILM_MAX_DEVICES];
+
+/* This device's address of the scratch array, looked up once per device with the device
+ * current (the caller has made it current). */
+static hipError_t noalloc_scratch_of(int device, void** out) {
+    *out = nullptr;
+    if (device < 0 || device >= ILM_MAX_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void* ptr = nullptr;
+        hipError_t e = hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        if (!ptr) return hipErrorInvalidSymbol;
+        g_scratch_ptr[device] = ptr;
+    }
+    *out = g_scratch_ptr[device];
+    return hipSuccess;
+}
+
 const char* ilm_global_error(void) { return g_err.c_str(); }
 int ilm_version(void) { return ILM_ABI_VERSION; }
 
@@ -503,6 +535,8 @@ int ilm_open(int device, const ilm_opts* opts, ilm_ctx** out) {
     }
     HIPTRY(ctx, hipHostMalloc((void**)&ctx->h_u64, 4 * sizeof(unsigned long long), hipHostMallocDefault));
     HIPTRY(ctx, hipHostMalloc((void**)&ctx->h_seg_ring, ILM_RING * sizeof(unsigned long long), hipHostMallocDefault));
+    for (int i = 0; i < 2; i++) HIPTRY(ctx, hipEventCreate(&ctx->na_ev[i]));
+    HIPTRY(ctx, noalloc_scratch_of(device, &ctx->noalloc_scratch));
     size_t free_b = 0, total_b = 0;
     HIPTRY(ctx, hipMemGetInfo(&free_b, &total_b));
     ctx->budget = ctx->opts.hbm_budget_bytes > 0 ? ctx->opts.hbm_budget_bytes : (int64_t)(0.9 * (double)free_b);
@@ -528,6 +562,8 @@ static void ctx_destroy(ilm_ctx* ctx) {
     }
     if (ctx->h_u64) (void)hipHostFree(ctx->h_u64);
     if (ctx->h_seg_ring) (void)hipHostFree(ctx->h_seg_ring);
+    for (int i = 0; i < 2; i++)
+        if (ctx->na_ev[i]) (void)hipEventDestroy(ctx->na_ev[i]);
     (void)hipStreamDestroy(ctx->stream);
     (void)hipGetLastError();
     delete ctx;
@@ -1077,6 +1113,147 @@ int ilm_render_batch(ilm_ctx* ctx, int32_t n, ilm_scene* const* scenes, const il
         for (Job* j : grp) job_release_device(j);
     }
     return rc;
+}
+
+/* ======================= render without allocation ======================= */
+
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+/* bytes per path of the pipeline buffers: two copies of o, d, c (float4) and rng (uint2), the
+ * hit record (float4 + int) and the per-path result (float4) */
+#define ILM_PATH_BYTES (2 * (3 * sizeof(float4) + sizeof(uint2)) + sizeof(float4) + sizeof(int) + sizeof(float4))
+/* spill area of a scene whose stack bound exceeds the LDS stack: at most this many extend
+ * workgroups (one wave each) run when the scratch array holds it */
+#define ILM_NOALLOC_SPILL_BLOCKS 1024
+
+/* The scratch array's carving for one render: fixed blocks, the spill area (spill_blocks > 0
+ * when the scene needs it), Kahan sums and compensations (2*W*H float4), pixels, then path
+ * state for cap paths. Returns false when it does not fit with at least 4096 paths. */
+struct NoallocLayout {
+    size_t off_ctl, off_chunk, off_seg, off_ovf, off_accum, off_px, off_paths;
+    int spill_blocks;
+    int64_t cap;
+};
+static bool noalloc_layout(const ilm_ctx* ctx, int64_t wh, bool spill, NoallocLayout& L) {
+    size_t o = 0;
+    L.off_ctl = o;    o = align256(o + sizeof(DCtl));
+    L.off_chunk = o;  o = align256(o + sizeof(DChunk));
+    L.off_seg = o;    o = align256(o + sizeof(float4));
+    L.off_ovf = o;
+    L.spill_blocks = spill ? std::min(ctx->ext_grid, ILM_NOALLOC_SPILL_BLOCKS) : 0;
+    o = align256(o + (size_t)L.spill_blocks * ilm_k_ext_block() * ilm_k_ovf_stack() * sizeof(int));
+    L.off_accum = o;  o = align256(o + 2 * (size_t)wh * sizeof(float4));
+    L.off_px = o;     o = align256(o + (size_t)wh * sizeof(ilm_pixel));
+    L.off_paths = o;
+    if (o >= (size_t)ILM_NOALLOC_SCRATCH_BYTES) return false;
+    /* 9 arrays, each rounded up to 256 B */
+    const size_t room = (size_t)ILM_NOALLOC_SCRATCH_BYTES - o;
+    if (room < 9 * 256) return false;
+    int64_t cap = (int64_t)((room - 9 * 256) / ILM_PATH_BYTES);
+    cap = std::min<int64_t>(cap & ~(int64_t)63, cap_of(ctx->opts));
+    L.cap = cap;
+    return cap >= 4096;
+}
+
+int ilm_render_noalloc(ilm_ctx* ctx, ilm_scene* scene, const ilm_render_params* params, float* accum_rgb,
+                       ilm_pixel* out, ilm_render_info* info) {
+    if (!ctx || !scene) return fail(ctx, ILM_E_INVALID, "null context or scene");
+    int rc = validate_params(ctx, params);
+    if (!rc) rc = check_scene_stack(ctx, scene);
+    if (rc) return rc;
+    if (!ctx->noalloc_scratch) return fail(ctx, ILM_E_DEVICE, "no scratch address for this device");
+    const int64_t wh = (int64_t)params->width * params->height;
+    const bool spill = scene_needs_ovf(ctx, scene);
+    NoallocLayout L;
+    if (!noalloc_layout(ctx, wh, spill, L))
+        return fail(ctx, ILM_E_UNSUPPORTED,
+                    "image too large for ilm_render_noalloc: " + std::to_string(wh) + " pixels x 56 B plus 4096 paths x " +
+                        std::to_string((int)ILM_PATH_BYTES) + " B" + (spill ? " and the traversal-stack spill area" : "") +
+                        " exceed the " + std::to_string((long long)ILM_NOALLOC_SCRATCH_BYTES) + "-byte scratch array");
+    std::lock_guard<std::mutex> na(g_noalloc_mu[ctx->device]);
+    std::lock_guard<std::mutex> dev(ctx->dev_mu);
+    HIPTRY(ctx, hipSetDevice(ctx->device));
+    unsigned char* base = (unsigned char*)ctx->noalloc_scratch;
+    Pipe q;
+    q.ctl = (DCtl*)(base + L.off_ctl);
+    q.chunk = (DChunk*)(base + L.off_chunk);
+    q.seg_total = (unsigned long long*)(base + L.off_seg);
+    q.ovf = spill ? (int*)(base + L.off_ovf) : nullptr;
+    q.ext_grid = spill ? L.spill_blocks : ctx->ext_grid;
+    {
+        unsigned char* pp = base + L.off_paths;
+        auto take = [&](size_t bytes) { void* r = pp; pp += align256(bytes); return r; };
+        const size_t n = (size_t)L.cap;
+        for (DPaths* D : {&q.A, &q.B}) {
+            D->o = (float4*)take(n * sizeof(float4));
+            D->d = (float4*)take(n * sizeof(float4));
+            D->c = (float4*)take(n * sizeof(float4));
+            D->rng = (uint2*)take(n * sizeof(uint2));
+        }
+        q.hits.h = (float4*)take(n * sizeof(float4));
+        q.hits.inst = (int*)take(n * sizeof(int));
+        q.contrib = (float4*)take(n * sizeof(float4));
+        if (pp > base + (size_t)ILM_NOALLOC_SCRATCH_BYTES) return fail(ctx, ILM_E_DEVICE, "scratch carving overflow");
+    }
+    /* the array keeps the previous call's contents: the Kahan buffer and the segment counter
+     * start at zero; the control block is cleared by every chunk's first kernel */
+    float4* d_accum = (float4*)(base + L.off_accum);
+    ilm_pixel* d_px = (ilm_pixel*)(base + L.off_px);
+    HIPTRY(ctx, ilm_k_fill_zero4(d_accum, 2 * (size_t)wh, ctx->stream));
+    HIPTRY(ctx, ilm_k_fill_zero4((float4*)q.seg_total, 1, ctx->stream));
+    Job j;
+    j.scene = scene;
+    j.p = *params;
+    j.d_accum = d_accum;
+    job_fill_req(&j);
+    HIPTRY(ctx, hipEventRecord(ctx->na_ev[0], ctx->stream));
+    int chunks = 0;
+    int64_t max_paths = 0;
+    for (;;) {
+        /* the descriptor is built in the pinned staging slot itself (no host allocation) */
+        const int slot = ctx->ring_pos++ % ILM_RING;
+        HIPTRY(ctx, hipEventSynchronize(ctx->ring_ev[slot]));
+        DChunk& hc = *ctx->h_chunk[slot];
+        chunk_reset(hc);
+        if (!job_has_work(&j)) break;
+        add_slice(ctx, &j, L.cap, 0, hc);
+        hc.work[0].pass0 += params->first_sample;
+        HIPTRY(ctx, hipMemcpyAsync(q.chunk, &hc, sizeof(DChunk), hipMemcpyHostToDevice, ctx->stream));
+        HIPTRY(ctx, hipEventRecord(ctx->ring_ev[slot], ctx->stream));
+        KParams P = base_params(ctx, q);
+        P.trav_stats = nullptr;
+        P.use_ovf = spill ? 1 : 0;
+        P.uni_scene = 1;
+        rc = enqueue_pipeline(ctx, q, P, hc.depth_steps, nullptr);
+        if (rc) return rc;
+        chunks++;
+        max_paths = std::max<int64_t>(max_paths, hc.total_paths);
+    }
+    HIPTRY(ctx, ilm_k_tonemap(d_accum, d_px, (int)wh, params->spp, params->tonemap, ctx->stream));
+    HIPTRY(ctx, hipEventRecord(ctx->na_ev[1], ctx->stream));
+    /* packed RGB into the compensation half of the Kahan buffer, no longer needed */
+    float* d_rgb = (float*)(d_accum + wh);
+    if (accum_rgb) HIPTRY(ctx, ilm_k_sums_to_rgb(d_accum, d_rgb, (int)wh, ctx->stream));
+    unsigned long long segs = 0;
+    HIPTRY(ctx, hipMemcpyAsync(&segs, q.seg_total, sizeof(segs), hipMemcpyDeviceToHost, ctx->stream));
+    if (accum_rgb) HIPTRY(ctx, hipMemcpyAsync(accum_rgb, d_rgb, (size_t)wh * 3 * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    if (out) HIPTRY(ctx, hipMemcpyAsync(out, d_px, (size_t)wh * sizeof(ilm_pixel), hipMemcpyDeviceToHost, ctx->stream));
+    HIPTRY(ctx, hipStreamSynchronize(ctx->stream));
+    float ms = 0.0f;
+    HIPTRY(ctx, hipEventElapsedTime(&ms, ctx->na_ev[0], ctx->na_ev[1]));
+    if (info) {
+        std::memset(info, 0, sizeof(*info));
+        info->segments = segs;
+        info->device_ms = ms;
+        info->chunks = chunks;
+        info->paths_per_chunk = max_paths;
+    }
+    {
+        std::lock_guard<std::mutex> lk(ctx->stat_mu);
+        ctx->stat_busy_ms += ms;
+        ctx->stat_segments += segs;
+    }
+    return ILM_OK;
 }
 
 /* ======================= parity hooks ======================= */

@@ -594,6 +594,17 @@ __global__ __launch_bounds__(256) void k_tonemap(const float4* accum, ilm_pixel*
     gpw(out)[i] = px;
 }
 
+/* Kahan sums (float4 per pixel) to packed RGB floats, the layout of ilm_render's accum_rgb. */
+__global__ __launch_bounds__(256) void k_sums_to_rgb(const float4* sums, float* rgb, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 a = gp(sums)[i];
+    GLOBAL float* o = gpw(rgb) + 3 * (size_t)i;
+    o[0] = a.x;
+    o[1] = a.y;
+    o[2] = a.z;
+}
+
 __global__ void k_fill_zero4(float4* p, size_t n) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
         gpw(p)[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -688,6 +699,11 @@ hipError_t ilm_k_chunk_end(const KParams& P, unsigned long long* seg_total, hipS
 hipError_t ilm_k_tonemap(const float4* accum, ilm_pixel* out, int n, int spp, int op, hipStream_t s) {
     if (n <= 0) return hipSuccess;
     ILM_LAUNCH(k_tonemap, dim3((n + 255) / 256), dim3(256), 0, s, accum, out, n, spp, op);
+    return hipGetLastError();
+}
+hipError_t ilm_k_sums_to_rgb(const float4* sums, float* rgb, int n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    ILM_LAUNCH(k_sums_to_rgb, dim3((n + 255) / 256), dim3(256), 0, s, sums, rgb, n);
     return hipGetLastError();
 }
 hipError_t ilm_k_fill_zero4(float4* p, size_t n, hipStream_t s) {

@@ -22,6 +22,8 @@
  *   ilm_debug_closest_hit                get_min_hit                         src/renderer.cu:112-137
  *   ilm_debug_path_radiance              init_rays + pathtrace_kernel        src/renderer.cu:81-110,139-388
  *   ilm_submit .. ilm_stats, ilm_render_batch   no reference counterpart (served and batched forms)
+ *   ilm_render_noalloc                   no reference counterpart (ilm_render for callers that
+ *                                        cannot allocate device memory)
  */
 #ifndef ILLUME_C_H
 #define ILLUME_C_H
@@ -315,6 +317,27 @@ const char* ilm_loader_error(void);
  * out: W*H pixels after tonemapping, as src/renderer.cu:426-441 (nullable). */
 int ilm_render(ilm_ctx* ctx, ilm_scene* scene, const ilm_render_params* params,
                float* accum_rgb, ilm_pixel* out, ilm_render_info* info);
+
+/* ---- plain render without allocation ----
+ * No counterpart in the reference. For callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): the same render and
+ * outputs as ilm_render (bit-identical accum_rgb and pixels, same segment count), but every
+ * device buffer of the call - path state, hits, per-path results, control block, chunk
+ * descriptor, traversal-stack spill area, Kahan sums and pixels - is carved from one static
+ * device array of ILM_NOALLOC_SCRATCH_BYTES per GPU, which each GPU the library runs on holds
+ * until the process ends. The call makes no hipMalloc/hipFree and no host allocation, launches
+ * kernels directly (no graph) and synchronises before it returns.
+ * Limit: W*H*56 bytes of pixel buffers plus at least 4096 paths (148 bytes each), and for a
+ * scene whose BVH needs the spilling traversal stack a spill area of up to 28 MB, must fit the
+ * array (up to about 2.38 M pixels, 1.87 M with the spill area); larger images return
+ * ILM_E_UNSUPPORTED. The array keeps the last call's contents (each call clears what it needs);
+ * calls on one GPU share it, so they would race: the library serialises them with a per-device
+ * lock held until the call returns. Paths per chunk are what the array holds (at most the
+ * context's paths_in_flight), so large images run as more chunks than under ilm_render.
+ * info: segments, device_ms, chunks, paths_per_chunk (test counts and extend timing are 0). */
+#define ILM_NOALLOC_SCRATCH_BYTES ((size_t)128 << 20)
+int ilm_render_noalloc(ilm_ctx* ctx, ilm_scene* scene, const ilm_render_params* params,
+                       float* accum_rgb, ilm_pixel* out, ilm_render_info* info);
 
 /* ---- batched render: n requests' paths share every launch of one pipeline ---- */
 int ilm_render_batch(ilm_ctx* ctx, int32_t n, ilm_scene* const* scenes, const ilm_render_params* params,
