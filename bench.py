@@ -10,9 +10,10 @@ Configurations (scene generators in illume_amd/scenes.py, fixed seeds):
      dielectric, 2048x1024 HDR sky + sun, 960x720, 32 spp, depth 45
   3  batched form: 64 requests over configurations 0-2 (side 128..1024, 4..256 spp) in one
      ilm_render_batch call, against the same requests rendered one at a time
-  4  served form: a prefix of the Poisson trace (20 req/s) over configurations 0-2 and a
-     12-instance 722k-triangle mesh, submitted at their arrival times to the scheduler, against
-     plain one-request-at-a-time renders of the same trace
+  4  served form: the first 120 requests of the Poisson trace (20 req/s) over configurations 0-2
+     and a 12-instance 722k-triangle mesh, submitted at their arrival times to the scheduler,
+     against plain one-request-at-a-time renders of the same trace (offered vs achieved load,
+     p50/p99 latency, rejections, image identity, RMSE against the FP64 oracle)
 
 For 0-2 a step is one full render of the workload (all passes and depths) through the C ABI
 (ilm_render: ray generation, BVH extend, shade + compaction per depth, Kahan accumulation,
@@ -444,12 +445,15 @@ def _replay(ctx_or_none, trace, start, submit_fn):
 
 
 def bench_served(args):
-    """Served form vs plain one-request-at-a-time renders on a prefix of the Poisson trace."""
+    """Served form vs plain one-request-at-a-time renders on a prefix of the Poisson trace (120
+    requests by default): throughput, offered vs achieved load, p50/p99 latency (completion -
+    arrival), rejections, image identity, RMSE against the FP64 oracle."""
+    import hashlib
     import illume_amd
     from illume_amd import scenes as S
     from illume_amd._abi import render_params
 
-    trace = S.serve_trace()[:args.requests or 24]
+    trace = S.serve_trace()[:args.requests or 120]
     t0 = trace[0]["t"]
     trace = [dict(r, t=r["t"] - t0) for r in trace]
     specs, sts = {}, {}
@@ -457,36 +461,44 @@ def bench_served(args):
         specs[k], sts[k] = S.MIX_SCENES[k]()
     params = [render_params(**dict(sts[r["scene"]], width=r["width"], height=r["height"], spp=r["spp"],
                                    max_depth=r["max_depth"])) for r in trace]
-    forms = {}
-    accs = {}
+
+    def middle(p, acc):
+        n = min(128, p.width)
+        pb = (p.height // 2) * p.width + (p.width - n) // 2
+        return np.asarray(acc, np.float32).reshape(-1, 3)[pb:pb + n].copy()
+
+    def digest(acc, px):
+        h = hashlib.blake2b(digest_size=16)
+        h.update(np.ascontiguousarray(acc).tobytes())
+        if px is not None:
+            h.update(np.ascontiguousarray(px).tobytes())
+        return h.hexdigest()
+
+    forms, digests, slices = {}, {}, {}
     with illume_amd.Context(0, **ctx_opts(args)) as ctx:
         sc = _upload_mix(ctx, specs)
         # warm-up: capture the graphs of every depth in the trace
         for d in sorted({p.max_depth for p in params}):
             ctx.render(sc[0], render_params(**dict(sts[0], width=64, height=64, spp=1, max_depth=d)))
 
-        def plain_form(name, **tune):
-            saved = ctx.settings()
-            if tune:
-                ctx.tune(**tune)
-            lat, segs = [], 0
-            out = []
+        def plain_form(name):
+            lat, segs, dig, sl = [], 0, [], []
             start = time.perf_counter()
             for r, p in zip(trace, params):
                 dt = start + r["t"] - time.perf_counter()
                 if dt > 0:
                     time.sleep(dt)
-                a, _, i = ctx.render(sc[r["scene"]], p, want_pixels=False)
+                a, px, i = ctx.render(sc[r["scene"]], p)
                 lat.append((time.perf_counter() - start - r["t"]) * 1e3)
                 segs += i["segments"]
-                out.append(a)
+                dig.append(digest(a, px))
+                sl.append(middle(p, a))
             span = time.perf_counter() - start
-            ctx.tune(**saved)
             forms[name] = dict(Mrays_s=segs / span / 1e6, makespan_s=span, segments=segs,
                                p50_ms=percentile(lat, 0.5), p99_ms=percentile(lat, 0.99))
+            digests[name], slices[name] = dig, sl
             log("%s: %.1f Mrays/s, makespan %.2f s, p50 %.0f ms, p99 %.0f ms"
                 % (name, segs / span / 1e6, span, forms[name]["p50_ms"], forms[name]["p99_ms"]))
-            return out
 
         def served_form(name, fair):
             """submissions at arrival times; the scheduler packs whatever is admitted into chunks"""
@@ -496,53 +508,64 @@ def bench_served(args):
             handles = []
             start = time.perf_counter()
             _replay(None, trace, start, lambda r: handles.append(ctx.submit(sc[r["scene"]], params[len(handles)])))
-            out, lat = [], []
-            for h in handles:
+            submitted = time.perf_counter() - start
+            lat, dig, sl = [], [], []
+            for h, p in zip(handles, params):
                 h.wait()
-            span = time.perf_counter() - start
-            for h in handles:
-                a, _ = h.result()
-                out.append(a)
+                a, px = h.result()
                 st = h.poll()
                 lat.append(st["end_ms"] - st["submit_ms"])
+                dig.append(digest(a, px))
+                sl.append(middle(p, a))
                 h.free()
-            segs = ctx.stats()["segments"] - s0["segments"]
+            span = time.perf_counter() - start
+            s1 = ctx.stats()
+            segs = s1["segments"] - s0["segments"]
             ctx.tune(**saved)
-            forms[name] = dict(Mrays_s=segs / span / 1e6, makespan_s=span, segments=segs,
-                               p50_ms=percentile(lat, 0.5), p99_ms=percentile(lat, 0.99))
+            forms[name] = dict(Mrays_s=segs / span / 1e6, makespan_s=span, segments=segs, submit_span_s=submitted,
+                               p50_ms=percentile(lat, 0.5), p99_ms=percentile(lat, 0.99),
+                               rejected=s1["requests_rejected"] - s0["requests_rejected"])
+            digests[name], slices[name] = dig, sl
             log("%s: %.1f Mrays/s, makespan %.2f s, p50 %.0f ms, p99 %.0f ms"
                 % (name, segs / span / 1e6, span, forms[name]["p50_ms"], forms[name]["p99_ms"]))
-            return out
 
-        accs["served"] = served_form("served", 0)
-        accs["served_fair"] = served_form("served_fair", 1)
-        accs["plain"] = plain_form("plain")
-        plain_form("plain_per_pass", chunk_passes=1)
-        same = sum(int(np.array_equal(a, b) and np.array_equal(a, f))
-                   for a, b, f in zip(accs["served"], accs["plain"], accs["served_fair"]))
+        served_form("served", 0)
+        if args.fair:
+            served_form("served_fair", 1)
+        plain_form("plain")
         for s in sc.values():
             s.free()
+    same = sum(int(a == b) for a, b in zip(digests["served"], digests["plain"]))
     rm = None
     if not args.no_rmse:
         from oracle.pyoracle import lib as orc_lib
         orc = Fp64Oracle(specs, orc_lib().orc_threads_available())
-        errs = [orc.rmse(r["scene"], p, a)[0] for r, p, a in zip(trace, params, accs["served"])]
+        errs = []
+        for r, p, sl in zip(trace, params, slices["served"]):
+            n = min(128, p.width)
+            pb = (p.height // 2) * p.width + (p.width - n) // 2
+            ref, _, _, _ = orc.scenes[r["scene"]].render(p, pb, pb + n, kahan=True, fp64=True, threads=orc.threads,
+                                                         pixels=False)
+            errs.append(rel_rmse(sl, ref, p.spp))
         orc.free()
         rm = {"max": round(max(errs), 5), "mean": round(float(np.mean(errs)), 5)}
         log("rmse vs fp64 oracle: %s" % rm)
-    sv = forms["served"]
+    sv, pl = forms["served"], forms["plain"]
+    arrival_span = max(trace[-1]["t"], 1e-9)
+    offered = pl["segments"] / arrival_span / 1e6
     config = {"workload": "config4: first %d requests of the Poisson trace (seed 4, 20 req/s) over configs 0-2 and "
                           "the 12-instance 722k-tri mesh, 256^2..1920x1080, 8..512 spp, depth 8..32" % len(trace),
               "requests": len(trace), "parallelism": "1 GPU"}
     out = base_line(args, "served path segments/s (Mrays/s); p50/p99 request latency", sv["Mrays_s"], "Mrays/s",
                     sv["makespan_s"] * 1e3, config)
-    fa = forms["served_fair"]
     out["latency_ms"] = {"p50": round(sv["p50_ms"], 1), "p99": round(sv["p99_ms"], 1),
                          "form": "served, arrival-order packing (completion - arrival)",
-                         "fair_share": {"p50": round(fa["p50_ms"], 1), "p99": round(fa["p99_ms"], 1)}}
+                         "plain": {"p50": round(pl["p50_ms"], 1), "p99": round(pl["p99_ms"], 1)}}
+    out["load"] = {"arrival_span_s": round(arrival_span, 3), "offered_Mrays_s": round(offered, 1),
+                   "achieved_Mrays_s": round(sv["Mrays_s"], 1), "offered_over_achieved": round(offered / sv["Mrays_s"], 3),
+                   "rejected": sv["rejected"]}
     out["forms"] = {k: {kk: round(vv, 3) for kk, vv in v.items()} for k, v in forms.items()}
-    out["served_vs_plain"] = round(sv["Mrays_s"] / forms["plain"]["Mrays_s"], 3)
-    out["served_vs_plain_per_pass"] = round(sv["Mrays_s"] / forms["plain_per_pass"]["Mrays_s"], 3)
+    out["served_vs_plain"] = round(sv["Mrays_s"] / pl["Mrays_s"], 3)
     out["identical_to_plain"] = "%d/%d" % (same, len(trace))
     out["rmse_vs_fp64"] = rm
     out["roofline"] = None
@@ -622,7 +645,8 @@ def main():
                     help="launch shape to apply before measuring, as key=value[,key=value] over the tunables "
                          "(e.g. ext_fetch=128,leaf_wait_pct=50); with --no-tune this reproduces a tuned run")
     ap.add_argument("--tune-budget", type=float, default=30.0)
-    ap.add_argument("--requests", type=int, default=0, help="configs 3/4: number of requests")
+    ap.add_argument("--requests", type=int, default=0, help="configs 3/4: number of requests (0: 64 / 120)")
+    ap.add_argument("--fair", action="store_true", help="config 4: also run the fair-share packing")
     ap.add_argument("--no-rmse", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-count", action="store_true")

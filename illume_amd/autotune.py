@@ -10,8 +10,6 @@ measures time.
     extend_mode      1: each wave traces one batch of 64 rays to completion
                      2: idle lanes refill from a wave-local queue at BLAS leaves
                      3 (and 0, the default): idle lanes refill after every traversal step
-                     4: as 3, and a lane that reaches a BLAS leaf parks it and walks on until the
-                        wave's next leaf trip (speculative traversal)
     refill_min       idle lanes that trigger a refill (modes 1 and 2)
     ext_fetch        rays an extend wave reserves from the queue at a time (work-queue granularity)
     leaf_wait_pct    per-step refill: lanes waiting at BLAS leaves (as a percentage of the other
@@ -39,7 +37,7 @@ DEFAULT_SPACE = (
     ("bvh_layout", (0, 1)),
     ("sample_group", (1, 4, 16, 64)),
     ("tile_rows", (1, 4, 8)),
-    (("extend_mode", "refill_min"), ((1, 0), (2, 16), (2, 32), (3, 8), (3, 16), (3, 48), (4, 16), (4, 32))),
+    (("extend_mode", "refill_min"), ((1, 0), (2, 16), (2, 32), (3, 8), (3, 16), (3, 48))),
     ("leaf_wait_pct", (50, 100, 150, 200, 400)),
     ("shade_sort", (1, 2)),
     ("camera_packets", (1, 2)),

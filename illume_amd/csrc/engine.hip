@@ -462,7 +462,7 @@ static int ctx_alloc_paths(ilm_ctx* ctx, int64_t cap) {
 static bool opts_valid(const ilm_opts& o) {
     const bool sg_ok = o.sample_group >= 0 && o.sample_group <= 4096 && (o.sample_group & (o.sample_group - 1)) == 0;
     return !(o.bvh_layout < 0 || o.bvh_layout > ILM_BVH_QUANT8 || o.tri_test < 0 || o.tri_test > ILM_TRI_WATERTIGHT ||
-             o.extend_mode < 0 || o.extend_mode > ILM_EXTEND_SPECULATIVE || !sg_ok || o.tile_rows < 0 || o.tile_rows > 256 ||
+             o.extend_mode < 0 || o.extend_mode > ILM_EXTEND_STEP_REFILL || !sg_ok || o.tile_rows < 0 || o.tile_rows > 256 ||
              o.refill_min < 0 || o.refill_min > 64 || o.ext_fetch < 0 || o.ext_fetch > 65536 || (o.ext_fetch & 63) != 0 ||
              o.leaf_wait_pct < 0 || o.leaf_wait_pct > 10000 || o.shade_sort < 0 || o.shade_sort > 2 ||
              o.sched_fair < 0 || o.sched_fair > 1 || o.camera_packets < 0 || o.camera_packets > 2 ||
@@ -717,7 +717,7 @@ static int stage_chunk(ilm_ctx* ctx, const DChunk& hc) { return stage_chunk(ctx,
  * read the rays k_raygen writes out. */
 static hipError_t start_paths(ilm_ctx* ctx, const Pipe& q, KParams& P, hipStream_t s) {
     P.in = q.A;
-    P.cam0 = (P.extend_mode == ILM_EXTEND_STEP_REFILL || P.extend_mode == ILM_EXTEND_SPECULATIVE) ? 1 : 0;
+    P.cam0 = P.extend_mode == ILM_EXTEND_STEP_REFILL ? 1 : 0;
     if (P.cam0 && P.packet0) return hipSuccess;
     return P.cam0 ? ilm_k_path_clear(P, ctx->gen_grid, s) : ilm_k_raygen(P, ctx->gen_grid, s);
 }

@@ -636,23 +636,18 @@ hipError_t ilm_k_path_clear(const KParams& P, int grid, hipStream_t s) {
 
 template <int T, int Q, int C, int O>
 static void launch_extend(const KParams& P, int depth, int* ovf, int grid, hipStream_t s) {
-    const bool per_step = P.extend_mode == ILM_EXTEND_STEP_REFILL || P.extend_mode == ILM_EXTEND_SPECULATIVE;
-    const bool spec = P.extend_mode == ILM_EXTEND_SPECULATIVE;
-    if (per_step && P.cam0 && P.packet0 && depth == 0)
+    if (P.extend_mode == ILM_EXTEND_STEP_REFILL && P.cam0 && P.packet0 && depth == 0)
         ILM_LAUNCH((k_extend_packet<T, Q, C>), dim3(grid), dim3(EXT_BLOCK), 0, s, P, depth);
-    else if (per_step && P.cam0 && depth == 0) {
-        if (spec) ILM_LAUNCH((k_extend_pers<T, Q, C, O, 1, 0, 1>), dim3(grid), dim3(EXT_BLOCK), 0, s, P, depth, ovf);
-        else ILM_LAUNCH((k_extend_pers<T, Q, C, O, 1, 0, 0>), dim3(grid), dim3(EXT_BLOCK), 0, s, P, depth, ovf);
-    } else if (per_step) {
+    else if (P.extend_mode == ILM_EXTEND_STEP_REFILL && P.cam0 && depth == 0)
+        ILM_LAUNCH((k_extend_pers<T, Q, C, O, 1, 0>), dim3(grid), dim3(EXT_BLOCK), 0, s, P, depth, ovf);
+    else if (P.extend_mode == ILM_EXTEND_STEP_REFILL) {
         if constexpr (C == 0) {
             if (P.uni_scene) {
-                if (spec) ILM_LAUNCH((k_extend_pers<T, Q, 0, O, 0, 1, 1>), dim3(grid), dim3(EXT_BLOCK), 0, s, P, depth, ovf);
-                else ILM_LAUNCH((k_extend_pers<T, Q, 0, O, 0, 1, 0>), dim3(grid), dim3(EXT_BLOCK), 0, s, P, depth, ovf);
+                ILM_LAUNCH((k_extend_pers<T, Q, 0, O, 0, 1>), dim3(grid), dim3(EXT_BLOCK), 0, s, P, depth, ovf);
                 return;
             }
         }
-        if (spec) ILM_LAUNCH((k_extend_pers<T, Q, C, O, 0, 0, 1>), dim3(grid), dim3(EXT_BLOCK), 0, s, P, depth, ovf);
-        else ILM_LAUNCH((k_extend_pers<T, Q, C, O, 0, 0, 0>), dim3(grid), dim3(EXT_BLOCK), 0, s, P, depth, ovf);
+        ILM_LAUNCH((k_extend_pers<T, Q, C, O, 0, 0>), dim3(grid), dim3(EXT_BLOCK), 0, s, P, depth, ovf);
     }
     else if (P.extend_mode == ILM_EXTEND_LEAF_REFILL)
         ILM_LAUNCH((k_extend_refill<T, Q, C, O>), dim3(grid), dim3(EXT_BLOCK), 0, s, P, depth, ovf);
@@ -723,15 +718,10 @@ static int occ_of(K kernel) {
 int ilm_k_extend_occupancy(int tri_test, int bvh_layout, int extend_mode) {
     int a, b;
     if (extend_mode == ILM_EXTEND_STEP_REFILL) {
-        if (!tri_test && !bvh_layout) { a = occ_of(k_extend_pers<0, 0, 0, 0, 0, 0, 0>); b = occ_of(k_extend_pers<0, 0, 0, 1, 0, 0, 0>); }
-        else if (tri_test && !bvh_layout) { a = occ_of(k_extend_pers<1, 0, 0, 0, 0, 0, 0>); b = occ_of(k_extend_pers<1, 0, 0, 1, 0, 0, 0>); }
-        else if (!tri_test) { a = occ_of(k_extend_pers<0, 1, 0, 0, 0, 0, 0>); b = occ_of(k_extend_pers<0, 1, 0, 1, 0, 0, 0>); }
-        else { a = occ_of(k_extend_pers<1, 1, 0, 0, 0, 0, 0>); b = occ_of(k_extend_pers<1, 1, 0, 1, 0, 0, 0>); }
-    } else if (extend_mode == ILM_EXTEND_SPECULATIVE) {
-        if (!tri_test && !bvh_layout) { a = occ_of(k_extend_pers<0, 0, 0, 0, 0, 0, 1>); b = occ_of(k_extend_pers<0, 0, 0, 1, 0, 0, 1>); }
-        else if (tri_test && !bvh_layout) { a = occ_of(k_extend_pers<1, 0, 0, 0, 0, 0, 1>); b = occ_of(k_extend_pers<1, 0, 0, 1, 0, 0, 1>); }
-        else if (!tri_test) { a = occ_of(k_extend_pers<0, 1, 0, 0, 0, 0, 1>); b = occ_of(k_extend_pers<0, 1, 0, 1, 0, 0, 1>); }
-        else { a = occ_of(k_extend_pers<1, 1, 0, 0, 0, 0, 1>); b = occ_of(k_extend_pers<1, 1, 0, 1, 0, 0, 1>); }
+        if (!tri_test && !bvh_layout) { a = occ_of(k_extend_pers<0, 0, 0, 0, 0, 0>); b = occ_of(k_extend_pers<0, 0, 0, 1, 0, 0>); }
+        else if (tri_test && !bvh_layout) { a = occ_of(k_extend_pers<1, 0, 0, 0, 0, 0>); b = occ_of(k_extend_pers<1, 0, 0, 1, 0, 0>); }
+        else if (!tri_test) { a = occ_of(k_extend_pers<0, 1, 0, 0, 0, 0>); b = occ_of(k_extend_pers<0, 1, 0, 1, 0, 0>); }
+        else { a = occ_of(k_extend_pers<1, 1, 0, 0, 0, 0>); b = occ_of(k_extend_pers<1, 1, 0, 1, 0, 0>); }
     } else if (extend_mode == ILM_EXTEND_LEAF_REFILL) {
         if (!tri_test && !bvh_layout) { a = occ_of(k_extend_refill<0, 0, 0, 0>); b = occ_of(k_extend_refill<0, 0, 0, 1>); }
         else if (tri_test && !bvh_layout) { a = occ_of(k_extend_refill<1, 0, 0, 0>); b = occ_of(k_extend_refill<1, 0, 0, 1>); }

@@ -203,12 +203,6 @@ struct Stack {
         if (!OVF || sp < LDS_STACK) return lds[sp * EXT_BLOCK];
         return *spill(sp);
     }
-    /* the entry pop() would return (sp > 0) */
-    __device__ inline int top() const {
-        const int s = sp - 1;
-        if (!OVF || s < LDS_STACK) return lds[s * EXT_BLOCK];
-        return *spill(s);
-    }
 };
 
 struct TravCount {
@@ -943,10 +937,8 @@ struct PState {
     int node;
     int cur;
     int blas;
-    int pend;               /* speculative walk: a parked BLAS leaf, or NO_PEND */
     WTRay wt;
 };
-#define NO_PEND ((int)0x7fffffff)
 
 template <int QUANT, int OVF>
 __device__ inline void pstep_init(PState& T, Best& B, Stack<OVF>& st, const GLOBAL DScene* S, V3 O, V3 D, float tmax0) {
@@ -961,7 +953,6 @@ __device__ inline void pstep_init(PState& T, Best& B, Stack<OVF>& st, const GLOB
     T.tris = nullptr;
     T.blas = 0;
     T.cur = -1;
-    T.pend = NO_PEND;
     T.O = O;
     T.D = D;
     T.o = O;
@@ -1032,64 +1023,9 @@ __device__ inline bool pstep(PState& T, Best& B, Stack<OVF>& st, TravCount* tc, 
     return false;
 }
 
-/* ---------------- speculative walk (extend_mode 4) ----------------
- * Per-step refill with speculative traversal (Aila and Laine, HPG 2009): on a node trip a lane
- * whose walk has reached a BLAS leaf parks that leaf and walks on inside the same BLAS, so it
- * visits nodes instead of idling until the wave's next leaf trip; the parked leaf is intersected
- * on a leaf trip. A lane holds one parked leaf and is blocked (waits for a leaf trip) at a second
- * leaf, at the instance boundary (the parked leaf is in this BLAS's object space) and at the end
- * of its stack. The closest hit is unchanged: only the order of the tests changes, and while a
- * leaf is parked the boxes are tested against a closest hit that is not yet lowered by it, so a
- * few more boxes pass. */
-template <int OVF>
-__device__ inline bool spec_blocked(const PState& T, const Stack<OVF>& st) {
-    if (T.pend == NO_PEND) return false;
-    if (T.node < 0 && T.blas) return true;
-    return T.node == POP_MARK && (st.sp == 0 || st.top() == STACK_SENTINEL);
-}
-
-/* A node trip of an unblocked lane; true when the ray is finished. */
-template <int TRI, int QUANT, int COUNT, int OVF, int UNI>
-__device__ inline bool pstep_spec_node(PState& T, Best& B, Stack<OVF>& st, TravCount* tc, const GLOBAL DScene* Su) {
-    int node = T.node;
-    if (node < 0 && T.blas) {            /* reached a leaf with none parked: park it */
-        T.pend = node;
-        node = POP_MARK;
-    }
-    if (T.pend == NO_PEND) return pstep<TRI, QUANT, COUNT, OVF, UNI>(T, B, st, tc, Su);
-    if (node == POP_MARK) {
-        if (st.sp == 0 || st.top() == STACK_SENTINEL) {
-            T.node = POP_MARK;
-            return false;
-        }
-        node = st.pop();
-        if (node < 0) {                  /* a second leaf: wait for a leaf trip */
-            T.node = node;
-            return false;
-        }
-    }
-    /* an inner BLAS node with a leaf parked: one visit; pops stay inside the BLAS */
-    if (!inner_step_od<QUANT, OVF, COUNT>(T.nodes, T.qnodes, node, T.o, T.d, B.t, st, tc)) {
-        if (st.sp == 0 || st.top() == STACK_SENTINEL) node = POP_MARK;
-        else node = st.pop();
-    }
-    T.node = node;
-    return false;
-}
-
-/* A leaf trip: the parked leaf if there is one, else the leaf the walk stands at; true when the
- * ray is finished. */
-template <int TRI, int QUANT, int COUNT, int OVF, int UNI>
-__device__ inline bool pstep_spec_leaf(PState& T, Best& B, Stack<OVF>& st, TravCount* tc, const GLOBAL DScene* Su) {
-    if (T.pend == NO_PEND) return pstep<TRI, QUANT, COUNT, OVF, UNI>(T, B, st, tc, Su);
-    leaf_tris<TRI, COUNT>(T.tris, T.pend, T.o, T.d, T.wt, T.cur, B, tc);
-    T.pend = NO_PEND;
-    return T.node == POP_MARK && st.sp == 0;
-}
-
 /* UNI: every request of the chunk renders the same scene (the host checks), so the scene
  * pointer is read once, uniform over the wave, instead of per lane at every refill. */
-template <int TRI, int QUANT, int COUNT, int OVF, int CAM, int UNI, int SPEC>
+template <int TRI, int QUANT, int COUNT, int OVF, int CAM, int UNI>
 __global__ EXT_BOUNDS void k_extend_pers(KParams P, int depth, int* ovf_base) {
     __shared__ int stk[LDS_STACK * EXT_BLOCK];
     const int lane = threadIdx.x;
@@ -1154,34 +1090,14 @@ __global__ EXT_BOUNDS void k_extend_pers(KParams P, int depth, int* ovf_base) {
          * of the two code paths instead of both. Either group is non-empty when chosen unless
          * no lane is busy, so every trip makes progress. */
         const bool has = ridx != NO_RAY;
-        if (SPEC) {
-            /* speculative walk: a leaf trip once the blocked lanes reach leaf_pct percent of
-             * the others; a blocked lane always has a parked leaf, so either trip progresses */
-            const bool blocked = has && spec_blocked(T, st);
-            const bool leafy = has && (T.pend != NO_PEND || (T.node < 0 && T.blas));
-            const uint32_t n_blocked = (uint32_t)__popcll(__ballot(blocked));
-            const uint32_t n_free = (uint32_t)__popcll(__ballot(has)) - n_blocked;
-            const bool leaf_trip = n_blocked * 100u >= n_free * leaf_pct;
-            bool done = false;
-            if (leaf_trip) {
-                if (leafy) done = pstep_spec_leaf<TRI, QUANT, COUNT, OVF, UNI>(T, B, st, &tc, Su);
-            } else if (has && !blocked) {
-                done = pstep_spec_node<TRI, QUANT, COUNT, OVF, UNI>(T, B, st, &tc, Su);
-            }
-            if (done) {
+        const bool at_leaf = has && T.node < 0 && T.blas;
+        const uint32_t n_leaf = (uint32_t)__popcll(__ballot(at_leaf));
+        const uint32_t n_rest = (uint32_t)__popcll(__ballot(has)) - n_leaf;
+        const bool leaf_trip = n_leaf * 100u >= n_rest * leaf_pct;
+        if (has && at_leaf == leaf_trip) {
+            if (pstep<TRI, QUANT, COUNT, OVF, UNI>(T, B, st, &tc, Su)) {
                 write_hit(P, ridx, B);
                 ridx = NO_RAY;
-            }
-        } else {
-            const bool at_leaf = has && T.node < 0 && T.blas;
-            const uint32_t n_leaf = (uint32_t)__popcll(__ballot(at_leaf));
-            const uint32_t n_rest = (uint32_t)__popcll(__ballot(has)) - n_leaf;
-            const bool leaf_trip = n_leaf * 100u >= n_rest * leaf_pct;
-            if (has && at_leaf == leaf_trip) {
-                if (pstep<TRI, QUANT, COUNT, OVF, UNI>(T, B, st, &tc, Su)) {
-                    write_hit(P, ridx, B);
-                    ridx = NO_RAY;
-                }
             }
         }
     }

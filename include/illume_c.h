@@ -179,11 +179,8 @@ typedef struct ilm_pixel {
 
 /* Extend-kernel scheduling (results are identical for every mode). BATCH: each wave traces 64
  * rays to completion; LEAF_REFILL: idle lanes take new rays when the wave reaches BLAS leaves;
- * STEP_REFILL: idle lanes take new rays after every traversal step (the default); SPECULATIVE:
- * STEP_REFILL where a lane that reaches a BLAS leaf parks it and walks on until the wave's next
- * leaf trip. */
-enum { ILM_EXTEND_DEFAULT = 0, ILM_EXTEND_BATCH = 1, ILM_EXTEND_LEAF_REFILL = 2, ILM_EXTEND_STEP_REFILL = 3,
-       ILM_EXTEND_SPECULATIVE = 4 };
+ * STEP_REFILL: idle lanes take new rays after every traversal step (the default). */
+enum { ILM_EXTEND_DEFAULT = 0, ILM_EXTEND_BATCH = 1, ILM_EXTEND_LEAF_REFILL = 2, ILM_EXTEND_STEP_REFILL = 3 };
 
 typedef struct ilm_opts {
     int32_t bvh_layout;         /* ILM_BVH_* (default FP32) */

@@ -58,7 +58,7 @@ def test_open_fails_cleanly_without_device():
 
 
 @pytest.mark.parametrize("field,value", [("sample_group", 3), ("sample_group", -1), ("tile_rows", -2),
-                                         ("extend_mode", 5), ("refill_min", 65), ("bvh_layout", 7), ("chunk_passes", -1)])
+                                         ("extend_mode", 4), ("refill_min", 65), ("bvh_layout", 7), ("chunk_passes", -1)])
 def test_open_rejects_bad_options(field, value):
     """Option checks run before any device call, so they hold with or without a GPU."""
     import illume_amd

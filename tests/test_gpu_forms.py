@@ -54,9 +54,6 @@ def test_graph_replays_match_direct_launches(scenes):
                                    dict(extend_mode=1), dict(extend_mode=2, refill_min=4), dict(extend_mode=3, refill_min=1),
                                    dict(extend_mode=3, refill_min=48, sample_group=16),
                                    dict(extend_mode=3, leaf_wait_pct=10), dict(extend_mode=3, leaf_wait_pct=5000),
-                                   dict(extend_mode=4), dict(extend_mode=4, leaf_wait_pct=10, refill_min=1),
-                                   dict(extend_mode=4, leaf_wait_pct=5000, bvh_layout="quant8"),
-                                   dict(extend_mode=4, camera_packets=2),
                                    dict(shade_sort=2), dict(shade_sort=1, paths_in_flight=5000),
                                    dict(camera_packets=2), dict(camera_packets=1, sample_group=1, tile_rows=1),
                                    dict(sched_fair=1, paths_in_flight=3000)])
@@ -237,8 +234,7 @@ def test_long_queues_match_across_reservations_and_oracle(scenes):
     spec, st, _ = scenes["gold"]
     p = _p(st, width=640, height=480, spp=8, max_depth=6)
     shapes = [dict(ext_fetch=64), dict(ext_fetch=64, camera_packets=2), dict(ext_fetch=64, extend_mode=1),
-              dict(ext_fetch=64, extend_mode=2, refill_min=4), dict(ext_fetch=4096),
-              dict(ext_fetch=64, extend_mode=4, camera_packets=2)]
+              dict(ext_fetch=64, extend_mode=2, refill_min=4), dict(ext_fetch=4096)]
     outs = []
     for kw in shapes:
         with illume_amd.Context(0, bvh_layout="quant8", **kw) as c:

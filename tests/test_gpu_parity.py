@@ -79,7 +79,7 @@ def test_closest_hit_gold_100k(ctx, gold):
     assert t_exact
 
 
-@pytest.mark.parametrize("mode", [1, 2, 3, 4])
+@pytest.mark.parametrize("mode", [1, 2, 3])
 def test_closest_hit_refill_extend_modes(gold, mode):
     import illume_amd
     spec, st, _, orc = gold
@@ -223,8 +223,7 @@ def test_subsurface_render_independent_of_launch_shape(subsurface):
     spec, st, _, _ = subsurface
     p = _params(st, width=96, height=72, spp=4)
     outs = []
-    for opts in [dict(extend_mode=1, shade_sort=2, sample_group=1, tile_rows=1), dict(), dict(extend_mode=2, ext_fetch=128),
-                 dict(extend_mode=4), dict(extend_mode=4, bvh_layout="quant8")]:
+    for opts in [dict(extend_mode=1, shade_sort=2, sample_group=1, tile_rows=1), dict(), dict(extend_mode=2, ext_fetch=128)]:
         with illume_amd.Context(0, **opts) as c:
             s = c.scene(spec)
             outs.append(c.render(s, p))
