@@ -210,7 +210,8 @@ def base_line(args, metric, value, unit, ms_per_step, config, dtype="f32"):
 
 
 def ctx_opts(args):
-    o = dict(bvh_layout=args.layout, tri_test=args.tri, paths_in_flight=args.paths, extend_mode=args.extend_mode)
+    o = dict(bvh_layout=args.layout, tri_test=args.tri, paths_in_flight=args.paths, extend_mode=args.extend_mode,
+             use_graphs=0 if args.no_graphs else 1)
     if args.sample_group is not None:
         o["sample_group"] = args.sample_group
     if args.tile_rows is not None:
@@ -351,7 +352,7 @@ def bench_plain(args):
     config = {"workload": workload_name(args.config, spec, st), "width": st["width"], "height": st["height"],
               "spp": st["spp"], "max_depth": st["max_depth"], "bvh_layout": layout, "tri_test": args.tri,
               "segments_per_step": segs // max(1, args.steps), "parallelism": "1 GPU",
-              "launch_shape": shape, "autotuned": tuned is not None, "graphs": True}
+              "launch_shape": shape, "autotuned": tuned is not None, "graphs": not args.no_graphs}
     out = base_line(args, METRIC, value, "Mrays/s", dev_ms / args.steps, config)
     out["latency_ms"] = {"p50": round(percentile(lat, 0.5), 3), "p99": round(percentile(lat, 0.99), 3),
                          "form": "plain (one request per ilm_render, image copied to the host)"}
@@ -540,15 +541,28 @@ def bench_served(args):
     if not args.no_rmse:
         from oracle.pyoracle import lib as orc_lib
         orc = Fp64Oracle(specs, orc_lib().orc_threads_available())
-        errs = []
+        errs, errs32, within, total = [], [], 0, 0
         for r, p, sl in zip(trace, params, slices["served"]):
             n = min(128, p.width)
             pb = (p.height // 2) * p.width + (p.width - n) // 2
-            ref, _, _, _ = orc.scenes[r["scene"]].render(p, pb, pb + n, kahan=True, fp64=True, threads=orc.threads,
-                                                         pixels=False)
+            o = orc.scenes[r["scene"]]
+            ref, _, _, _ = o.render(p, pb, pb + n, kahan=True, fp64=True, threads=orc.threads, pixels=False)
+            r32, _, _, _ = o.render(p, pb, pb + n, kahan=True, fp64=False, threads=orc.threads, pixels=False)
             errs.append(rel_rmse(sl, ref, p.spp))
+            errs32.append(rel_rmse(r32, ref, p.spp))
+            g = np.asarray(sl, np.float32).reshape(-1, 3)
+            rel = np.abs(g - r32) / np.maximum(np.abs(r32), 1e-6)
+            within += int(np.all(rel <= 1e-4, axis=1).sum())
+            total += n
         orc.free()
-        rm = {"max": round(max(errs), 5), "mean": round(float(np.mean(errs)), 5)}
+        w = int(np.argmax(errs))
+        rq = trace[w]
+        rm = {"max": round(max(errs), 5), "mean": round(float(np.mean(errs)), 5),
+              "worst": {"request": w, "scene": rq["scene"], "width": rq["width"], "height": rq["height"],
+                        "spp": rq["spp"], "max_depth": rq["max_depth"],
+                        "fp32_oracle_rmse_vs_fp64": round(errs32[w], 5)},
+              "fp32_oracle_max": round(max(errs32), 5),
+              "pixels_within_1e-4_of_fp32_oracle": round(within / total, 5)}
         log("rmse vs fp64 oracle: %s" % rm)
     sv, pl = forms["served"], forms["plain"]
     arrival_span = max(trace[-1]["t"], 1e-9)
@@ -650,6 +664,8 @@ def main():
     ap.add_argument("--no-rmse", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-count", action="store_true")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="launch every kernel directly (for counter collection); the default replays hipGraphs")
     ap.add_argument("--calibrate-copy", action="store_true", help="only run the 1 GiB copy used to scale HBM counters")
     ap.add_argument("--obj-load", action="store_true", help="only time the OBJ loader on the 722k-triangle mesh (host)")
     args = ap.parse_args()

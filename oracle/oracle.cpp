@@ -1375,6 +1375,23 @@ int orc_path_segments(orc_scene* s, const ilm_render_params* p, uint32_t pixel, 
     return st.segments;
 }
 
+/* orc_path_segments over n (pixel, pass) pairs on all threads: rec holds rec_max records of 9
+ * floats per path (unused records untouched), counts[i] = min(segments of path i, rec_max). */
+int orc_path_segments_batch(orc_scene* s, const ilm_render_params* p, const uint32_t* pixel_pass, int n,
+                            float* rec, int rec_max, int32_t* counts, int threads) {
+    set_threads(threads);
+    RenderInfo<float> ri = make_render_info(*s->f, p->width, p->height);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int i = 0; i < n; i++) {
+        PathStats st;
+        st.rec = rec + (size_t)9 * rec_max * i;
+        st.rec_max = rec_max;
+        trace_path(*s->f, ri, *p, pixel_pass[2 * i + 1], pixel_pass[2 * i], &st);
+        counts[i] = st.segments < rec_max ? st.segments : rec_max;
+    }
+    return 0;
+}
+
 /* Render pixels [pix_begin, pix_end) of the image for passes first_sample..first_sample+spp-1.
  * Per pixel the samples are summed in pass order: plain float sum (kahan=0, the reference's
  * final_colors += c) or Kahan-compensated (kahan=1). accum: (pix_end-pix_begin)*3 floats or

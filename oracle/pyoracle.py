@@ -37,6 +37,7 @@ def lib():
         "orc_closest_hit": (C.c_int, [vp, fp, C.c_int, ip, ip, fp, C.c_int, ip, C.c_int]),
         "orc_path_radiance": (C.c_int, [vp, P(RenderParams), P(C.c_uint32), C.c_int, vp, ip, C.c_int, C.c_int]),
         "orc_path_segments": (C.c_int, [vp, P(RenderParams), C.c_uint32, C.c_uint32, fp, C.c_int]),
+        "orc_path_segments_batch": (C.c_int, [vp, P(RenderParams), P(C.c_uint32), C.c_int, fp, C.c_int, ip, C.c_int]),
         "orc_render": (C.c_double, [vp, P(RenderParams), C.c_int64, C.c_int64, C.c_int, C.c_int, vp, P(Pixel),
                                     P(C.c_uint64), C.c_int]),
         "orc_tonemap_image": (C.c_int, [fp, C.c_int64, C.c_int, C.c_int, P(Pixel)]),
@@ -142,6 +143,17 @@ class OracleScene:
         rec = np.zeros((max_segments, 9), np.float32)
         n = lib().orc_path_segments(self.h, C.byref(params), int(pixel), int(pass_), f32p(rec), max_segments)
         return rec[:min(n, max_segments)]
+
+    def path_segments_batch(self, params, pixel_pass, max_segments, threads=0):
+        """The first max_segments segments of many FP32 paths: (records (n, max_segments, 9),
+        counts (n,)); records past a path's count are zero."""
+        pp = np.ascontiguousarray(pixel_pass, np.uint32).reshape(-1, 2)
+        n = len(pp)
+        rec = np.zeros((n, max_segments, 9), np.float32)
+        cnt = np.zeros(n, np.int32)
+        lib().orc_path_segments_batch(self.h, C.byref(params), pp.ctypes.data_as(C.POINTER(C.c_uint32)), n,
+                                      f32p(rec), int(max_segments), i32p(cnt), int(threads))
+        return rec, cnt
 
     def render(self, params, pix_begin=0, pix_end=None, kahan=True, fp64=False, threads=0, pixels=True):
         """Returns (accum (n,3), pixels (n,3) uint64 or None, segments, seconds)."""

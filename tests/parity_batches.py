@@ -1,0 +1,175 @@
+"""Closest-hit parity batches: 2^20 world-space rays per scene.
+
+Three quarters are the segments of real paths at depths 0-3 as the CPU oracle traces them
+(camera rays with the scene's own pixel jitter, then the bounce, refraction and in-medium
+scatter rays the oracle forms; each comes with the oracle's own closest hit). The rest are
+adversarial: rays aimed exactly at mesh vertices and at points on mesh edges, axis-parallel rays
+(zero direction components, some with origins on a vertex's coordinate planes), rays grazing a
+triangle's plane at angles of 1e-3..1e-7 rad, and rays tangent to spheres to within 1e-6 of the
+radius. Axis-parallel rays are AXIS_RAYS of them: the reference's slab test skips an axis whose
+direction component is 0 (src/math/aabb.cu:119-145), so a ray
+along an axis enters every box its one slab admits and the oracle's walk visits most of the tree.
+"""
+import ctypes as C
+
+import numpy as np
+
+from illume_amd._abi import render_params, transform_desc
+
+N_RAYS = 1 << 20
+RECORDED_FRAC = 0.75
+RECORD_DEPTHS = 4          # segments 0..3: camera rays and secondaries at depths 1-3
+AXIS_RAYS = 2048
+
+
+def _matrix(t):
+    from oracle.pyoracle import lib
+    m = np.empty(48, np.float32)
+    lib().orc_transform_matrices(C.byref(transform_desc(t)), m.ctypes.data_as(C.POINTER(C.c_float)), None)
+    return m[:16].reshape(4, 4).astype(np.float64)
+
+
+def world_triangles(spec, n, rng):
+    """n random triangles of the scene's instances in world space: (n, 3, 3) float64."""
+    insts = spec["instances"]
+    sizes = np.array([len(spec["meshes"][i["mesh"]]["indices"]) for i in insts], np.float64)
+    which = rng.choice(len(insts), n, p=sizes / sizes.sum())
+    out = np.empty((n, 3, 3))
+    for k, inst in enumerate(insts):
+        sel = np.nonzero(which == k)[0]
+        if not len(sel):
+            continue
+        mesh = spec["meshes"][inst["mesh"]]
+        pos = np.asarray(mesh["positions"], np.float64).reshape(-1, 3)
+        idx = np.asarray(mesh["indices"]).reshape(-1, 3)
+        tri = pos[idx[rng.integers(0, len(idx), len(sel))]]
+        M = _matrix(inst.get("transform"))
+        out[sel] = tri @ M[:3, :3].T + M[:3, 3]
+    return out
+
+
+def scene_bounds(spec):
+    lo, hi = np.full(3, np.inf), np.full(3, -np.inf)
+    for inst in spec["instances"]:
+        pos = np.asarray(spec["meshes"][inst["mesh"]]["positions"], np.float64).reshape(-1, 3)
+        M = _matrix(inst.get("transform"))
+        w = pos @ M[:3, :3].T + M[:3, 3]
+        lo, hi = np.minimum(lo, w.min(0)), np.maximum(hi, w.max(0))
+    for s in spec.get("spheres", []):
+        c, r = np.asarray(s["center"], np.float64), s["radius"]
+        lo, hi = np.minimum(lo, c - r), np.maximum(hi, c + r)
+    return lo, hi
+
+
+def _unit(v):
+    return v / np.linalg.norm(v, axis=-1, keepdims=True)
+
+
+def _rays(o, d):
+    return np.concatenate([o, _unit(d)], 1).astype(np.float32)
+
+
+def adversarial_rays(spec, n, seed):
+    """n rays in five families: (rays (n, 6), family (n,) 0 vertex / 1 edge / 2 axis-parallel /
+    3 grazing / 4 sphere tangent, target (n, 3): the point a vertex or edge ray is aimed at, NaN
+    for the other families)."""
+    rng = np.random.default_rng(seed)
+    lo, hi = scene_bounds(spec)
+    # keep the sampling box finite for scenes with a huge floor quad
+    ctr, half = (lo + hi) / 2, np.minimum((hi - lo) / 2, 6.0)
+    lo, hi = ctr - 1.2 * half, ctr + 1.2 * half
+    spheres = spec.get("spheres", [])
+    others = [0, 1, 3, 4] if spheres else [0, 1, 3]
+    n_axis = min(AXIS_RAYS, n // 8)
+    m = n - n_axis
+    per = [0] * 5
+    per[2] = n_axis
+    for j, k in enumerate(others):
+        per[k] = m // len(others) + (1 if j < m % len(others) else 0)
+    out, targets = [], []
+    # 1. through vertices
+    tri = world_triangles(spec, per[0], rng)
+    target = tri[np.arange(per[0]), rng.integers(0, 3, per[0])]
+    o = rng.uniform(lo, hi, (per[0], 3))
+    out.append(_rays(o, target - o))
+    targets.append(target)
+    # 2. through points on edges (a quarter of them exactly at edge midpoints)
+    tri = world_triangles(spec, per[1], rng)
+    a = rng.integers(0, 3, per[1])
+    v0, v1 = tri[np.arange(per[1]), a], tri[np.arange(per[1]), (a + 1) % 3]
+    s = rng.uniform(0, 1, (per[1], 1))
+    s[: per[1] // 4] = 0.5
+    o = rng.uniform(lo, hi, (per[1], 3))
+    target = v0 + s * (v1 - v0)
+    out.append(_rays(o, target - o))
+    targets.append(target)
+    # 3. axis-parallel; half the origins share a coordinate with a vertex in the two other axes
+    axis = rng.integers(0, 3, per[2])
+    sign = rng.choice([-1.0, 1.0], per[2])
+    d = np.zeros((per[2], 3))
+    d[np.arange(per[2]), axis] = sign
+    o = rng.uniform(lo, hi, (per[2], 3))
+    h = per[2] // 2
+    tri = world_triangles(spec, h, rng)
+    vert = tri[np.arange(h), rng.integers(0, 3, h)]
+    for k in range(3):
+        m = axis[:h] != k
+        o[:h][m, k] = vert[m, k]
+    out.append(np.concatenate([o, d], 1).astype(np.float32))
+    # 4. grazing a triangle's plane, aimed at its centroid from outside it
+    tri = world_triangles(spec, per[3], rng)
+    nrm = _unit(np.cross(tri[:, 1] - tri[:, 0], tri[:, 2] - tri[:, 0]))
+    cen = tri.mean(1)
+    e = _unit(tri[np.arange(per[3]), rng.integers(0, 3, per[3])] - cen)
+    ang = 10.0 ** rng.uniform(-7, -3, per[3])
+    side = rng.choice([-1.0, 1.0], per[3])
+    d = _unit(-e + (side * np.tan(ang))[:, None] * nrm)
+    size = np.linalg.norm(tri[:, 1] - tri[:, 0], axis=1)
+    o = cen - d * (size * rng.uniform(1.5, 40.0, per[3]))[:, None]
+    out.append(_rays(o, d))
+    # 5. tangent to spheres, passing the centre at r * (1 +- 1e-6 .. 1e-3)
+    if spheres:
+        k = rng.integers(0, len(spheres), per[4])
+        c = np.array([spheres[i]["center"] for i in k], np.float64)
+        r = np.array([spheres[i]["radius"] for i in k], np.float64)
+        u = _unit(rng.normal(size=(per[4], 3)))
+        w = _unit(np.cross(u, rng.normal(size=(per[4], 3))))
+        miss = r * (1 + rng.choice([-1.0, 1.0], per[4]) * 10.0 ** rng.uniform(-6, -3, per[4]))
+        o = c + miss[:, None] * w - u * (r * rng.uniform(2, 10, per[4]))[:, None]
+        out.append(_rays(o, u))
+    fam = np.repeat(np.arange(5), per)
+    tg = np.concatenate(targets + [np.full((n - per[0] - per[1], 3), np.nan)])
+    return np.concatenate(out), fam, tg
+
+
+def recorded_rays(orc, settings, n, seed, record_depth_cap=True):
+    """Segments 0..3 of oracle paths at random (pixel, pass): (rays (m, 6), hits (inst, prim, t),
+    depth (m,)), m = n, drawn over paths until n segments are collected. With record_depth_cap the
+    paths run at most 4 segments, which leaves their first 4 unchanged (depth only ends the loop)."""
+    rng = np.random.default_rng(seed)
+    st = dict(settings)
+    if record_depth_cap:
+        st["max_depth"] = min(st["max_depth"], RECORD_DEPTHS)
+    p = render_params(**st)
+    W, H = settings["width"], settings["height"]
+    chunks, total = [], 0
+    while total < n:
+        k = max((n - total) // 2, 1024)
+        pairs = np.stack([rng.integers(0, W * H, k), rng.integers(0, 1 << 16, k)], 1).astype(np.uint32)
+        rec, cnt = orc.path_segments_batch(p, pairs, RECORD_DEPTHS)
+        keep = np.arange(RECORD_DEPTHS)[None, :] < cnt[:, None]
+        depth = np.broadcast_to(np.arange(RECORD_DEPTHS), keep.shape)[keep]
+        chunks.append((rec[keep], depth))
+        total += len(depth)
+    rec = np.concatenate([c[0] for c in chunks])[:n]
+    depth = np.concatenate([c[1] for c in chunks])[:n]
+    hits = (rec[:, 6].astype(np.int32), rec[:, 7].astype(np.int32), rec[:, 8].copy())
+    return np.ascontiguousarray(rec[:, :6]), hits, depth
+
+
+def parity_batch(spec, settings, orc, seed):
+    """(rays (2^20, 6), recorded hits of the first m rays, depth of those rays)."""
+    m = int(N_RAYS * RECORDED_FRAC)
+    rays, hits, depth = recorded_rays(orc, settings, m, seed)
+    adv, _, _ = adversarial_rays(spec, N_RAYS - m, seed + 1)
+    return np.concatenate([rays, adv]), hits, depth

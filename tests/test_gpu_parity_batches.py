@@ -1,0 +1,77 @@
+"""Closest-hit parity batches on the GPU: 2^20 world-space rays per scene, both BVH layouts.
+
+Scenes: the Cornell box, the 100,820-triangle gold spheres scene, the 524,880-triangle subsurface
+mesh and the 12-instance 722,000-triangle mesh of the request trace. Per scene, three quarters of
+the batch are the segments at depths 0-3 of paths the CPU oracle traces (camera rays, bounces,
+refractions, in-medium scatters), checked against the closest hit the oracle found for each while
+tracing; the rest are adversarial rays (through vertices, through edge points, axis-parallel,
+grazing triangle planes, tangent to spheres), checked against the oracle's closest-hit routine
+(reference traversal: instances in a linear loop, src/renderer.cu:112-137, each mesh walked as
+in src/primitives/mesh.cu:479-562).
+
+Bar: the same (instance, primitive) or a distance tie within 1e-6 relative (the reference takes
+the first primitive met at equal distances, the device the smaller (instance, primitive)); hit
+distances bit-identical wherever the primitive agrees.
+"""
+import numpy as np
+import pytest
+
+import illume_amd.scenes as S
+from oracle.pyoracle import OracleScene
+from conftest import compare_hits
+from parity_batches import N_RAYS, RECORD_DEPTHS, parity_batch
+
+pytestmark = pytest.mark.gpu
+
+SCENES = {"cornell": S.cornell, "gold": S.gold_spheres, "subsurface": S.subsurface, "instanced": S.instanced}
+
+
+@pytest.fixture(scope="module", params=sorted(SCENES))
+def batch(request):
+    name = request.param
+    spec, st = SCENES[name]()
+    orc = OracleScene(spec, median_bvh=True)
+    rays, hits, depth = parity_batch(spec, st, orc, seed=100 + sorted(SCENES).index(name))
+    m = len(depth)
+    adv = orc.closest_hit(rays[m:])
+    orc.free()
+    expect = tuple(np.concatenate([a, b]) for a, b in zip(hits, adv))
+    return name, spec, rays, expect, depth
+
+
+def _check(label, ray_slice, g, o, depth=None):
+    exact, tie, bad = compare_hits(g, o)
+    same = (g[0] == o[0]) & (g[1] == o[1]) & (g[0] != -1)
+    t_exact = np.array_equal(g[2][same], o[2][same])
+    print("  %s: %d rays, %d same, %d ties, %d mismatches, %d hits, t bit-exact %s"
+          % (label, len(g[0]), exact, tie, len(bad), int((o[0] != -1).sum()), t_exact))
+    if depth is not None and len(bad):
+        print("    mismatches by depth", np.bincount(depth[bad], minlength=RECORD_DEPTHS))
+    for i in bad[:5]:
+        print("    ray", ray_slice[i], "gpu", g[0][i], g[1][i], g[2][i], "oracle", o[0][i], o[1][i], o[2][i])
+    return len(bad), t_exact
+
+
+@pytest.mark.parametrize("layout", ["fp32", "quant8"])
+def test_closest_hit_batch_2pow20(batch, layout, ctx, ctx_quant):
+    name, spec, rays, expect, depth = batch
+    assert len(rays) == N_RAYS
+    m = len(depth)
+    counts = np.bincount(depth, minlength=RECORD_DEPTHS)
+    print("%s [%s]: recorded rays by depth %s, adversarial %d" % (name, layout, counts.tolist(), N_RAYS - m))
+    assert all(c >= 0.04 * m for c in counts), counts   # every depth 0..3 represented
+    c = ctx if layout == "fp32" else ctx_quant
+    sc = c.scene(spec)
+    try:
+        g = c.closest_hit(sc, rays)
+    finally:
+        sc.free()
+    bad = 0
+    ok_t = True
+    for lab, sl in (("recorded", slice(0, m)), ("adversarial", slice(m, N_RAYS))):
+        b, t = _check(lab, rays[sl], tuple(a[sl] for a in g), tuple(a[sl] for a in expect),
+                      depth if lab == "recorded" else None)
+        bad += b
+        ok_t = ok_t and t
+    assert bad == 0
+    assert ok_t

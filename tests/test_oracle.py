@@ -624,3 +624,42 @@ def test_free_flight_mean_is_one_over_sigma_s():
     assert len(gaps) > 5000
     assert gaps.mean() == pytest.approx(0.5, abs=4 * 0.5 / math.sqrt(len(gaps)))
     assert stats.kstest(gaps, "expon", args=(0, 0.5)).pvalue > 1e-3
+
+
+def test_segment_batch_equals_single_paths_and_closest_hit():
+    """The batched segment recorder that builds the closest-hit parity batches: its records equal
+    the single-path recorder's; capping max_depth at 4 leaves the first 4 segments unchanged
+    (depth only ends the path loop, src/renderer.cu:538-557); each recorded hit equals what the
+    closest-hit routine returns for the recorded ray; a batch holds 2^20 rays, the last quarter
+    adversarial with unit directions; every vertex and edge ray passes through its target point."""
+    import illume_amd.scenes as S
+    from illume_amd._abi import render_params
+    from parity_batches import N_RAYS, RECORD_DEPTHS, adversarial_rays, parity_batch, recorded_rays
+    spec, st = S.tiny_mesh_scene()
+    orc = _orc(spec)
+    p = render_params(**st)
+    pairs = np.array([[px, s] for px in range(0, st["width"] * st["height"], 7) for s in range(3)], np.uint32)
+    rec, cnt = orc.path_segments_batch(p, pairs, RECORD_DEPTHS)
+    for i in range(0, len(pairs), 5):
+        one = orc.path_segments(p, int(pairs[i, 0]), int(pairs[i, 1]), max_segments=RECORD_DEPTHS)
+        assert cnt[i] == len(one)
+        assert np.array_equal(rec[i, :cnt[i]], one)
+    capped = dict(st, max_depth=RECORD_DEPTHS)
+    rec4, cnt4 = orc.path_segments_batch(render_params(**capped), pairs, RECORD_DEPTHS)
+    assert np.array_equal(cnt4, cnt) and np.array_equal(rec4, rec)
+    rays, hits, depth = recorded_rays(orc, st, 20000, seed=5)
+    o = orc.closest_hit(rays)
+    assert all(np.array_equal(a, b) for a, b in zip(o, hits))
+    assert set(np.unique(depth)) == set(range(RECORD_DEPTHS))
+    rays, hits, depth = parity_batch(spec, st, orc, seed=9)
+    assert rays.shape == (N_RAYS, 6) and len(depth) == 3 * N_RAYS // 4
+    assert np.allclose(np.linalg.norm(rays[len(depth):, 3:], axis=1), 1.0, atol=1e-6)
+    adv, fam, tgt = adversarial_rays(spec, 4000, seed=3)
+    orc.free()
+    assert adv.shape == (4000, 6) and np.bincount(fam, minlength=5)[2] == 4000 // 8
+    aimed = fam <= 1
+    o, d = adv[aimed, :3].astype(np.float64), adv[aimed, 3:].astype(np.float64)
+    v = tgt[aimed] - o
+    miss = np.linalg.norm(v - np.sum(v * d, 1)[:, None] * d, axis=1)
+    assert np.all(miss <= 1e-6 * np.linalg.norm(v, axis=1) + 1e-6)
+    assert np.all(np.isnan(tgt[~aimed]))
