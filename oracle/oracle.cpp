@@ -1333,6 +1333,56 @@ int orc_closest_hit(orc_scene* s, const float* rays, int n, int32_t* inst, int32
     return 0;
 }
 
+/* Every hit of one ray under the reference's own tests, nearest first (brute force over every
+ * sphere and every instance's triangles; a triangle is tested on the FP32 object-space ray as
+ * get_min_hit forms it): inst, prim (-1 for a sphere), FP32 distance, and for a triangle the
+ * margin min(u, v, 1 - u - v) of the same Moller-Trumbore test evaluated in FP64 on the same
+ * FP32 inputs (+inf for a sphere). A negative margin marks a point outside the triangle that
+ * FP32 rounding accepted: whether a BVH walk ever tests that triangle depends on the tree's boxes
+ * (the reference tests the triangles of every leaf whose parent box the ray meets), so a walk
+ * over another tree may report the next hit instead. Fills at most cap entries; returns the count. */
+int orc_ray_hits(orc_scene* s, const float* ray6, int cap, int32_t* inst, int32_t* prim, float* t, double* margin) {
+    const Scene<float>& sc = *s->f;
+    Ray<float> r;
+    r.o = v3(ray6[0], ray6[1], ray6[2]);
+    r.d = v3(ray6[3], ray6[4], ray6[5]);
+    struct H { int inst, prim; float t; double m; };
+    std::vector<H> hits;
+    for (size_t k = 0; k < sc.spheres.size(); k++) {
+        Hit<float> hh;
+        sphere_intersect(sc.spheres[k], r, &hh);
+        if (hh.is) hits.push_back({-2 - (int)k, -1, hh.d, HUGE_VAL});
+    }
+    for (size_t k = 0; k < sc.inst.size(); k++) {
+        const Instance<float>& in = sc.inst[k];
+        Ray<float> r2;
+        r2.o = m4_mul_v3(in.t.inv, r.o, 1.0f);
+        r2.d = m4_mul_v3(in.t.inv, r.d, 0.0f);
+        const Mesh<float>& m = sc.meshes[in.mesh];
+        for (size_t q = 0; q < m.idx.size() / 3; q++) {
+            const V3<float> a = m.pos[m.idx[3 * q]], b = m.pos[m.idx[3 * q + 1]], c = m.pos[m.idx[3 * q + 2]];
+            float u, v;
+            const float d = tri_intersect(a, b, c, r2, u, v);
+            if (!(d > 0)) continue;
+            const V3<double> A = v3((double)a.x, (double)a.y, (double)a.z);
+            const V3<double> e1 = vsub(v3((double)b.x, (double)b.y, (double)b.z), A);
+            const V3<double> e2 = vsub(v3((double)c.x, (double)c.y, (double)c.z), A);
+            const V3<double> D = v3((double)r2.d.x, (double)r2.d.y, (double)r2.d.z);
+            const V3<double> P = vcross(D, e2);
+            const double det = vdot(e1, P);
+            const V3<double> T = vsub(v3((double)r2.o.x, (double)r2.o.y, (double)r2.o.z), A);
+            const double uu = vdot(T, P) / det;
+            const double vv = vdot(D, vcross(T, e1)) / det;
+            hits.push_back({(int)k, (int)q, d, std::min(std::min(uu, vv), 1.0 - uu - vv)});
+        }
+    }
+    std::stable_sort(hits.begin(), hits.end(), [](const H& x, const H& y) { return x.t < y.t; });
+    for (int i = 0; i < (int)hits.size() && i < cap; i++) {
+        inst[i] = hits[i].inst; prim[i] = hits[i].prim; t[i] = hits[i].t; margin[i] = hits[i].m;
+    }
+    return (int)hits.size();
+}
+
 /* Radiance of paths (pixel, pass). fp64 selects the twin; rgb is n*3 doubles when fp64 else floats. */
 int orc_path_radiance(orc_scene* s, const ilm_render_params* p, const uint32_t* pixel_pass, int n,
                       void* rgb, int32_t* segments, int fp64, int threads) {

@@ -35,6 +35,7 @@ def lib():
         "orc_ref_bvh_size": (C.c_int, [vp, C.c_int, ip]),
         "orc_ref_bvh_export": (C.c_int, [vp, C.c_int, fp, P(C.c_uint32), ip]),
         "orc_closest_hit": (C.c_int, [vp, fp, C.c_int, ip, ip, fp, C.c_int, ip, C.c_int]),
+        "orc_ray_hits": (C.c_int, [vp, fp, C.c_int, ip, ip, fp, P(C.c_double)]),
         "orc_path_radiance": (C.c_int, [vp, P(RenderParams), P(C.c_uint32), C.c_int, vp, ip, C.c_int, C.c_int]),
         "orc_path_segments": (C.c_int, [vp, P(RenderParams), C.c_uint32, C.c_uint32, fp, C.c_int]),
         "orc_path_segments_batch": (C.c_int, [vp, P(RenderParams), P(C.c_uint32), C.c_int, fp, C.c_int, ip, C.c_int]),
@@ -127,6 +128,19 @@ class OracleScene:
         lib().orc_closest_hit(self.h, f32p(rays), n, i32p(inst), i32p(prim), f32p(t), int(brute),
                               i32p(cnt) if counts else None, int(threads))
         return (inst, prim, t, cnt) if counts else (inst, prim, t)
+
+    def ray_hits(self, ray, cap=256):
+        """Every hit of one ray, nearest first: (inst, prim, t, margin) arrays; margin < 0 marks a
+        triangle the FP32 test accepts outside its FP64 barycentric range (see orc_ray_hits)."""
+        r = np.ascontiguousarray(ray, np.float32).reshape(6)
+        inst = np.zeros(cap, np.int32)
+        prim = np.zeros(cap, np.int32)
+        t = np.zeros(cap, np.float32)
+        m = np.zeros(cap, np.float64)
+        n = lib().orc_ray_hits(self.h, f32p(r), cap, i32p(inst), i32p(prim), f32p(t),
+                               m.ctypes.data_as(C.POINTER(C.c_double)))
+        n = min(n, cap)
+        return inst[:n], prim[:n], t[:n], m[:n]
 
     def path_radiance(self, params, pixel_pass, fp64=False, threads=0):
         pp = np.ascontiguousarray(pixel_pass, dtype=np.uint32).reshape(-1, 2)

@@ -11,7 +11,13 @@ in src/primitives/mesh.cu:479-562).
 
 Bar: the same (instance, primitive) or a distance tie within 1e-6 relative (the reference takes
 the first primitive met at equal distances, the device the smaller (instance, primitive)); hit
-distances bit-identical wherever the primitive agrees.
+distances bit-identical wherever the primitive agrees. One more class of difference is allowed
+and counted: near-edge acceptances. The FP32 Moller-Trumbore test (src/primitives/mesh.cu:446-476)
+accepts some points that lie just outside the triangle (FP64 on the same FP32 inputs puts them
+outside), and whether a walk tests such a triangle at all depends on its tree's boxes; where the
+device and the oracle differ, both answers must be ones a conservative walk can give (every hit
+nearer than each answer is such an acceptance, parity_batches.near_edge_split), and they may make
+up at most 1e-4 of the recorded rays and 1e-3 of the adversarial ones.
 """
 import numpy as np
 import pytest
@@ -19,7 +25,7 @@ import pytest
 import illume_amd.scenes as S
 from oracle.pyoracle import OracleScene
 from conftest import compare_hits
-from parity_batches import N_RAYS, RECORD_DEPTHS, parity_batch
+from parity_batches import N_RAYS, RECORD_DEPTHS, near_edge_split, parity_batch
 
 pytestmark = pytest.mark.gpu
 
@@ -34,27 +40,30 @@ def batch(request):
     rays, hits, depth = parity_batch(spec, st, orc, seed=100 + sorted(SCENES).index(name))
     m = len(depth)
     adv = orc.closest_hit(rays[m:])
-    orc.free()
     expect = tuple(np.concatenate([a, b]) for a, b in zip(hits, adv))
-    return name, spec, rays, expect, depth
+    yield name, spec, rays, expect, depth, orc
+    orc.free()
 
 
-def _check(label, ray_slice, g, o, depth=None):
+def _check(label, orc, ray_slice, g, o, max_exempt, depth=None):
     exact, tie, bad = compare_hits(g, o)
     same = (g[0] == o[0]) & (g[1] == o[1]) & (g[0] != -1)
     t_exact = np.array_equal(g[2][same], o[2][same])
-    print("  %s: %d rays, %d same, %d ties, %d mismatches, %d hits, t bit-exact %s"
-          % (label, len(g[0]), exact, tie, len(bad), int((o[0] != -1).sum()), t_exact))
+    assert len(bad) <= 4 * max_exempt + 16, (label, len(bad))
+    exempt, failing = near_edge_split(orc, ray_slice, g, o, bad)
+    print("  %s: %d rays, %d same, %d ties, %d near-edge acceptances, %d mismatches, %d hits, t bit-exact %s"
+          % (label, len(g[0]), exact, tie, len(exempt), len(failing), int((o[0] != -1).sum()), t_exact))
     if depth is not None and len(bad):
-        print("    mismatches by depth", np.bincount(depth[bad], minlength=RECORD_DEPTHS))
-    for i in bad[:5]:
+        print("    differing rays by depth", np.bincount(depth[bad], minlength=RECORD_DEPTHS))
+    for i in failing[:5]:
         print("    ray", ray_slice[i], "gpu", g[0][i], g[1][i], g[2][i], "oracle", o[0][i], o[1][i], o[2][i])
-    return len(bad), t_exact
+    assert len(exempt) <= max_exempt, (label, len(exempt))
+    return len(failing), t_exact
 
 
 @pytest.mark.parametrize("layout", ["fp32", "quant8"])
 def test_closest_hit_batch_2pow20(batch, layout, ctx, ctx_quant):
-    name, spec, rays, expect, depth = batch
+    name, spec, rays, expect, depth, orc = batch
     assert len(rays) == N_RAYS
     m = len(depth)
     counts = np.bincount(depth, minlength=RECORD_DEPTHS)
@@ -68,8 +77,8 @@ def test_closest_hit_batch_2pow20(batch, layout, ctx, ctx_quant):
         sc.free()
     bad = 0
     ok_t = True
-    for lab, sl in (("recorded", slice(0, m)), ("adversarial", slice(m, N_RAYS))):
-        b, t = _check(lab, rays[sl], tuple(a[sl] for a in g), tuple(a[sl] for a in expect),
+    for lab, sl, cap in (("recorded", slice(0, m), m // 10000), ("adversarial", slice(m, N_RAYS), (N_RAYS - m) // 1000)):
+        b, t = _check(lab, orc, rays[sl], tuple(a[sl] for a in g), tuple(a[sl] for a in expect), cap,
                       depth if lab == "recorded" else None)
         bad += b
         ok_t = ok_t and t

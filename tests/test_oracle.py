@@ -663,3 +663,38 @@ def test_segment_batch_equals_single_paths_and_closest_hit():
     miss = np.linalg.norm(v - np.sum(v * d, 1)[:, None] * d, axis=1)
     assert np.all(miss <= 1e-6 * np.linalg.norm(v, axis=1) + 1e-6)
     assert np.all(np.isnan(tgt[~aimed]))
+
+
+def test_ray_hits_margins_and_near_edge_answers():
+    """orc_ray_hits lists every hit nearest first; its first entry is the brute-force closest hit,
+    and its FP64 margin is positive for rays aimed at triangle centroids. Three grazing rays under
+    the Cornell box's short block (y about -1e-5, rising 2e-6 per unit) are accepted by the FP32
+    Moller-Trumbore test on a block face at a point 1e-6 outside it in FP64 (margin < 0); the
+    floor hit behind it is genuine, and both are answers a conservative walk may give."""
+    import illume_amd.scenes as S
+    from parity_batches import _valid_answer, world_triangles
+    spec, _ = S.cornell()
+    orc = _orc(spec)
+    rng = np.random.default_rng(4)
+    tri = world_triangles(spec, 200, rng)
+    o = rng.uniform(-0.4, 0.4, (200, 3)) + np.array([0.0, 0.5, -1.5])
+    d = tri.mean(1) - o
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.concatenate([o, d], 1).astype(np.float32)
+    b = orc.closest_hit(rays, brute=True)
+    for i in range(200):
+        hi, hp, ht, hm = orc.ray_hits(rays[i])
+        assert np.all(np.diff(ht) >= 0)
+        assert (hi[0], hp[0], ht[0]) == (b[0][i], b[1][i], b[2][i])
+        assert hm[0] > 0
+    grazing = np.array([[-4.6435094e+00, -1.1551218e-05, -3.2243564e+00, 8.3451194e-01, 2.0060797e-06, 5.5098987e-01],
+                        [2.7501789e-01, 6.0002095e-01, -8.3837233e+00, -5.9915286e-02, -2.4425719e-06, 9.9820346e-01],
+                        [2.2245555e+00, 3.0002624e-01, -7.7503853e+00, -2.5881901e-01, -3.3489050e-06, 9.6592581e-01]],
+                       np.float32)
+    for r in grazing:
+        hits = orc.ray_hits(r)
+        hi, hp, ht, hm = hits
+        assert -1e-5 < hm[0] < 0 and hm[1] > 0
+        assert _valid_answer(hits, hi[0], hp[0], ht[0]) and _valid_answer(hits, hi[1], hp[1], ht[1])
+        assert not _valid_answer(hits, hi[2], hp[2], ht[2]) and not _valid_answer(hits, -1, -1, np.float32(3.4028235e38))
+    orc.free()
