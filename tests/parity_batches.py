@@ -187,13 +187,18 @@ def _valid_answer(hits, inst, prim, t):
 
 
 def near_edge_split(orc, rays, g, o, bad):
-    """Split mismatching rays into (exempt, failing): exempt when both answers are valid under
-    _valid_answer, i.e. the two walks differ only in whether they tested a triangle the FP32
-    Moller-Trumbore test accepts at a point FP64 puts outside it (src/primitives/mesh.cu:446-476:
-    the reference's own answer for such a ray depends on which leaves its tree lets it reach)."""
-    exempt, failing = [], []
+    """Classify rays where the device's closest hit differs from the oracle's tree walk by the
+    ray's full hit list (every triangle tested, no tree): (exempt, failing, walk_invalid).
+    exempt: the device's answer is one a conservative walk can give (_valid_answer); the two
+    walks then differ only near a triangle edge, where the FP32 Moller-Trumbore test
+    (src/primitives/mesh.cu:446-476) accepts points FP64 puts outside the triangle and where the
+    reference's unpadded boxes (src/primitives/mesh.cu:479-530) can also cull a genuine hit, so
+    the reference's own answer depends on its tree. failing: the device's answer is not valid.
+    walk_invalid: rays whose oracle walk answer is itself not valid (a genuine hit culled)."""
+    exempt, failing, walk_invalid = [], [], []
     for i in bad:
         hits = orc.ray_hits(rays[i], cap=4096)
-        ok = (_valid_answer(hits, g[0][i], g[1][i], g[2][i]) and _valid_answer(hits, o[0][i], o[1][i], o[2][i]))
-        (exempt if ok else failing).append(int(i))
-    return exempt, failing
+        (exempt if _valid_answer(hits, g[0][i], g[1][i], g[2][i]) else failing).append(int(i))
+        if not _valid_answer(hits, o[0][i], o[1][i], o[2][i]):
+            walk_invalid.append(int(i))
+    return exempt, failing, walk_invalid

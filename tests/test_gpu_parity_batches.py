@@ -11,13 +11,15 @@ in src/primitives/mesh.cu:479-562).
 
 Bar: the same (instance, primitive) or a distance tie within 1e-6 relative (the reference takes
 the first primitive met at equal distances, the device the smaller (instance, primitive)); hit
-distances bit-identical wherever the primitive agrees. One more class of difference is allowed
-and counted: near-edge acceptances. The FP32 Moller-Trumbore test (src/primitives/mesh.cu:446-476)
-accepts some points that lie just outside the triangle (FP64 on the same FP32 inputs puts them
-outside), and whether a walk tests such a triangle at all depends on its tree's boxes; where the
-device and the oracle differ, both answers must be ones a conservative walk can give (every hit
-nearer than each answer is such an acceptance, parity_batches.near_edge_split), and they may make
-up at most 1e-4 of the recorded rays and 1e-3 of the adversarial ones.
+distances bit-identical wherever the primitive agrees. Where the device and the oracle's walk
+still differ, the ray's full hit list decides (the oracle's triangle test on every triangle, no
+tree, orc_ray_hits): the device's hit must be one that a conservative walk can give, i.e. every
+hit nearer than it is a point the FP32 Moller-Trumbore test (src/primitives/mesh.cu:446-476)
+accepts outside the triangle. These near-edge differences come from the trees: such a point is
+tested or not depending on the boxes, and the reference's unpadded boxes
+(src/primitives/mesh.cu:479-530) also cull genuine hits within about 1e-6 of an edge, which the
+device's padded boxes keep (the count of such oracle answers is printed). They may make up at
+most 1e-4 of the recorded rays and 2e-3 of the adversarial ones.
 """
 import numpy as np
 import pytest
@@ -50,9 +52,11 @@ def _check(label, orc, ray_slice, g, o, max_exempt, depth=None):
     same = (g[0] == o[0]) & (g[1] == o[1]) & (g[0] != -1)
     t_exact = np.array_equal(g[2][same], o[2][same])
     assert len(bad) <= 4 * max_exempt + 16, (label, len(bad))
-    exempt, failing = near_edge_split(orc, ray_slice, g, o, bad)
-    print("  %s: %d rays, %d same, %d ties, %d near-edge acceptances, %d mismatches, %d hits, t bit-exact %s"
-          % (label, len(g[0]), exact, tie, len(exempt), len(failing), int((o[0] != -1).sum()), t_exact))
+    exempt, failing, walk_invalid = near_edge_split(orc, ray_slice, g, o, bad)
+    print("  %s: %d rays, %d same, %d ties, %d near-edge differences (%d where the oracle's walk culled a "
+          "genuine hit), %d mismatches, %d hits, t bit-exact %s"
+          % (label, len(g[0]), exact, tie, len(exempt), len(walk_invalid), len(failing), int((o[0] != -1).sum()),
+             t_exact))
     if depth is not None and len(bad):
         print("    differing rays by depth", np.bincount(depth[bad], minlength=RECORD_DEPTHS))
     for i in failing[:5]:
@@ -77,7 +81,7 @@ def test_closest_hit_batch_2pow20(batch, layout, ctx, ctx_quant):
         sc.free()
     bad = 0
     ok_t = True
-    for lab, sl, cap in (("recorded", slice(0, m), m // 10000), ("adversarial", slice(m, N_RAYS), (N_RAYS - m) // 1000)):
+    for lab, sl, cap in (("recorded", slice(0, m), m // 10000), ("adversarial", slice(m, N_RAYS), (N_RAYS - m) // 500)):
         b, t = _check(lab, orc, rays[sl], tuple(a[sl] for a in g), tuple(a[sl] for a in expect), cap,
                       depth if lab == "recorded" else None)
         bad += b

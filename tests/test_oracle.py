@@ -698,3 +698,26 @@ def test_ray_hits_margins_and_near_edge_answers():
         assert _valid_answer(hits, hi[0], hp[0], ht[0]) and _valid_answer(hits, hi[1], hp[1], ht[1])
         assert not _valid_answer(hits, hi[2], hp[2], ht[2]) and not _valid_answer(hits, -1, -1, np.float32(3.4028235e38))
     orc.free()
+
+
+def test_reference_walk_culls_a_genuine_near_edge_hit():
+    """The reference's walk over its own tree (unpadded boxes, src/primitives/mesh.cu:479-530) can
+    miss a genuine hit within a few 1e-6 of a triangle edge: this ray into the gold-spheres mesh
+    meets triangle 25885 at t = 6.1709 with FP64 margin +2.7e-6, which brute force reports and the
+    walk does not (it returns the next hit, 44170 at t = 8.0312). The device keeps such hits (its
+    triangle boxes are padded), so closest-hit parity for near-edge rays is judged on the full
+    hit list rather than on the walk's answer."""
+    import illume_amd.scenes as S
+    from parity_batches import _valid_answer
+    spec, _ = S.gold_spheres()
+    orc = _orc(spec)
+    ray = np.array([[3.3840985, 0.59444964, 5.447949, -0.6038722, 0.07593211, -0.7934562]], np.float32)
+    walk = orc.closest_hit(ray)
+    brute = orc.closest_hit(ray, brute=True)
+    hits = orc.ray_hits(ray[0])
+    orc.free()
+    assert (brute[1][0], brute[2][0]) == (25885, np.float32(6.170903))
+    assert (walk[1][0], walk[2][0]) == (44170, np.float32(8.031214))
+    assert hits[1][0] == 25885 and 0 < hits[3][0] < 1e-5
+    assert _valid_answer(hits, brute[0][0], brute[1][0], brute[2][0])
+    assert not _valid_answer(hits, walk[0][0], walk[1][0], walk[2][0])
