@@ -33,9 +33,15 @@
 #define EXT_BOUNDS __launch_bounds__(EXT_BLOCK)
 #endif
 
-/* conservative slab-test margins (relative); cover rcp and fma rounding of the box test */
+/* Conservative slab-test margins (relative); they cover the rcp and fma rounding of the box
+ * test. A child is hit when tn * BOX_LO <= tf * BOX_HI and tn * BOX_LO <= tmax; the tests fold
+ * the two factors into one: tn <= tf * BOX_K and tn <= tmax * BOX_TMAX, with BOX_K and BOX_TMAX
+ * the floats just above BOX_HI / BOX_LO and 1 / BOX_LO (0x3f8000a9 and 0x3f800055), so they
+ * accept every box the unfolded form accepts. */
 #define BOX_LO 0.99999f
 #define BOX_HI 1.00001f
+#define BOX_K 1.0000201f
+#define BOX_TMAX 1.0000101f
 
 __device__ inline uint32_t f2u(float f) { return __float_as_uint(f); }
 __device__ inline float u2f(uint32_t u) { return __uint_as_float(u); }
@@ -113,11 +119,13 @@ __device__ inline V3 rcp3(V3 d) {
 
 /* Two child boxes of an FP32 BVH2 node against the ray (o, 1/d). A slab distance is formed as
  * (p - o) * (1/d): each of the two roundings and v_rcp_f32's error is relative to the distance
- * itself, so BOX_LO / BOX_HI cover them wherever the ray starts (the fma form p/d - o/d carries
+ * itself, so the BOX_ margins cover them wherever the ray starts (the fma form p/d - o/d carries
  * the rounding of o/d, an absolute error of |o/d| * 2^-24 that a ray far from the world origin
- * meeting a box at small distance would not be covered for). */
+ * meeting a box at small distance would not be covered for). t0, t1: entry distances, used to
+ * order the children. */
 __device__ inline void box2(const float4 a, const float4 b, const float4 c, V3 id, V3 o, float tmax,
                             float& t0, float& t1, bool& h0, bool& h1) {
+    const float tm = tmax * BOX_TMAX;
     float x0 = (a.x - o.x) * id.x, x1 = (a.y - o.x) * id.x;
     float y0 = (a.z - o.y) * id.y, y1 = (a.w - o.y) * id.y;
     float z0 = (b.x - o.z) * id.z, z1 = (b.y - o.z) * id.z;
@@ -125,8 +133,8 @@ __device__ inline void box2(const float4 a, const float4 b, const float4 c, V3 i
                                __builtin_fmaxf(__builtin_fminf(z0, z1), 0.0f));
     float tf = __builtin_fminf(__builtin_fminf(__builtin_fmaxf(x0, x1), __builtin_fmaxf(y0, y1)),
                                __builtin_fmaxf(z0, z1));
-    t0 = tn * BOX_LO;
-    h0 = (t0 <= tf * BOX_HI) && (t0 <= tmax);
+    t0 = tn;
+    h0 = (tn <= tf * BOX_K) && (tn <= tm);
 
     float X0 = (b.z - o.x) * id.x, X1 = (b.w - o.x) * id.x;
     float Y0 = (c.x - o.y) * id.y, Y1 = (c.y - o.y) * id.y;
@@ -135,50 +143,58 @@ __device__ inline void box2(const float4 a, const float4 b, const float4 c, V3 i
                                __builtin_fmaxf(__builtin_fminf(Z0, Z1), 0.0f));
     float Tf = __builtin_fminf(__builtin_fminf(__builtin_fmaxf(X0, X1), __builtin_fmaxf(Y0, Y1)),
                                __builtin_fmaxf(Z0, Z1));
-    t1 = Tn * BOX_LO;
-    h1 = (t1 <= Tf * BOX_HI) && (t1 <= tmax);
+    t1 = Tn;
+    h1 = (Tn <= Tf * BOX_K) && (Tn <= tm);
 }
 
-/* Up to four dequantised child boxes of a QNode (dequantisation as bvh.cpp's qnode_child_box). */
+/* Up to four dequantised child boxes of a QNode (dequantisation as bvh.cpp's qnode_child_box):
+ * bit k of hitmask is set when child k is hit, and tn_out[k] is its entry distance, or +inf for a
+ * child missed (or absent), so the distances sort the hit children first. */
 /* w0 = ox oy oz | ex ey ez nchild, w1 = qlo[x][0..3] qlo[y] qlo[z] qhi[x], w2 = qhi[y] qhi[z] */
 __device__ inline void box4q(const float4 w0, const float4 w1, const float4 w2, V3 id, V3 o, float tmax,
                              float* tn_out, uint32_t& hitmask, int& nchild) {
     const uint32_t e = f2u(w0.w);
     const float sx = u2f((e & 0xffu) << 23), sy = u2f(((e >> 8) & 0xffu) << 23), sz = u2f(((e >> 16) & 0xffu) << 23);
     nchild = (int)(e >> 24);
-    const uint32_t lx = f2u(w1.x), ly = f2u(w1.y), lz = f2u(w1.z);
-    const uint32_t hx = f2u(w1.w), hy = f2u(w2.x), hz = f2u(w2.y);
-    hitmask = 0;
     /* slab distance of plane q: (o_n + q*s - o) / d = q * (s/d) + (o_n - o)/d. s is a power of
      * two, so A = s * (1/d) is exact while it is a normal float, and a plane costs one fma on top
      * of the byte conversion. The node origin and the ray origin are differenced first, so no
      * error grows with the distance of either from the world origin: with B = (o_n - o) * (1/d)
-     * rounded twice, a plane's error is relative to its distance (covered by BOX_LO / BOX_HI)
+     * rounded twice, a plane's error is relative to its distance (covered by the BOX_ margins)
      * plus at most 2^-23 * 255 * |A|, which the builder's outward pad of every child box
      * (bvh.cpp QPAD_REL, 2^-20 of the node extent) covers four times over. |1/d| <= 1e30
      * (safe_dir) and s <= 2^ILM_QUANT_MAX_EXP (scenes that need a larger quantum do not get the
      * 8-bit layout) keep A below 2^124; a subnormal A only adds an absolute error below 2^-140. */
     const float ax = sx * id.x, ay = sy * id.y, az = sz * id.z;
     const float bx = (w0.x - o.x) * id.x, by = (w0.y - o.y) * id.y, bz = (w0.z - o.z) * id.z;
+    /* q -> fma(q, A, B) is non-decreasing for A >= 0 and non-increasing for A < 0 (rounding is
+     * monotonic), so of a child's two planes on an axis the lo plane is the near one exactly when
+     * A >= 0: picking each axis's near and far byte words once per visit gives the same tn and tf
+     * as the min and max of the two planes, for two fewer operations per axis and child. */
+    const uint32_t lx = f2u(w1.x), ly = f2u(w1.y), lz = f2u(w1.z);
+    const uint32_t hx = f2u(w1.w), hy = f2u(w2.x), hz = f2u(w2.y);
+    const bool px = f2u(ax) >> 31 == 0u, py = f2u(ay) >> 31 == 0u, pz = f2u(az) >> 31 == 0u;
+    const uint32_t nx = px ? lx : hx, fx = px ? hx : lx;
+    const uint32_t ny = py ? ly : hy, fy = py ? hy : ly;
+    const uint32_t nz = pz ? lz : hz, fz = pz ? hz : lz;
+    const float tm = tmax * BOX_TMAX;
+    hitmask = 0;
 #pragma unroll
     for (int k = 0; k < 4; k++) {
         const int sh = 8 * k;
-        const float x0 = __builtin_fmaf((float)((lx >> sh) & 0xffu), ax, bx);
-        const float x1 = __builtin_fmaf((float)((hx >> sh) & 0xffu), ax, bx);
-        const float y0 = __builtin_fmaf((float)((ly >> sh) & 0xffu), ay, by);
-        const float y1 = __builtin_fmaf((float)((hy >> sh) & 0xffu), ay, by);
-        const float z0 = __builtin_fmaf((float)((lz >> sh) & 0xffu), az, bz);
-        const float z1 = __builtin_fmaf((float)((hz >> sh) & 0xffu), az, bz);
-        float tn = __builtin_fmaxf(__builtin_fmaxf(__builtin_fminf(x0, x1), __builtin_fminf(y0, y1)),
-                                   __builtin_fmaxf(__builtin_fminf(z0, z1), 0.0f));
-        float tf = __builtin_fminf(__builtin_fminf(__builtin_fmaxf(x0, x1), __builtin_fmaxf(y0, y1)),
-                                   __builtin_fmaxf(z0, z1));
-        tn = tn * BOX_LO;
-        tn_out[k] = tn;
+        const float xn = __builtin_fmaf((float)((nx >> sh) & 0xffu), ax, bx);
+        const float xf = __builtin_fmaf((float)((fx >> sh) & 0xffu), ax, bx);
+        const float yn = __builtin_fmaf((float)((ny >> sh) & 0xffu), ay, by);
+        const float yf = __builtin_fmaf((float)((fy >> sh) & 0xffu), ay, by);
+        const float zn = __builtin_fmaf((float)((nz >> sh) & 0xffu), az, bz);
+        const float zf = __builtin_fmaf((float)((fz >> sh) & 0xffu), az, bz);
+        const float tn = __builtin_fmaxf(__builtin_fmaxf(xn, yn), __builtin_fmaxf(zn, 0.0f));
+        const float tf = __builtin_fminf(__builtin_fminf(xf, yf), zf);
         /* evaluated without short-circuit: a branch per child would split the walk into
          * exec-masked blocks (and make the compiler re-canonicalise the slab values in each) */
-        const uint32_t hit = (uint32_t)(k < nchild) & (uint32_t)(tn <= tf * BOX_HI) & (uint32_t)(tn <= tmax);
-        hitmask |= hit << k;
+        const bool hit = (k < nchild) & (tn <= tf * BOX_K) & (tn <= tm);
+        tn_out[k] = hit ? tn : __builtin_inff();
+        hitmask |= (uint32_t)hit << k;
     }
 }
 
@@ -336,8 +352,7 @@ __device__ inline bool inner_step_words(const float4 n0, const float4 n1, const 
         /* order the hit children near to far with a 4-input sorting network (missed children
          * sort last as +inf), descend into the nearest and push the rest far-to-near: no
          * data-dependent loops, so the lanes of a wave stay together */
-        float k0 = (hm & 1u) ? tn[0] : __builtin_inff(), k1 = (hm & 2u) ? tn[1] : __builtin_inff();
-        float k2 = (hm & 4u) ? tn[2] : __builtin_inff(), k3 = (hm & 8u) ? tn[3] : __builtin_inff();
+        float k0 = tn[0], k1 = tn[1], k2 = tn[2], k3 = tn[3];
         int c0 = ch.x, c1 = ch.y, c2 = ch.z, c3 = ch.w;
 #define ILM_CE(ka, ca, kb, cb)                                        \
         {                                                             \
