@@ -90,7 +90,15 @@ struct DInst {
     int material;
     int flags;          /* ILM_INST_* copied from the mesh */
     int pad;
+    /* the mesh's BLAS, copied from its DMesh so that entering the instance reads this record
+     * only (bytes 112..143: one 32-byte span next to the inverse rows) */
+    const BNode* nodes;
+    const QNode* qnodes;
+    const float4* tris;
+    int root;
+    int qroot;
 };
+static_assert(sizeof(DInst) == 144, "DInst is 9 16-byte words");
 #define ILM_INST_FACE_NORMALS 1
 #define ILM_INST_TEXCOORDS 2
 
@@ -117,8 +125,7 @@ struct DSphere {
 
 struct DScene {
     const BNode* tlas;
-    const QNode* qtlas;
-    const int* tlas_prims;  /* TLAS leaf slot -> instance index */
+    const QNode* qtlas;     /* TLAS leaves (one instance each) hold the instance index as `first` */
     int tlas_root;
     int qtlas_root;
     int n_inst;

@@ -258,6 +258,11 @@ int ilm_scene_upload(ilm_ctx* ctx, const ilm_scene_desc* d, ilm_scene** out) {
         m4_rows(tinv, di.tinv);
         di.mesh = d->instances[i].mesh;
         di.material = d->instances[i].material;
+        di.nodes = dmeshes[di.mesh].nodes;
+        di.qnodes = dmeshes[di.mesh].qnodes;
+        di.tris = dmeshes[di.mesh].tris;
+        di.root = dmeshes[di.mesh].root;
+        di.qroot = dmeshes[di.mesh].qroot;
         di.flags = (d->meshes[di.mesh].face_normals ? ILM_INST_FACE_NORMALS : 0) |
                    (d->meshes[di.mesh].has_texcoords ? ILM_INST_TEXCOORDS : 0);
         for (int k = 0; k < 16; k++) {
@@ -289,7 +294,6 @@ int ilm_scene_upload(ilm_ctx* ctx, const ilm_scene_desc* d, ilm_scene** out) {
         }
     }
     int tlas_depth = 0, tlas_qdepth = 0;
-    std::vector<int> tlas_prims;
     hs.tlas_root = ILM_NO_CHILD;
     hs.qtlas_root = ILM_NO_CHILD;
     if (d->num_instances > 0) {
@@ -301,14 +305,24 @@ int ilm_scene_upload(ilm_ctx* ctx, const ilm_scene_desc* d, ilm_scene** out) {
         tlas_depth = tt.max_depth;
         tlas_qdepth = TQ.max_depth;
         s->quant_max_exp = std::max(s->quant_max_exp, qnodes_max_exponent(TQ.nodes));
-        tlas_prims = tt.order;
-        BNode* bn; QNode* qn; int* tp;
+        /* TLAS leaves hold one instance each: store the instance index itself as the leaf's
+         * `first`, so that an entry reads the DInst record directly */
+        auto direct = [&tt](int32_t c) {
+            if (c == ILM_NO_CHILD || c >= 0) return c;
+            return ilm_leaf_code((uint32_t)tt.order[ilm_leaf_first(c)], 1u);
+        };
+        for (BNode& n : T2.nodes) {
+            n.d.x = direct(n.d.x);
+            n.d.y = direct(n.d.y);
+        }
+        for (QNode& n : TQ.nodes)
+            for (int k = 0; k < 4; k++) n.child[k] = direct(n.child[k]);
+        BNode* bn; QNode* qn;
         HIPTRY(ctx, dupload(s.get(), &bn, T2.nodes.data(), T2.nodes.size()));
         HIPTRY(ctx, dupload(s.get(), &qn, TQ.nodes.data(), TQ.nodes.size()));
-        HIPTRY(ctx, dupload(s.get(), &tp, tlas_prims.data(), tlas_prims.size()));
-        hs.tlas = bn; hs.qtlas = qn; hs.tlas_prims = tp;
-        hs.tlas_root = T2.root;
-        hs.qtlas_root = TQ.root;
+        hs.tlas = bn; hs.qtlas = qn;
+        hs.tlas_root = direct(T2.root);
+        hs.qtlas_root = direct(TQ.root);
     }
     /* device stack bound: pushes never exceed the depth of the current node per level */
     const int cap = ilm_k_lds_stack() + ilm_k_ovf_stack();

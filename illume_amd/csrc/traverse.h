@@ -233,6 +233,26 @@ __device__ inline void load_rows12(const GLOBAL float* src, float* dst) {
     dst[8] = c.x; dst[9] = c.y; dst[10] = c.z; dst[11] = c.w;
 }
 
+/* Entering instance k (a TLAS leaf's `first`): its inverse rows, BLAS pointers and root come
+ * from its one DInst record (bytes 0..47 and 112..143). All five 16-byte words are requested
+ * before the root is tested: the empty asm takes one value of each, so the compiler cannot sink
+ * the matrix and pointer loads behind the test and make an entry wait on memory twice. Returns
+ * the BLAS root (ILM_NO_CHILD for an empty mesh). */
+template <int QUANT>
+__device__ inline int inst_load(const GLOBAL DInst* I, float* inv, const GLOBAL BNode*& nodes,
+                                const GLOBAL QNode*& qnodes, const GLOBAL float4*& tris) {
+    const GLOBAL int4* w = (const GLOBAL int4*)I;
+    const int4 r0 = w[0], r1 = w[1], r2 = w[2], p0 = w[7], p1 = w[8];
+    asm volatile("" ::"v"(r0.x), "v"(r1.x), "v"(r2.x), "v"(p0.x), "v"(p1.x));
+    inv[0] = __int_as_float(r0.x); inv[1] = __int_as_float(r0.y); inv[2] = __int_as_float(r0.z); inv[3] = __int_as_float(r0.w);
+    inv[4] = __int_as_float(r1.x); inv[5] = __int_as_float(r1.y); inv[6] = __int_as_float(r1.z); inv[7] = __int_as_float(r1.w);
+    inv[8] = __int_as_float(r2.x); inv[9] = __int_as_float(r2.y); inv[10] = __int_as_float(r2.z); inv[11] = __int_as_float(r2.w);
+    nodes = (const GLOBAL BNode*)(((uint64_t)(uint32_t)p0.y << 32) | (uint32_t)p0.x);
+    qnodes = (const GLOBAL QNode*)(((uint64_t)(uint32_t)p0.w << 32) | (uint32_t)p0.z);
+    tris = (const GLOBAL float4*)(((uint64_t)(uint32_t)p1.y << 32) | (uint32_t)p1.x);
+    return QUANT ? p1.w : p1.z;
+}
+
 /* Camera ray of path id i, src/renderer.cu:81-110 with the per-pass seeding of :61-69: origin
  * (w = bits of req << 16: no medium), unit direction (w = bits of the pixel) and the PCG32
  * state after the ray's draws. A function of the chunk descriptor and i alone, so the first
@@ -447,20 +467,20 @@ __device__ inline void trace_ray(const GLOBAL DScene* S, V3 O, V3 D, float tmax0
             if (inner_step<QUANT, OVF, COUNT>(nodes, qnodes, node, id, o, B.t, st, tc)) continue;
         } else if (!blas) {
             /* TLAS leaf: one instance; enter its BLAS in object space */
-            const int k = gp(S->tlas_prims)[ilm_leaf_first(node)];
-            const GLOBAL DInst* I = gp(S->inst) + k;
-            const GLOBAL DMesh* M = gp(S->meshes) + I->mesh;
-            const int broot = QUANT ? M->qroot : M->root;
+            const int k = (int)ilm_leaf_first(node);
+            float inv[12];
+            const GLOBAL BNode* bn;
+            const GLOBAL QNode* qn;
+            const GLOBAL float4* tr;
+            const int broot = inst_load<QUANT>(gp(S->inst) + k, inv, bn, qn, tr);
             if (broot != ILM_NO_CHILD) {
-                float inv[12];
-                load_rows12(I->inv, inv);
                 o = xform3(inv, O, 1.0f);
                 d = xform3(inv, D, 0.0f);
                 id = rcp3(d);
                 if (TRI) wt = wt_setup(d);
-                nodes = gp(M->nodes);
-                qnodes = gp(M->qnodes);
-                tris = gp(M->tris);
+                nodes = bn;
+                qnodes = qn;
+                tris = tr;
                 st.push(STACK_SENTINEL);
                 blas = true;
                 cur = k;
@@ -564,20 +584,18 @@ __device__ inline void packet_trace(const GLOBAL DScene* S, V3 O, V3 D, Best& B,
                 pop = false;
             }
         } else if (!blas) {
-            const int k = uni_i(gp(S->tlas_prims)[ilm_leaf_first(node)]);
+            const int k = uni_i((int)ilm_leaf_first(node));
             const GLOBAL DInst* I = gp(S->inst) + k;
-            const int mesh = uni_i(I->mesh);
-            const GLOBAL DMesh* M = gp(S->meshes) + mesh;
-            const int broot = uni_i(QUANT ? M->qroot : M->root);
+            const int broot = uni_i(QUANT ? I->qroot : I->root);
             if (broot != ILM_NO_CHILD) {
                 float inv[12];
                 load_rows12(I->inv, inv);
                 o = xform3(inv, O, 1.0f);
                 d = xform3(inv, D, 0.0f);
                 if (TRI) wt = wt_setup(d);
-                nodes = uni_p(gp(M->nodes));
-                qnodes = uni_p(gp(M->qnodes));
-                tris = uni_p(gp(M->tris));
+                nodes = uni_p(gp(I->nodes));
+                qnodes = uni_p(gp(I->qnodes));
+                tris = uni_p(gp(I->tris));
                 if (sp < PKT_STACK) stk[sp] = STACK_SENTINEL;
                 sp++;
                 blas = true;
@@ -789,21 +807,21 @@ __device__ inline bool seg_run(TState& T, Best& B, Stack<OVF>& st, const GLOBAL 
             if (inner_step<QUANT, OVF, COUNT>(T.nodes, T.qnodes, node, T.id, T.o, B.t, st, tc)) continue;
             goto pop;
         } else if (!T.blas) {
-            const int k = gp(T.S->tlas_prims)[ilm_leaf_first(node)];
-            const GLOBAL DInst* I = gp(T.S->inst) + k;
-            const GLOBAL DMesh* M = gp(T.S->meshes) + I->mesh;
-            const int broot = QUANT ? M->qroot : M->root;
-            if (broot == ILM_NO_CHILD) goto pop;
+            const int k = (int)ilm_leaf_first(node);
             float inv[12];
-            load_rows12(I->inv, inv);
+            const GLOBAL BNode* bn;
+            const GLOBAL QNode* qn;
+            const GLOBAL float4* tr;
+            const int broot = inst_load<QUANT>(gp(T.S->inst) + k, inv, bn, qn, tr);
+            if (broot == ILM_NO_CHILD) goto pop;
             V3 O = T.o, D = T.d;
             T.o = xform3(inv, O, 1.0f);
             T.d = xform3(inv, D, 0.0f);
             T.id = rcp3(T.d);
             if (TRI) T.wt = wt_setup(T.d);
-            T.nodes = gp(M->nodes);
-            T.qnodes = gp(M->qnodes);
-            T.tris = gp(M->tris);
+            T.nodes = bn;
+            T.qnodes = qn;
+            T.tris = tr;
             st.push(STACK_SENTINEL);
             T.blas = 1;
             T.cur = k;
@@ -979,45 +997,53 @@ __device__ inline void pstep_init(PState& T, Best& B, Stack<OVF>& st, const GLOB
 }
 
 /* One action of the walk; true when the ray is finished. UNI: every lane's ray belongs to scene
- * Su (a value uniform over the wave), which then replaces the per-lane T.S. */
+ * Su (a value uniform over the wave), which then replaces the per-lane T.S. A leaf trip tests a
+ * BLAS leaf's triangles; a node trip visits an inner node, and a lane at a TLAS leaf enters the
+ * instance and visits its BLAS root in the same trip (the root's box words are then the trip's
+ * second dependent fetch after the instance record), unless the root is itself a leaf. */
 template <int TRI, int QUANT, int COUNT, int OVF, int UNI>
-__device__ inline bool pstep(PState& T, Best& B, Stack<OVF>& st, TravCount* tc, const GLOBAL DScene* Su) {
+__device__ inline bool pstep(PState& T, Best& B, Stack<OVF>& st, TravCount* tc, const GLOBAL DScene* Su,
+                             const GLOBAL DInst* Iu) {
     const GLOBAL DScene* S = UNI ? Su : T.S;
     int node = T.node;
-    bool pop = true;
-    if (node == POP_MARK) {
-        pop = true;
-    } else if (node >= 0) {
-#if PSTATE_LEAN
-        pop = !inner_step_od<QUANT, OVF, COUNT>(T.nodes, T.qnodes, node, T.o, T.d, B.t, st, tc);
-#else
-        pop = !inner_step<QUANT, OVF, COUNT>(T.nodes, T.qnodes, node, T.id, T.o, B.t, st, tc);
-#endif
-    } else if (!T.blas) {
-        const int k = gp(S->tlas_prims)[ilm_leaf_first(node)];
-        const GLOBAL DInst* I = gp(S->inst) + k;
-        const GLOBAL DMesh* M = gp(S->meshes) + I->mesh;
-        const int broot = QUANT ? M->qroot : M->root;
-        if (broot != ILM_NO_CHILD) {
-            float inv[12];
-            load_rows12(I->inv, inv);
-            T.o = xform3(inv, T.O, 1.0f);
-            T.d = xform3(inv, T.D, 0.0f);
-#if !PSTATE_LEAN
-            T.id = rcp3(T.d);
-#endif
-            if (TRI) T.wt = wt_setup(T.d);
-            T.nodes = gp(M->nodes);
-            T.qnodes = gp(M->qnodes);
-            T.tris = gp(M->tris);
-            st.push(STACK_SENTINEL);
-            T.blas = 1;
-            T.cur = k;
-            node = broot;
-            pop = false;
-        }
-    } else {
+    bool pop;
+    if (node < 0 && T.blas) {
         leaf_tris<TRI, COUNT>(T.tris, node, T.o, T.d, T.wt, T.cur, B, tc);
+        pop = true;
+    } else {
+        pop = node == POP_MARK;
+        if (node < 0) {
+            const int k = (int)ilm_leaf_first(node);
+            float inv[12];
+            const GLOBAL BNode* bn;
+            const GLOBAL QNode* qn;
+            const GLOBAL float4* tr;
+            const int broot = inst_load<QUANT>((UNI ? Iu : gp(S->inst)) + k, inv, bn, qn, tr);
+            if (broot != ILM_NO_CHILD) {
+                T.o = xform3(inv, T.O, 1.0f);
+                T.d = xform3(inv, T.D, 0.0f);
+#if !PSTATE_LEAN
+                T.id = rcp3(T.d);
+#endif
+                if (TRI) T.wt = wt_setup(T.d);
+                T.nodes = bn;
+                T.qnodes = qn;
+                T.tris = tr;
+                st.push(STACK_SENTINEL);
+                T.blas = 1;
+                T.cur = k;
+                node = broot;
+            } else {
+                pop = true;
+            }
+        }
+        if (!pop && node >= 0) {
+#if PSTATE_LEAN
+            pop = !inner_step_od<QUANT, OVF, COUNT>(T.nodes, T.qnodes, node, T.o, T.d, B.t, st, tc);
+#else
+            pop = !inner_step<QUANT, OVF, COUNT>(T.nodes, T.qnodes, node, T.id, T.o, B.t, st, tc);
+#endif
+        }
     }
     if (pop) {
         for (;;) {
@@ -1056,6 +1082,7 @@ __global__ EXT_BOUNDS void k_extend_pers(KParams P, int depth, int* ovf_base) {
     const GLOBAL uint2* inr = gp(P.in.rng);
     const GLOBAL DReq* reqs = gp(P.chunk)->reqs;
     const GLOBAL DScene* Su = UNI ? uni_p(gp(reqs[0].scene)) : nullptr;
+    const GLOBAL DInst* Iu = UNI ? uni_p(gp(Su->inst)) : nullptr;
     const uint32_t refill_min = (uint32_t)P.refill_min;
     const uint32_t leaf_pct = (uint32_t)P.leaf_pct;
     uint32_t ridx = NO_RAY;
@@ -1110,7 +1137,7 @@ __global__ EXT_BOUNDS void k_extend_pers(KParams P, int depth, int* ovf_base) {
         const uint32_t n_rest = (uint32_t)__popcll(__ballot(has)) - n_leaf;
         const bool leaf_trip = n_leaf * 100u >= n_rest * leaf_pct;
         if (has && at_leaf == leaf_trip) {
-            if (pstep<TRI, QUANT, COUNT, OVF, UNI>(T, B, st, &tc, Su)) {
+            if (pstep<TRI, QUANT, COUNT, OVF, UNI>(T, B, st, &tc, Su, Iu)) {
                 write_hit(P, ridx, B);
                 ridx = NO_RAY;
             }
