@@ -319,19 +319,26 @@ __device__ inline void spheres_first(const GLOBAL DScene* S, V3 O, V3 D, Best& B
  * the cap is the next float above s so that a hit at exactly s is still found and shade's
  * comparison decides as it would have. FLT_MAX when there is no cap (no medium, inactive
  * medium, or s not finite). */
-__device__ inline float medium_tmax(const GLOBAL DScene* S, uint32_t tag, uint2 rg, uint32_t pixel) {
+__device__ inline float medium_tmax_m(const GLOBAL DMedium* mediums, uint32_t tag, uint2 rg, uint32_t pixel) {
     const int medium = (int)(tag & 0xffffu) - 1;
     if (medium < 0) return ILM_FLT_MAX;
-    const GLOBAL DMedium* md = gp(S->mediums) + medium;
-    if (!md->active) return ILM_FLT_MAX;
+    /* both fields in one request: the test of `active` does not wait for a second load */
+    const GLOBAL DMedium* md = mediums + medium;
+    const float scattering = md->scattering;
+    const int active = md->active;
+    asm volatile("" ::"v"(scattering), "v"(active));
+    if (!active) return ILM_FLT_MAX;
     uint64_t rs = ((uint64_t)rg.y << 32) | (uint64_t)rg.x;
     const float u = ilm_pcg32_uniform(&rs, ilm_pcg32_inc(pixel));
-    const float sdist = -pm_logf(u) / md->scattering;
+    const float sdist = -pm_logf(u) / scattering;
     if (!(sdist < ILM_FLT_MAX)) return ILM_FLT_MAX;
     /* u = 1 gives sdist = -0.0: the path scatters whatever it hits, and the cap must stay
      * positive (the next float above +0) so that shade's "sdist < t" still holds */
     const float s0 = sdist > 0.0f ? sdist : 0.0f;
     return __uint_as_float(__float_as_uint(s0) + 1u);
+}
+__device__ inline float medium_tmax(const GLOBAL DScene* S, uint32_t tag, uint2 rg, uint32_t pixel) {
+    return medium_tmax_m(gp(S->mediums), tag, rg, pixel);
 }
 
 /* Intersect the triangles of BLAS leaf `node`. A triangle at exactly the distance of the best
@@ -973,16 +980,78 @@ struct PState {
     WTRay wt;
 };
 
-template <int QUANT, int OVF>
-__device__ inline void pstep_init(PState& T, Best& B, Stack<OVF>& st, const GLOBAL DScene* S, V3 O, V3 D, float tmax0) {
+/* The scene values a single-scene chunk's lanes all share, read once per wave (in SGPRs) instead
+ * of at every refill and instance exit: a refill then waits on memory for its ray alone. The
+ * first USCENE_SPHERES spheres (centre, radius) come along. */
+#define USCENE_SPHERES 2
+struct UScene {
+    const GLOBAL DInst* inst;
+    const GLOBAL BNode* tlas;
+    const GLOBAL QNode* qtlas;
+    const GLOBAL DSphere* spheres;
+    const GLOBAL DMedium* mediums;
+    int root;               /* the layout's TLAS root code, POP_MARK when there are no instances */
+    int n_spheres;
+    float4 sph[USCENE_SPHERES];
+};
+__device__ inline float uni_f(float v) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v))); }
+template <int QUANT>
+__device__ inline UScene uscene_load(const GLOBAL DScene* S) {
+    UScene U;
+    U.inst = uni_p(gp(S->inst));
+    U.tlas = uni_p(gp(S->tlas));
+    U.qtlas = uni_p(gp(S->qtlas));
+    U.spheres = uni_p(gp(S->spheres));
+    U.mediums = uni_p(gp(S->mediums));
+    const int root = uni_i(QUANT ? S->qtlas_root : S->tlas_root);
+    U.root = root == ILM_NO_CHILD ? POP_MARK : root;
+    U.n_spheres = uni_i(S->n_spheres);
+#pragma unroll
+    for (int k = 0; k < USCENE_SPHERES; k++) {
+        U.sph[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (k < U.n_spheres) {
+            const GLOBAL DSphere* q = U.spheres + k;
+            U.sph[k] = make_float4(uni_f(q->c[0]), uni_f(q->c[1]), uni_f(q->c[2]), uni_f(q->r));
+        }
+    }
+    return U;
+}
+/* spheres_first over the snapshot: the same spheres in the same order and comparisons */
+__device__ inline void spheres_first_u(const UScene& U, V3 O, V3 D, Best& B) {
+#pragma unroll
+    for (int k = 0; k < USCENE_SPHERES; k++) {
+        if (k < U.n_spheres) {
+            DSphere sp;
+            sp.c[0] = U.sph[k].x; sp.c[1] = U.sph[k].y; sp.c[2] = U.sph[k].z; sp.r = U.sph[k].w;
+            float t;
+            if (sphere_hit(sp, O, D, t) && t < B.t) { B.t = t; B.inst = -2 - k; }
+        }
+    }
+    for (int k = USCENE_SPHERES; k < U.n_spheres; k++) {
+        DSphere sp = U.spheres[k];
+        float t;
+        if (sphere_hit(sp, O, D, t) && t < B.t) { B.t = t; B.inst = -2 - k; }
+    }
+}
+
+template <int QUANT, int OVF, int UNI>
+__device__ inline void pstep_init(PState& T, Best& B, Stack<OVF>& st, const GLOBAL DScene* S, const UScene& U, V3 O,
+                                  V3 D, float tmax0) {
     best_init(B);
     B.t = tmax0;
-    spheres_first(S, O, D, B);
+    if (UNI) {
+        spheres_first_u(U, O, D, B);
+        T.node = U.root;
+        T.nodes = U.tlas;
+        T.qnodes = U.qtlas;
+    } else {
+        spheres_first(S, O, D, B);
+        T.node = QUANT ? S->qtlas_root : S->tlas_root;
+        if (T.node == ILM_NO_CHILD) T.node = POP_MARK;
+        T.nodes = gp(S->tlas);
+        T.qnodes = gp(S->qtlas);
+    }
     T.S = S;
-    T.node = QUANT ? S->qtlas_root : S->tlas_root;
-    if (T.node == ILM_NO_CHILD) T.node = POP_MARK;
-    T.nodes = gp(S->tlas);
-    T.qnodes = gp(S->qtlas);
     T.tris = nullptr;
     T.blas = 0;
     T.cur = -1;
@@ -1002,9 +1071,8 @@ __device__ inline void pstep_init(PState& T, Best& B, Stack<OVF>& st, const GLOB
  * instance and visits its BLAS root in the same trip (the root's box words are then the trip's
  * second dependent fetch after the instance record), unless the root is itself a leaf. */
 template <int TRI, int QUANT, int COUNT, int OVF, int UNI>
-__device__ inline bool pstep(PState& T, Best& B, Stack<OVF>& st, TravCount* tc, const GLOBAL DScene* Su,
-                             const GLOBAL DInst* Iu) {
-    const GLOBAL DScene* S = UNI ? Su : T.S;
+__device__ inline bool pstep(PState& T, Best& B, Stack<OVF>& st, TravCount* tc, const UScene& U) {
+    const GLOBAL DScene* S = T.S;
     int node = T.node;
     bool pop;
     if (node < 0 && T.blas) {
@@ -1018,7 +1086,7 @@ __device__ inline bool pstep(PState& T, Best& B, Stack<OVF>& st, TravCount* tc, 
             const GLOBAL BNode* bn;
             const GLOBAL QNode* qn;
             const GLOBAL float4* tr;
-            const int broot = inst_load<QUANT>((UNI ? Iu : gp(S->inst)) + k, inv, bn, qn, tr);
+            const int broot = inst_load<QUANT>((UNI ? U.inst : gp(S->inst)) + k, inv, bn, qn, tr);
             if (broot != ILM_NO_CHILD) {
                 T.o = xform3(inv, T.O, 1.0f);
                 T.d = xform3(inv, T.D, 0.0f);
@@ -1056,8 +1124,8 @@ __device__ inline bool pstep(PState& T, Best& B, Stack<OVF>& st, TravCount* tc, 
             T.id = rcp3(T.D);
 #endif
             T.blas = 0;
-            T.nodes = gp(S->tlas);
-            T.qnodes = gp(S->qtlas);
+            T.nodes = UNI ? U.tlas : gp(S->tlas);
+            T.qnodes = UNI ? U.qtlas : gp(S->qtlas);
         }
     }
     T.node = node;
@@ -1082,7 +1150,8 @@ __global__ EXT_BOUNDS void k_extend_pers(KParams P, int depth, int* ovf_base) {
     const GLOBAL uint2* inr = gp(P.in.rng);
     const GLOBAL DReq* reqs = gp(P.chunk)->reqs;
     const GLOBAL DScene* Su = UNI ? uni_p(gp(reqs[0].scene)) : nullptr;
-    const GLOBAL DInst* Iu = UNI ? uni_p(gp(Su->inst)) : nullptr;
+    UScene U = {};
+    if (UNI) U = uscene_load<QUANT>(Su);
     const uint32_t refill_min = (uint32_t)P.refill_min;
     const uint32_t leaf_pct = (uint32_t)P.leaf_pct;
     uint32_t ridx = NO_RAY;
@@ -1114,12 +1183,17 @@ __global__ EXT_BOUNDS void k_extend_pers(KParams P, int depth, int* ovf_base) {
                         uint2 rg;
                         camera_ray(P, r, o4, d4, rg);
                         const GLOBAL DScene* S = UNI ? Su : gp(reqs[f2u(o4.w) >> 16].scene);
-                        pstep_init<QUANT, OVF>(T, B, st, S, mk3(o4.x, o4.y, o4.z), mk3(d4.x, d4.y, d4.z), ILM_FLT_MAX);
+                        pstep_init<QUANT, OVF, UNI>(T, B, st, S, U, mk3(o4.x, o4.y, o4.z), mk3(d4.x, d4.y, d4.z),
+                                                    ILM_FLT_MAX);
                     } else {
-                        float4 o4 = ino[r], d4 = ind[r];
+                        /* the PCG state is read with the ray (it is needed for rays inside a medium) */
+                        const float4 o4 = ino[r], d4 = ind[r];
+                        const uint2 rg = inr[r];
+                        asm volatile("" ::"v"(o4.w), "v"(d4.w), "v"(rg.x));
                         const GLOBAL DScene* S = UNI ? Su : gp(reqs[f2u(o4.w) >> 16].scene);
-                        pstep_init<QUANT, OVF>(T, B, st, S, mk3(o4.x, o4.y, o4.z), mk3(d4.x, d4.y, d4.z),
-                                               medium_tmax(S, f2u(o4.w), inr[r], f2u(d4.w)));
+                        pstep_init<QUANT, OVF, UNI>(T, B, st, S, U, mk3(o4.x, o4.y, o4.z), mk3(d4.x, d4.y, d4.z),
+                                                    medium_tmax_m(UNI ? U.mediums : gp(S->mediums), f2u(o4.w), rg,
+                                                                  f2u(d4.w)));
                     }
                 }
             }
@@ -1137,7 +1211,7 @@ __global__ EXT_BOUNDS void k_extend_pers(KParams P, int depth, int* ovf_base) {
         const uint32_t n_rest = (uint32_t)__popcll(__ballot(has)) - n_leaf;
         const bool leaf_trip = n_leaf * 100u >= n_rest * leaf_pct;
         if (has && at_leaf == leaf_trip) {
-            if (pstep<TRI, QUANT, COUNT, OVF, UNI>(T, B, st, &tc, Su, Iu)) {
+            if (pstep<TRI, QUANT, COUNT, OVF, UNI>(T, B, st, &tc, U)) {
                 write_hit(P, ridx, B);
                 ridx = NO_RAY;
             }
