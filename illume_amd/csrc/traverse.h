@@ -993,6 +993,8 @@ struct UScene {
     int root;               /* the layout's TLAS root code, POP_MARK when there are no instances */
     int n_spheres;
     float4 sph[USCENE_SPHERES];
+    float4 r0, r1, r2;      /* the TLAS root's box words and child codes, when the root is an */
+    int4 rch;               /* inner node (root >= 0) */
 };
 __device__ inline float uni_f(float v) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v))); }
 template <int QUANT>
@@ -1005,6 +1007,17 @@ __device__ inline UScene uscene_load(const GLOBAL DScene* S) {
     U.mediums = uni_p(gp(S->mediums));
     const int root = uni_i(QUANT ? S->qtlas_root : S->tlas_root);
     U.root = root == ILM_NO_CHILD ? POP_MARK : root;
+    U.r0 = U.r1 = U.r2 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    U.rch = make_int4(0, 0, 0, 0);
+    if (QUANT && U.root >= 0 && U.root != POP_MARK) {
+        float4 a, b, c;
+        int4 ch;
+        load_node<QUANT>(U.tlas, U.qtlas, U.root, a, b, c, ch);
+        U.r0 = make_float4(uni_f(a.x), uni_f(a.y), uni_f(a.z), uni_f(a.w));
+        U.r1 = make_float4(uni_f(b.x), uni_f(b.y), uni_f(b.z), uni_f(b.w));
+        U.r2 = make_float4(uni_f(c.x), uni_f(c.y), uni_f(c.z), uni_f(c.w));
+        U.rch = make_int4(uni_i(ch.x), uni_i(ch.y), uni_i(ch.z), uni_i(ch.w));
+    }
     U.n_spheres = uni_i(S->n_spheres);
 #pragma unroll
     for (int k = 0; k < USCENE_SPHERES; k++) {
@@ -1034,9 +1047,9 @@ __device__ inline void spheres_first_u(const UScene& U, V3 O, V3 D, Best& B) {
     }
 }
 
-template <int QUANT, int OVF, int UNI>
+template <int QUANT, int OVF, int UNI, int COUNT>
 __device__ inline void pstep_init(PState& T, Best& B, Stack<OVF>& st, const GLOBAL DScene* S, const UScene& U, V3 O,
-                                  V3 D, float tmax0) {
+                                  V3 D, float tmax0, TravCount* tc) {
     best_init(B);
     B.t = tmax0;
     if (UNI) {
@@ -1063,6 +1076,14 @@ __device__ inline void pstep_init(PState& T, Best& B, Stack<OVF>& st, const GLOB
     T.id = rcp3(D);
 #endif
     st.sp = 0;
+    if (UNI && QUANT && U.root >= 0 && U.root != POP_MARK) {
+        /* the TLAS root's words are in registers: visit it now, so the ray's first trip starts
+         * one level down (POP_MARK when no child is hit: the next trip ends the ray). 8-bit
+         * layout only: in the FP32 kernel the extra code costs a wave of occupancy. */
+        int node = U.root;
+        T.node = inner_step_words<QUANT, OVF, COUNT>(U.r0, U.r1, U.r2, U.rch, node, rcp3(D), O, B.t, st, tc)
+                     ? node : POP_MARK;
+    }
 }
 
 /* One action of the walk; true when the ray is finished. UNI: every lane's ray belongs to scene
@@ -1183,17 +1204,18 @@ __global__ EXT_BOUNDS void k_extend_pers(KParams P, int depth, int* ovf_base) {
                         uint2 rg;
                         camera_ray(P, r, o4, d4, rg);
                         const GLOBAL DScene* S = UNI ? Su : gp(reqs[f2u(o4.w) >> 16].scene);
-                        pstep_init<QUANT, OVF, UNI>(T, B, st, S, U, mk3(o4.x, o4.y, o4.z), mk3(d4.x, d4.y, d4.z),
-                                                    ILM_FLT_MAX);
+                        pstep_init<QUANT, OVF, UNI, COUNT>(T, B, st, S, U, mk3(o4.x, o4.y, o4.z),
+                                                           mk3(d4.x, d4.y, d4.z), ILM_FLT_MAX, &tc);
                     } else {
                         /* the PCG state is read with the ray (it is needed for rays inside a medium) */
                         const float4 o4 = ino[r], d4 = ind[r];
                         const uint2 rg = inr[r];
                         asm volatile("" ::"v"(o4.w), "v"(d4.w), "v"(rg.x));
                         const GLOBAL DScene* S = UNI ? Su : gp(reqs[f2u(o4.w) >> 16].scene);
-                        pstep_init<QUANT, OVF, UNI>(T, B, st, S, U, mk3(o4.x, o4.y, o4.z), mk3(d4.x, d4.y, d4.z),
-                                                    medium_tmax_m(UNI ? U.mediums : gp(S->mediums), f2u(o4.w), rg,
-                                                                  f2u(d4.w)));
+                        pstep_init<QUANT, OVF, UNI, COUNT>(T, B, st, S, U, mk3(o4.x, o4.y, o4.z),
+                                                           mk3(d4.x, d4.y, d4.z),
+                                                           medium_tmax_m(UNI ? U.mediums : gp(S->mediums), f2u(o4.w),
+                                                                         rg, f2u(d4.w)), &tc);
                     }
                 }
             }
