@@ -947,16 +947,26 @@ static void build_chunk(ilm_ctx* ctx, const std::vector<Job*>& jobs, DChunk& c, 
     int nreq = 0;
     std::vector<int64_t> quota(jobs.size(), ctx->cap);
     if (ctx->opts.sched_fair && jobs.size() > 1) fair_quotas(jobs, ctx->cap, quota);
-    for (size_t jk = 0; jk < jobs.size(); jk++) {
-        Job* j = jobs[jk];
-        if (!job_has_work(j) || j->cancel.load()) continue;
-        if (nreq >= ILM_MAX_JOBS || c.n_work >= ILM_MAX_WORK || left <= 0) break;
-        const int64_t lim = std::min(left, std::max<int64_t>(quota[jk], 1));
-        const uint32_t before = c.total_paths;
-        add_slice(ctx, j, lim, nreq, c);
-        left -= (int64_t)(c.total_paths - before);
-        nreq++;
-        in_chunk.push_back(j);
+    /* First the requests that render the scene of the first request with work, so that a chunk
+     * usually holds one scene (the extend kernel then reads that scene's values once per wave);
+     * requests of other scenes join only a chunk that would otherwise stay under half full. */
+    const ilm_scene* s0 = nullptr;
+    for (Job* j : jobs)
+        if (job_has_work(j) && !j->cancel.load()) { s0 = j->scene; break; }
+    for (int round = 0; round < 2; round++) {
+        if (round == 1 && (int64_t)c.total_paths * 2 >= ctx->cap) break;
+        for (size_t jk = 0; jk < jobs.size(); jk++) {
+            Job* j = jobs[jk];
+            if (!job_has_work(j) || j->cancel.load()) continue;
+            if ((j->scene == s0) != (round == 0)) continue;
+            if (nreq >= ILM_MAX_JOBS || c.n_work >= ILM_MAX_WORK || left <= 0) break;
+            const int64_t lim = std::min(left, std::max<int64_t>(quota[jk], 1));
+            const uint32_t before = c.total_paths;
+            add_slice(ctx, j, lim, nreq, c);
+            left -= (int64_t)(c.total_paths - before);
+            nreq++;
+            in_chunk.push_back(j);
+        }
     }
 }
 
