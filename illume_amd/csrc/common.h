@@ -97,8 +97,13 @@ struct DInst {
     const float4* tris;
     int root;
     int qroot;
+    /* the mesh's vertex data, for shading (bytes 144..175) */
+    const int* idx;
+    const float* pos;
+    const float* nrm;
+    const float* uv;
 };
-static_assert(sizeof(DInst) == 144, "DInst is 9 16-byte words");
+static_assert(sizeof(DInst) == 176, "DInst is 11 16-byte words");
 #define ILM_INST_FACE_NORMALS 1
 #define ILM_INST_TEXCOORDS 2
 

@@ -263,6 +263,10 @@ int ilm_scene_upload(ilm_ctx* ctx, const ilm_scene_desc* d, ilm_scene** out) {
         di.tris = dmeshes[di.mesh].tris;
         di.root = dmeshes[di.mesh].root;
         di.qroot = dmeshes[di.mesh].qroot;
+        di.idx = dmeshes[di.mesh].idx;
+        di.pos = dmeshes[di.mesh].pos;
+        di.nrm = dmeshes[di.mesh].nrm;
+        di.uv = dmeshes[di.mesh].uv;
         di.flags = (d->meshes[di.mesh].face_normals ? ILM_INST_FACE_NORMALS : 0) |
                    (d->meshes[di.mesh].has_texcoords ? ILM_INST_TEXCOORDS : 0);
         for (int k = 0; k < 16; k++) {

@@ -110,12 +110,20 @@ __device__ inline V3 conductor3(const GLOBAL DMat* m, float cosI) {
                fresnel_conductor(m->eta[2], m->k[2], cosI));
 }
 
+/* Scene arrays a shading step reads: per path from its scene, or once per block for a chunk
+ * of one scene. */
+struct ShadeRefs {
+    const GLOBAL DInst* inst;
+    const GLOBAL DMat* mats;
+    const GLOBAL DMedium* mediums;
+};
+
 /* One path segment after its closest hit: src/renderer.cu:152-386. */
-__device__ inline int shade_path(const GLOBAL DScene* S, const float ray_bias, const float4 h4, int hinst, V3& ro, V3& rd, V3& color, int& medium, uint64_t& rs, uint64_t inc) {
+__device__ inline int shade_path(const GLOBAL DScene* S, const ShadeRefs& SR, const float ray_bias, const float4 h4, int hinst, V3& ro, V3& rd, V3& color, int& medium, uint64_t& rs, uint64_t inc) {
     const float tmin = h4.x;
     const bool is = hinst != -1;
     if (medium >= 0) {
-        const GLOBAL DMedium* md = gp(S->mediums) + medium;
+        const GLOBAL DMedium* md = SR.mediums + medium;
         if (md->active) {
             const float sig_s = md->scattering;
             const float a0 = md->absorption[0], a1 = md->absorption[1], a2 = md->absorption[2];
@@ -147,38 +155,37 @@ __device__ inline int shade_path(const GLOBAL DScene* S, const float ray_bias, c
     float hu, hv;
     int mat;
     if (hinst >= 0) {
-        const GLOBAL DInst* I = gp(S->inst) + hinst;
-        const int flags = I->flags;
+        /* the instance record's flags, material and vertex pointers are requested together
+         * (bytes 96..111 and 144..159), so the vertex fetches wait on one round trip, not on the
+         * record and then a mesh record */
+        const GLOBAL DInst* I = SR.inst + hinst;
+        const GLOBAL int4* iw = (const GLOBAL int4*)I;
+        const int4 misc = iw[6], vp0 = iw[9];
+        asm volatile("" ::"v"(misc.x), "v"(vp0.x));
+        const int flags = misc.z;
+        const GLOBAL int* idx = (const GLOBAL int*)(((uint64_t)(uint32_t)vp0.y << 32) | (uint32_t)vp0.x);
+        const GLOBAL float* pos = (const GLOBAL float*)(((uint64_t)(uint32_t)vp0.w << 32) | (uint32_t)vp0.z);
         const int prim = __float_as_int(h4.w);
         const float bu = h4.y, bv = h4.z;
+        const int i0 = idx[3 * prim], i1 = idx[3 * prim + 1], i2 = idx[3 * prim + 2];
         if (flags & ILM_INST_FACE_NORMALS) {
             /* cross(v1 - v0, v2 - v0) of the triangle's vertices, as src/primitives/mesh.cu:541 */
-            const GLOBAL DMesh* M = gp(S->meshes) + I->mesh;
-            const GLOBAL int* idx = gp(M->idx);
-            const GLOBAL float* pos = gp(M->pos);
-            int i0 = idx[3 * prim], i1 = idx[3 * prim + 1], i2 = idx[3 * prim + 2];
             V3 p0 = mk3(pos[3 * i0], pos[3 * i0 + 1], pos[3 * i0 + 2]);
             V3 p1 = mk3(pos[3 * i1], pos[3 * i1 + 1], pos[3 * i1 + 2]);
             V3 p2 = mk3(pos[3 * i2], pos[3 * i2 + 1], pos[3 * i2 + 2]);
             n = normalize3(cross3(sub3(p1, p0), sub3(p2, p0)));
         } else {
-            const GLOBAL DMesh* M = gp(S->meshes) + I->mesh;
-            const GLOBAL int* idx = gp(M->idx);
-            const GLOBAL float* nrm = gp(M->nrm);
-            int i0 = idx[3 * prim], i1 = idx[3 * prim + 1], i2 = idx[3 * prim + 2];
+            const GLOBAL float* nrm = gp(I->nrm);
             V3 n0 = mk3(nrm[3 * i0], nrm[3 * i0 + 1], nrm[3 * i0 + 2]);
             V3 n1 = mk3(nrm[3 * i1], nrm[3 * i1 + 1], nrm[3 * i1 + 2]);
             V3 n2 = mk3(nrm[3 * i2], nrm[3 * i2 + 1], nrm[3 * i2 + 2]);
             n = normalize3(add3(mul3(n0, 1.0f - bu - bv), add3(mul3(n1, bu), mul3(n2, bv))));
         }
         if (flags & ILM_INST_TEXCOORDS) {
-            const GLOBAL DMesh* M = gp(S->meshes) + I->mesh;
-            const GLOBAL int* idx = gp(M->idx);
-            const GLOBAL float* uv = gp(M->uv);
-            int i0 = idx[3 * prim], i1 = idx[3 * prim + 1], i2 = idx[3 * prim + 2];
+            const GLOBAL float* uvp = gp(I->uv);
             float w = 1.0f - bu - bv;
-            hu = (uv[2 * i0] * w + uv[2 * i1] * bu) + uv[2 * i2] * bv;
-            hv = (uv[2 * i0 + 1] * w + uv[2 * i1 + 1] * bu) + uv[2 * i2 + 1] * bv;
+            hu = (uvp[2 * i0] * w + uvp[2 * i1] * bu) + uvp[2 * i2] * bv;
+            hv = (uvp[2 * i0 + 1] * w + uvp[2 * i1 + 1] * bu) + uvp[2 * i2 + 1] * bv;
         } else {
             hu = bu;
             hv = bv;
@@ -186,7 +193,7 @@ __device__ inline int shade_path(const GLOBAL DScene* S, const float ray_bias, c
         float ti[12];
         load_rows12(I->tinv, ti);
         n = normalize3(xform3(ti, n, 0.0f));
-        mat = I->material;
+        mat = misc.y;
     } else {
         const GLOBAL DSphere* sp = gp(S->spheres) + (-2 - hinst);
         V3 pos = add3(ro, mul3(rd, tmin));
@@ -195,7 +202,7 @@ __device__ inline int shade_path(const GLOBAL DScene* S, const float ray_bias, c
         hv = (float)(0.5f - (double)pm_asinf(n.y) * ILM_INV_PI_D);
         mat = sp->material;
     }
-    const GLOBAL DMat* m = gp(S->mats) + mat;
+    const GLOBAL DMat* m = SR.mats + mat;
     const int type = m->type;
     const V3 d = rd;
     V3 new_dir;
@@ -419,6 +426,15 @@ __global__ SHADE_BOUNDS void k_shade(KParams P, int depth) {
     const int lane = threadIdx.x & 63;
     const int wid = threadIdx.x >> 6;
     const GLOBAL DReq* reqs = gp(P.chunk)->reqs;
+    /* a chunk of one scene: its arrays are read once per block */
+    const GLOBAL DScene* Su = nullptr;
+    ShadeRefs SRu = {nullptr, nullptr, nullptr};
+    if (P.uni_scene) {
+        Su = uni_p(gp(reqs[0].scene));
+        SRu.inst = uni_p(gp(Su->inst));
+        SRu.mats = uni_p(gp(Su->mats));
+        SRu.mediums = uni_p(gp(Su->mediums));
+    }
     const GLOBAL float4* ino = gp(P.in.o);
     const GLOBAL float4* ind = gp(P.in.d);
     const GLOBAL float4* inc4 = gp(P.in.c);
@@ -468,11 +484,13 @@ __global__ SHADE_BOUNDS void k_shade(KParams P, int depth) {
             uint32_t pixel = f2u(d4.w);
             uint32_t pid = f2u(c4.w);
             const GLOBAL DReq* R = reqs + req;
-            const GLOBAL DScene* S = gp(R->scene);
+            const GLOBAL DScene* S = P.uni_scene ? Su : gp(R->scene);
+            ShadeRefs SR = SRu;
+            if (!P.uni_scene) { SR.inst = gp(S->inst); SR.mats = gp(S->mats); SR.mediums = gp(S->mediums); }
             uint64_t rs = ((uint64_t)rg.y << 32) | (uint64_t)rg.x;
             V3 ro = mk3(o4.x, o4.y, o4.z), rd = mk3(d4.x, d4.y, d4.z), color = mk3(c4.x, c4.y, c4.z);
             if (P.seg_count) gpw(P.seg_count)[pid] += 1;
-            status = shade_path(S, R->ray_bias, h4, hinst, ro, rd, color, medium, rs, ilm_pcg32_inc(pixel));
+            status = shade_path(S, SR, R->ray_bias, h4, hinst, ro, rd, color, medium, rs, ilm_pcg32_inc(pixel));
             if (status == ST_EMIT) gpw(P.contrib)[pid] = make_float4(color.x, color.y, color.z, 0.0f);
             if (status == ST_ALIVE && depth + 1 >= R->max_depth) status = ST_DEAD;   /* src/renderer.cu:549 */
             oo = make_float4(ro.x, ro.y, ro.z, u2f(((uint32_t)req << 16) | (uint32_t)(medium + 1)));
