@@ -302,6 +302,95 @@ __device__ inline void camera_ray(const KParams& P, uint32_t i, float4& o4, floa
     rg = make_uint2((uint32_t)rs, (uint32_t)(rs >> 32));
 }
 
+/* camera_ray for a chunk of one work item (a whole plain render is one): the item, its request's
+ * pixel walk and image plane, and the scene's camera are the same for every path, so they are
+ * read once per wave into registers and a camera ray is then formed without a memory round trip
+ * (same operations, same results as camera_ray). */
+struct CamU {
+    uint32_t pid_base, px0, npx, sg_log2;
+    int pass0;
+    UDiv span_div;
+    int req, width;
+    float pixel_size, left, top;
+    uint32_t tile_th, tile_full;
+    UDiv w_div, band_div, th_div;
+    float cam[12];
+    float dof, aperture;
+};
+__device__ inline UDiv uni_div(UDiv d) {
+    UDiv r;
+    r.lo = (uint32_t)uni_i((int)d.lo);
+    r.hi = (uint32_t)uni_i((int)d.hi);
+    return r;
+}
+__device__ inline float uni_fl(float v) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v))); }
+/* false (U unset) unless the chunk is one work item of generated camera rays */
+__device__ inline bool camu_load(const KParams& P, CamU& U) {
+    if (P.list) return false;
+    const GLOBAL DChunk* C = gp(P.chunk);
+    if (uni_i(C->n_work) != 1) return false;
+    const GLOBAL DWork* W = C->work;
+    U.pid_base = (uint32_t)uni_i((int)W->pid_base);
+    U.px0 = (uint32_t)uni_i((int)W->px0);
+    U.npx = (uint32_t)uni_i((int)W->npx);
+    U.sg_log2 = (uint32_t)uni_i((int)W->sg_log2);
+    U.pass0 = uni_i(W->pass0);
+    U.span_div = uni_div(W->span_div);
+    U.req = uni_i(W->req);
+    const GLOBAL DReq* R = C->reqs + U.req;
+    U.width = uni_i(R->width);
+    U.pixel_size = uni_fl(R->pixel_size);
+    U.left = uni_fl(R->left);
+    U.top = uni_fl(R->top);
+    U.tile_th = (uint32_t)uni_i((int)R->tile_th);
+    U.tile_full = (uint32_t)uni_i((int)R->tile_full);
+    U.w_div = uni_div(R->w_div);
+    U.band_div = uni_div(R->band_div);
+    U.th_div = uni_div(R->th_div);
+    const GLOBAL DScene* S = gp(R->scene);
+    for (int k = 0; k < 12; k++) U.cam[k] = uni_fl(S->cam[k]);
+    U.dof = uni_fl(S->dof);
+    U.aperture = uni_fl(S->aperture);
+    return true;
+}
+__device__ inline void camera_ray_u(const CamU& U, uint32_t i, float4& o4, float4& d4, uint2& rg) {
+    const uint32_t l = i - U.pid_base;
+    const uint32_t sh = U.sg_log2, span = U.npx << sh;
+    const uint32_t kb = udiv(l, U.span_div), r = l - kb * span;
+    const uint32_t q = r >> sh;
+    const uint32_t k = (kb << sh) + (r - (q << sh));
+    const uint32_t pass = (uint32_t)U.pass0 + k;
+    uint32_t pixel = U.px0 + q;
+    /* tile_pixel_r */
+    if (U.tile_th > 1 && pixel < U.tile_full) {
+        const uint32_t th = U.tile_th, w = (uint32_t)U.width, band_px = th * w;
+        const uint32_t band = udiv(pixel, U.band_div), rr = pixel - band * band_px;
+        const uint32_t x = udiv(rr, U.th_div), y = band * th + (rr - x * th);
+        pixel = y * w + x;
+    }
+    uint64_t rs = ilm_path_seed(pass, pixel);
+    const uint64_t inc = ilm_pcg32_inc(pixel);
+    const float pixel_size = U.pixel_size;
+    const uint32_t yq = udiv(pixel, U.w_div);
+    int x = (int)(pixel - yq * (uint32_t)U.width), y = (int)yq;
+    float left_edge = U.left + pixel_size * (float)x;
+    float top_edge = U.top - pixel_size * (float)y;
+    float rx = left_edge + pixel_size * ilm_pcg32_uniform(&rs, inc);
+    float ry = top_edge - pixel_size * ilm_pcg32_uniform(&rs, inc);
+    V3 pos = mk3(0.0f, 0.0f, 0.0f);
+    if (U.aperture > ILM_FLT_EPSILON) {
+        float u1 = ilm_pcg32_uniform(&rs, inc);
+        float u2 = ilm_pcg32_uniform(&rs, inc);
+        pos = mul3(sample_circle(u1, u2), U.aperture);
+    }
+    V3 origin = xform3(U.cam, pos, 1.0f);
+    V3 image_pos = xform3(U.cam, mk3(rx, ry, U.dof), 1.0f);
+    V3 dir = normalize3(sub3(image_pos, origin));
+    o4 = make_float4(origin.x, origin.y, origin.z, u2f((uint32_t)U.req << 16));
+    d4 = make_float4(dir.x, dir.y, dir.z, u2f(pixel));
+    rg = make_uint2((uint32_t)rs, (uint32_t)(rs >> 32));
+}
+
 __device__ inline void spheres_first(const GLOBAL DScene* S, V3 O, V3 D, Best& B) {
     const int ns = S->n_spheres;
     const GLOBAL DSphere* sph = gp(S->spheres);
@@ -636,7 +725,7 @@ __device__ inline void packet_trace(const GLOBAL DScene* S, V3 O, V3 D, Best& B,
  * Every path id of the chunk passes through here once, so the kernel also clears the per-path
  * results (the chunk needs no separate clear). */
 template <int TRI, int QUANT, int COUNT>
-__global__ __launch_bounds__(EXT_BLOCK) void k_extend_packet(KParams P, int depth) {
+__global__ __launch_bounds__(EXT_BLOCK) __attribute__((amdgpu_waves_per_eu(7, 8))) void k_extend_packet(KParams P, int depth) {
     __shared__ int stk[PKT_STACK];
     const int lane = threadIdx.x;
     GLOBAL DCtl* ctl = gpw(P.ctl);
@@ -645,6 +734,9 @@ __global__ __launch_bounds__(EXT_BLOCK) void k_extend_packet(KParams P, int dept
     const GLOBAL DReq* reqs = gp(P.chunk)->reqs;
     TravCount tc = {0u, 0u};
     const uint32_t guide = gridDim.x * 2u;
+    CamU U;
+    const bool cu = camu_load(P, U);
+    const uint64_t su = cu ? (uint64_t)uni_p(gp(reqs[U.req].scene)) : 0ull;
     /* the first reservation is fixed by the wave's index; the cursor hands out the rest (one
      * address's atomics serialise, so a wave with nothing to do should not touch it) */
     const uint32_t first = gridDim.x * fetch;
@@ -682,8 +774,13 @@ __global__ __launch_bounds__(EXT_BLOCK) void k_extend_packet(KParams P, int dept
         Best B;
         best_init(B);
         if (valid) {
-            camera_ray(P, r, o4, d4, rg);
-            sbits = (uint64_t)gp(reqs[f2u(o4.w) >> 16].scene);
+            if (cu) {
+                camera_ray_u(U, r, o4, d4, rg);
+                sbits = su;
+            } else {
+                camera_ray(P, r, o4, d4, rg);
+                sbits = (uint64_t)gp(reqs[f2u(o4.w) >> 16].scene);
+            }
         }
         unsigned long long rem = __ballot(valid);
         while (rem) {
@@ -726,6 +823,9 @@ __global__ EXT_BOUNDS void k_extend(KParams P, int depth, int* ovf_base) {
     /* guided self-scheduling: reservations shrink with the rays left in the queue (down to one
      * batch), so the last waves of a launch finish together; ext_fetch is the upper bound */
     const uint32_t guide = gridDim.x * 2u;
+    CamU U;
+    const bool cu = camu_load(P, U);
+    const uint64_t su = cu ? (uint64_t)uni_p(gp(reqs[U.req].scene)) : 0ull;
     /* the first reservation is fixed by the wave's index; the cursor hands out the rest (one
      * address's atomics serialise, so a wave with nothing to do should not touch it) */
     const uint32_t first = gridDim.x * fetch;
