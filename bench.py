@@ -211,7 +211,7 @@ def base_line(args, metric, value, unit, ms_per_step, config, dtype="f32"):
 
 def ctx_opts(args):
     o = dict(bvh_layout=args.layout, tri_test=args.tri, paths_in_flight=args.paths, extend_mode=args.extend_mode,
-             use_graphs=0 if args.no_graphs else 1)
+             use_graphs=0 if args.no_graphs else 1, pipes=args.pipes)
     if args.sample_group is not None:
         o["sample_group"] = args.sample_group
     if args.tile_rows is not None:
@@ -664,6 +664,8 @@ def main():
     ap.add_argument("--no-rmse", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-count", action="store_true")
+    ap.add_argument("--pipes", type=int, default=0, choices=[0, 1, 2],
+                    help="chunk pipelines (0: library default, two; 1: one stream)")
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch every kernel directly (for counter collection); the default replays hipGraphs")
     ap.add_argument("--calibrate-copy", action="store_true", help="only run the 1 GiB copy used to scale HBM counters")

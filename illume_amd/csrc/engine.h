@@ -98,6 +98,8 @@ struct ProfileSink {
 /* Device buffers one chunk pipeline runs on: the context's own (allocated at ilm_open), or those
  * of ilm_render_noalloc, carved from its static device array. */
 struct Pipe {
+    hipStream_t stream = nullptr;
+    int idx = 0;                    /* 0 or 1 */
     DPaths A, B;                    /* path state, ping-ponged across depths */
     DHits hits;
     float4* contrib = nullptr;      /* per-path radiance */
@@ -132,6 +134,20 @@ struct ilm_ctx {
     int* ovf = nullptr;
     DCtl* ctl = nullptr;
     DChunk* d_chunk = nullptr;
+    /* the second chunk pipeline (opts.pipes == 2): consecutive chunks alternate between the
+     * two, each on its own stream, so one chunk's last short launches run beside the next
+     * chunk's first ones; accumulations stay in chunk order (acc_ev) */
+    int npipes = 1;
+    int next_pipe = 0;
+    hipStream_t stream2 = nullptr;
+    DPaths A2, B2;
+    DHits hits2;
+    float4* contrib2 = nullptr;
+    DCtl* ctl2 = nullptr;
+    DChunk* d_chunk2 = nullptr;
+    hipEvent_t acc_ev[2] = {};      /* after each pipeline's latest accumulation */
+    hipEvent_t fill_ev = nullptr;   /* after the latest clear of request buffers (stream) */
+    int last_pipe = 0;              /* pipeline of the latest chunk */
     unsigned long long* d_seg_total = nullptr;
     unsigned long long* d_trav = nullptr;
     std::vector<void*> allocs;

@@ -464,15 +464,18 @@ static int ctx_alloc_paths(ilm_ctx* ctx, int64_t cap) {
         if (e == hipSuccess) { ctx->path_allocs.push_back(*p); ctx->own_bytes += (int64_t)b; ctx->path_bytes += (int64_t)b; }
         return e;
     };
-    for (DPaths* P : {&ctx->A, &ctx->B}) {
-        HIPTRY(ctx, A((void**)&P->o, cap * sizeof(float4)));
-        HIPTRY(ctx, A((void**)&P->d, cap * sizeof(float4)));
-        HIPTRY(ctx, A((void**)&P->c, cap * sizeof(float4)));
-        HIPTRY(ctx, A((void**)&P->rng, cap * sizeof(uint2)));
+    for (int k = 0; k < ctx->npipes; k++) {
+        for (DPaths* P : {k ? &ctx->A2 : &ctx->A, k ? &ctx->B2 : &ctx->B}) {
+            HIPTRY(ctx, A((void**)&P->o, cap * sizeof(float4)));
+            HIPTRY(ctx, A((void**)&P->d, cap * sizeof(float4)));
+            HIPTRY(ctx, A((void**)&P->c, cap * sizeof(float4)));
+            HIPTRY(ctx, A((void**)&P->rng, cap * sizeof(uint2)));
+        }
+        DHits& h = k ? ctx->hits2 : ctx->hits;
+        HIPTRY(ctx, A((void**)&h.h, cap * sizeof(float4)));
+        HIPTRY(ctx, A((void**)&h.inst, cap * sizeof(int)));
+        HIPTRY(ctx, A((void**)(k ? &ctx->contrib2 : &ctx->contrib), cap * sizeof(float4)));
     }
-    HIPTRY(ctx, A((void**)&ctx->hits.h, cap * sizeof(float4)));
-    HIPTRY(ctx, A((void**)&ctx->hits.inst, cap * sizeof(int)));
-    HIPTRY(ctx, A((void**)&ctx->contrib, cap * sizeof(float4)));
     ctx->cap = cap;
     return ILM_OK;
 }
@@ -484,7 +487,8 @@ static bool opts_valid(const ilm_opts& o) {
              o.refill_min < 0 || o.refill_min > 64 || o.ext_fetch < 0 || o.ext_fetch > 65536 || (o.ext_fetch & 63) != 0 ||
              o.leaf_wait_pct < 0 || o.leaf_wait_pct > 10000 || o.shade_sort < 0 || o.shade_sort > 2 ||
              o.sched_fair < 0 || o.sched_fair > 1 || o.camera_packets < 0 || o.camera_packets > 2 ||
-             o.chunk_passes < 0 || o.paths_in_flight < 0 || o.hbm_budget_bytes < 0 || o.max_queued < 0);
+             o.chunk_passes < 0 || o.paths_in_flight < 0 || o.hbm_budget_bytes < 0 || o.max_queued < 0 ||
+             o.pipes < 0 || o.pipes > 2);
 }
 
 /* extend scheduling actually used (ILM_EXTEND_DEFAULT is the per-step refill) */
@@ -500,8 +504,12 @@ static int64_t cap_of(const ilm_opts& o) {
     return std::min<int64_t>(cap, (int64_t)1 << 30);
 }
 
+/* the traversal-stack spill area: one half per chunk pipeline */
+static size_t ovf_pipe_ints(const ilm_ctx* ctx) {
+    return (size_t)ctx->ovf_grid * ilm_k_ext_block() * ilm_k_ovf_stack();
+}
 static size_t ovf_bytes(const ilm_ctx* ctx) {
-    return (size_t)ctx->ovf_grid * ilm_k_ext_block() * ilm_k_ovf_stack() * sizeof(int);
+    return (size_t)ctx->npipes * ovf_pipe_ints(ctx) * sizeof(int);
 }
 
 int ilm_open(int device, const ilm_opts* opts, ilm_ctx** out) {
@@ -526,6 +534,10 @@ int ilm_open(int device, const ilm_opts* opts, ilm_ctx** out) {
     HIPTRY(ctx, hipGetDeviceProperties(&prop, device));
     ctx->n_cu = prop.multiProcessorCount;
     HIPTRY(ctx, hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    ctx->npipes = ctx->opts.pipes == 1 ? 1 : 2;
+    if (ctx->npipes == 2) HIPTRY(ctx, hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking));
+    for (int i = 0; i < 2; i++) HIPTRY(ctx, hipEventCreateWithFlags(&ctx->acc_ev[i], hipEventDisableTiming));
+    HIPTRY(ctx, hipEventCreateWithFlags(&ctx->fill_ev, hipEventDisableTiming));
     int occ = ilm_k_extend_occupancy(ctx->opts.tri_test, ctx->opts.bvh_layout, ext_mode_of(ctx->opts));
     ctx->ext_grid = ctx->n_cu * std::max(1, occ);
     ctx->gen_grid = ctx->n_cu * 8;
@@ -543,6 +555,10 @@ int ilm_open(int device, const ilm_opts* opts, ilm_ctx** out) {
     ctx->own_bytes += (int64_t)ovf_bytes(ctx);
     HIPTRY(ctx, A((void**)&ctx->ctl, sizeof(DCtl)));
     HIPTRY(ctx, A((void**)&ctx->d_chunk, sizeof(DChunk)));
+    if (ctx->npipes == 2) {
+        HIPTRY(ctx, A((void**)&ctx->ctl2, sizeof(DCtl)));
+        HIPTRY(ctx, A((void**)&ctx->d_chunk2, sizeof(DChunk)));
+    }
     HIPTRY(ctx, A((void**)&ctx->d_seg_total, 2 * sizeof(unsigned long long)));
     HIPTRY(ctx, A((void**)&ctx->d_trav, 2 * sizeof(unsigned long long)));
     HIPTRY(ctx, hipMemset(ctx->d_seg_total, 0, 2 * sizeof(unsigned long long)));
@@ -567,6 +583,7 @@ static void server_stop(ilm_ctx* ctx);
 static void ctx_destroy(ilm_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->stream2) (void)hipStreamSynchronize(ctx->stream2);
     for (auto& kv : ctx->graphs) (void)hipGraphExecDestroy(kv.second);
     ctx_free_paths(ctx);
     if (ctx->ovf) (void)hipFree(ctx->ovf);
@@ -580,8 +597,12 @@ static void ctx_destroy(ilm_ctx* ctx) {
     }
     if (ctx->h_u64) (void)hipHostFree(ctx->h_u64);
     if (ctx->h_seg_ring) (void)hipHostFree(ctx->h_seg_ring);
-    for (int i = 0; i < 2; i++)
+    for (int i = 0; i < 2; i++) {
         if (ctx->na_ev[i]) (void)hipEventDestroy(ctx->na_ev[i]);
+        if (ctx->acc_ev[i]) (void)hipEventDestroy(ctx->acc_ev[i]);
+    }
+    if (ctx->fill_ev) (void)hipEventDestroy(ctx->fill_ev);
+    if (ctx->stream2) (void)hipStreamDestroy(ctx->stream2);
     (void)hipStreamDestroy(ctx->stream);
     (void)hipGetLastError();
     delete ctx;
@@ -600,6 +621,7 @@ void ilm_close(ilm_ctx* ctx) {
         std::lock_guard<std::mutex> dev(ctx->dev_mu);
         (void)hipSetDevice(ctx->device);
         (void)hipStreamSynchronize(ctx->stream);
+        if (ctx->stream2) (void)hipStreamSynchronize(ctx->stream2);
     }
     ctx_unref(ctx);
 }
@@ -616,6 +638,7 @@ int ilm_tune(ilm_ctx* ctx, const ilm_opts* o) {
     std::lock_guard<std::mutex> dev(ctx->dev_mu);
     HIPTRY(ctx, hipSetDevice(ctx->device));
     HIPTRY(ctx, hipStreamSynchronize(ctx->stream));
+    if (ctx->stream2) HIPTRY(ctx, hipStreamSynchronize(ctx->stream2));
     for (auto& kv : ctx->graphs) (void)hipGraphExecDestroy(kv.second);
     ctx->graphs.clear();
     if (ext_mode_of(*o) != ext_mode_of(ctx->opts) || o->bvh_layout != ctx->opts.bvh_layout) {
@@ -624,7 +647,7 @@ int ilm_tune(ilm_ctx* ctx, const ilm_opts* o) {
         if (grid > ctx->ovf_grid) {
             /* the spill area is sized for the grid: grow it before the grid grows (on failure
              * the context keeps its old mode, grid and buffer) */
-            const size_t nb = (size_t)grid * ilm_k_ext_block() * ilm_k_ovf_stack() * sizeof(int);
+            const size_t nb = (size_t)ctx->npipes * grid * ilm_k_ext_block() * ilm_k_ovf_stack() * sizeof(int);
             int* fresh = nullptr;
             HIPTRY(ctx, hipMalloc((void**)&fresh, nb));
             (void)hipFree(ctx->ovf);
@@ -680,16 +703,18 @@ static bool jobs_need_ovf(const ilm_ctx* ctx, const std::vector<Job*>& jobs) {
     return false;
 }
 
-/* The context's own pipeline buffers. */
-static Pipe ctx_pipe(const ilm_ctx* ctx) {
+/* The context's own chunk pipeline k (0, or 1 when npipes == 2). */
+static Pipe ctx_pipe(const ilm_ctx* ctx, int k = 0) {
     Pipe q;
-    q.A = ctx->A;
-    q.B = ctx->B;
-    q.hits = ctx->hits;
-    q.contrib = ctx->contrib;
-    q.ctl = ctx->ctl;
-    q.chunk = ctx->d_chunk;
-    q.ovf = ctx->ovf;
+    q.idx = k;
+    q.stream = k ? ctx->stream2 : ctx->stream;
+    q.A = k ? ctx->A2 : ctx->A;
+    q.B = k ? ctx->B2 : ctx->B;
+    q.hits = k ? ctx->hits2 : ctx->hits;
+    q.contrib = k ? ctx->contrib2 : ctx->contrib;
+    q.ctl = k ? ctx->ctl2 : ctx->ctl;
+    q.chunk = k ? ctx->d_chunk2 : ctx->d_chunk;
+    q.ovf = ctx->ovf + (k ? ovf_pipe_ints(ctx) : 0);
     q.ext_grid = ctx->ext_grid;
     q.seg_total = ctx->d_seg_total;
     return q;
@@ -718,16 +743,15 @@ static KParams base_params(ilm_ctx* ctx, const Pipe& q) {
 }
 static KParams base_params(ilm_ctx* ctx) { return base_params(ctx, ctx_pipe(ctx)); }
 
-/* Stage a chunk descriptor: pinned ring slot -> the device DChunk dst (stream ordered). */
-static int stage_chunk(ilm_ctx* ctx, const DChunk& hc, DChunk* dst) {
+/* Stage a chunk descriptor: pinned ring slot -> the device DChunk dst (ordered on stream s). */
+static int stage_chunk(ilm_ctx* ctx, const DChunk& hc, DChunk* dst, hipStream_t s) {
     int slot = ctx->ring_pos++ % ILM_RING;
     HIPTRY(ctx, hipEventSynchronize(ctx->ring_ev[slot]));
     std::memcpy(ctx->h_chunk[slot], &hc, sizeof(DChunk));
-    HIPTRY(ctx, hipMemcpyAsync(dst, ctx->h_chunk[slot], sizeof(DChunk), hipMemcpyHostToDevice, ctx->stream));
-    HIPTRY(ctx, hipEventRecord(ctx->ring_ev[slot], ctx->stream));
+    HIPTRY(ctx, hipMemcpyAsync(dst, ctx->h_chunk[slot], sizeof(DChunk), hipMemcpyHostToDevice, s));
+    HIPTRY(ctx, hipEventRecord(ctx->ring_ev[slot], s));
     return ILM_OK;
 }
-static int stage_chunk(ilm_ctx* ctx, const DChunk& hc) { return stage_chunk(ctx, hc, ctx->d_chunk); }
 
 /* A chunk's first launch after k_ctl_init. With per-step refill the first depth's extend and
  * shade form the camera rays where they use them, so only the per-path results are cleared
@@ -740,9 +764,11 @@ static hipError_t start_paths(ilm_ctx* ctx, const Pipe& q, KParams& P, hipStream
     return P.cam0 ? ilm_k_path_clear(P, ctx->gen_grid, s) : ilm_k_raygen(P, ctx->gen_grid, s);
 }
 
-/* The launch sequence of one chunk with depth_steps depth iterations. */
-static int enqueue_pipeline(ilm_ctx* ctx, const Pipe& q, const KParams& base, int depth_steps, ProfileSink* prof) {
-    hipStream_t s = ctx->stream;
+/* The launch sequence of one chunk with depth_steps depth iterations, in two parts: the depths
+ * (ctl init, first rays, extend and shade per depth), then the accumulation into the requests'
+ * Kahan sums and the segment count. */
+static int enqueue_depths(ilm_ctx* ctx, const Pipe& q, const KParams& base, int depth_steps, ProfileSink* prof) {
+    hipStream_t s = q.stream;
     KParams P = base;
     HIPTRY(ctx, ilm_k_ctl_init(P, s));
     HIPTRY(ctx, start_paths(ctx, q, P, s));
@@ -762,9 +788,17 @@ static int enqueue_pipeline(ilm_ctx* ctx, const Pipe& q, const KParams& base, in
         }
         HIPTRY(ctx, ilm_k_shade(P, d, ctx->shade_grid, s));
     }
-    HIPTRY(ctx, ilm_k_accumulate(P, ctx->gen_grid, s));
-    HIPTRY(ctx, ilm_k_chunk_end(P, q.seg_total, s));
     return ILM_OK;
+}
+static int enqueue_finish(ilm_ctx* ctx, const Pipe& q, const KParams& base) {
+    HIPTRY(ctx, ilm_k_accumulate(base, ctx->gen_grid, q.stream));
+    HIPTRY(ctx, ilm_k_chunk_end(base, q.seg_total, q.stream));
+    return ILM_OK;
+}
+/* both parts, on one stream (debug paths and the allocation-free render) */
+static int enqueue_pipeline(ilm_ctx* ctx, const Pipe& q, const KParams& base, int depth_steps, ProfileSink* prof) {
+    int rc = enqueue_depths(ctx, q, base, depth_steps, prof);
+    return rc ? rc : enqueue_finish(ctx, q, base);
 }
 
 /* whether every request of the chunk renders the same scene (the extend kernel then reads it
@@ -775,35 +809,56 @@ static bool chunk_uni_scene(const DChunk& hc) {
     return uni && hc.work[0].req == 0;
 }
 
-static int launch_chunk(ilm_ctx* ctx, const DChunk& hc, bool ovf, ProfileSink* prof) {
-    int rc = stage_chunk(ctx, hc);
+/* Replay (capturing on first use) the graph of one part of a chunk's launch sequence. */
+static int run_graph(ilm_ctx* ctx, int key, const Pipe& q, const KParams& P, int D, bool finish) {
+    auto it = ctx->graphs.find(key);
+    if (it == ctx->graphs.end()) {
+        hipGraph_t g = nullptr;
+        hipGraphExec_t ge = nullptr;
+        HIPTRY(ctx, hipStreamBeginCapture(q.stream, hipStreamCaptureModeThreadLocal));
+        int rc2 = finish ? enqueue_finish(ctx, q, P) : enqueue_depths(ctx, q, P, D, nullptr);
+        hipError_t ec = hipStreamEndCapture(q.stream, &g);
+        if (rc2) return rc2;
+        if (ec != hipSuccess) return fail(ctx, ILM_E_DEVICE, std::string("graph capture: ") + hipGetErrorString(ec));
+        HIPTRY(ctx, hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+        (void)hipGraphDestroy(g);
+        it = ctx->graphs.emplace(key, ge).first;
+    }
+    HIPTRY(ctx, hipGraphLaunch(it->second, q.stream));
+    return ILM_OK;
+}
+
+/* Launch a chunk on the next pipeline. Its depths start at once, beside whatever the other
+ * pipeline still runs; its accumulation waits for the previous chunk's (acc_ev of the other
+ * pipeline) and for the latest clear of request buffers (fill_ev), so every pixel's sums grow
+ * in pass order. Profiling and test counting keep one pipeline. Sets *pipe_used. */
+static int launch_chunk(ilm_ctx* ctx, const DChunk& hc, bool ovf, ProfileSink* prof, int* pipe_used = nullptr) {
+    const bool two = ctx->npipes == 2 && !prof && !ctx->opts.count_tests;
+    const int k = two ? ctx->next_pipe : 0;
+    if (two) ctx->next_pipe ^= 1;
+    const Pipe q = ctx_pipe(ctx, k);
+    int rc = stage_chunk(ctx, hc, q.chunk, q.stream);
     if (rc) return rc;
-    const Pipe q = ctx_pipe(ctx);
     KParams P = base_params(ctx, q);
     P.use_ovf = ovf ? 1 : 0;
     const bool uni = chunk_uni_scene(hc);
     P.uni_scene = uni ? 1 : 0;
     const int D = hc.depth_steps;
     const bool graphs = ctx->opts.use_graphs && !prof && !ctx->opts.count_tests;
-    if (graphs) {
-        const int key = 4 * D + (uni ? 2 : 0) + (ovf ? 1 : 0);
-        auto it = ctx->graphs.find(key);
-        if (it == ctx->graphs.end()) {
-            hipGraph_t g = nullptr;
-            hipGraphExec_t ge = nullptr;
-            HIPTRY(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-            int rc2 = enqueue_pipeline(ctx, q, P, D, nullptr);
-            hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
-            if (rc2) return rc2;
-            if (ec != hipSuccess) return fail(ctx, ILM_E_DEVICE, std::string("graph capture: ") + hipGetErrorString(ec));
-            HIPTRY(ctx, hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
-            (void)hipGraphDestroy(g);
-            it = ctx->graphs.emplace(key, ge).first;
-        }
-        HIPTRY(ctx, hipGraphLaunch(it->second, ctx->stream));
-        return ILM_OK;
+    if (graphs) rc = run_graph(ctx, 2 * (8 * D + 4 * k + (uni ? 2 : 0) + (ovf ? 1 : 0)), q, P, D, false);
+    else rc = enqueue_depths(ctx, q, P, D, prof);
+    if (rc) return rc;
+    if (ctx->npipes == 2) {
+        HIPTRY(ctx, hipStreamWaitEvent(q.stream, ctx->acc_ev[k ^ 1], 0));
+        if (k == 1) HIPTRY(ctx, hipStreamWaitEvent(q.stream, ctx->fill_ev, 0));
     }
-    return enqueue_pipeline(ctx, q, P, D, prof);
+    if (graphs) rc = run_graph(ctx, 2 * k + 1, q, P, D, true);
+    else rc = enqueue_finish(ctx, q, P);
+    if (rc) return rc;
+    HIPTRY(ctx, hipEventRecord(ctx->acc_ev[k], q.stream));
+    ctx->last_pipe = k;
+    if (pipe_used) *pipe_used = k;
+    return ILM_OK;
 }
 
 /* ======================= jobs ======================= */
@@ -844,6 +899,7 @@ static int job_prepare(ilm_ctx* ctx, Job* j) {
         j->px_cap = pb;
     }
     HIPTRY(ctx, ilm_k_fill_zero4(j->d_accum, 2 * wh, ctx->stream));
+    HIPTRY(ctx, hipEventRecord(ctx->fill_ev, ctx->stream));
     job_fill_req(j);
     return ILM_OK;
 }
@@ -1019,6 +1075,7 @@ static int run_jobs_sync(ilm_ctx* ctx, std::vector<Job*>& jobs, ilm_render_info*
     HIPTRY(ctx, hipMemcpyAsync(ctx->h_u64 + 3, ctx->d_seg_total, sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx->stream));
     HIPTRY(ctx, hipMemsetAsync(ctx->d_trav, 0, 2 * sizeof(unsigned long long), ctx->stream));
     HIPTRY(ctx, hipEventRecord(e0, ctx->stream));
+    if (ctx->stream2) HIPTRY(ctx, hipStreamWaitEvent(ctx->stream2, e0, 0));
     DChunk* hc = new DChunk();
     std::unique_ptr<DChunk> hc_own(hc);
     std::vector<Job*> in_chunk;
@@ -1034,6 +1091,9 @@ static int run_jobs_sync(ilm_ctx* ctx, std::vector<Job*>& jobs, ilm_render_info*
         chunks++;
         max_paths = std::max<int64_t>(max_paths, hc->total_paths);
     }
+    /* the latest accumulation follows every earlier one, and each chunk's depths precede its
+     * accumulation on its stream */
+    if (chunks && ctx->last_pipe != 0) HIPTRY(ctx, hipStreamWaitEvent(ctx->stream, ctx->acc_ev[ctx->last_pipe], 0));
     for (Job* j : jobs) {
         const int64_t wh = (int64_t)j->p.width * j->p.height;
         HIPTRY(ctx, ilm_k_tonemap(j->d_accum, j->d_px, (int)wh, j->p.spp, j->p.tonemap, ctx->stream));
@@ -1203,6 +1263,7 @@ int ilm_render_noalloc(ilm_ctx* ctx, ilm_scene* scene, const ilm_render_params* 
     HIPTRY(ctx, hipSetDevice(ctx->device));
     unsigned char* base = (unsigned char*)ctx->noalloc_scratch;
     Pipe q;
+    q.stream = ctx->stream;
     q.ctl = (DCtl*)(base + L.off_ctl);
     q.chunk = (DChunk*)(base + L.off_chunk);
     q.seg_total = (unsigned long long*)(base + L.off_seg);
@@ -1308,7 +1369,7 @@ int ilm_debug_closest_hit(ilm_ctx* ctx, ilm_scene* scene, const float* rays, int
         }
         hc->total_paths = (uint32_t)m;
         hc->depth_steps = 1;
-        int rc = stage_chunk(ctx, *hc);
+        int rc = stage_chunk(ctx, *hc, ctx->d_chunk, ctx->stream);
         if (rc) return rc;
         HIPTRY(ctx, hipMemcpyAsync(ctx->A.o, o4.data(), m * sizeof(float4), hipMemcpyHostToDevice, ctx->stream));
         HIPTRY(ctx, hipMemcpyAsync(ctx->A.d, d4.data(), m * sizeof(float4), hipMemcpyHostToDevice, ctx->stream));
@@ -1362,7 +1423,7 @@ int ilm_debug_path_radiance(ilm_ctx* ctx, ilm_scene* scene, const ilm_render_par
         int64_t m = std::min<int64_t>(ctx->cap, n - base);
         hc->total_paths = (uint32_t)m;
         hc->depth_steps = params->max_depth;
-        rc = stage_chunk(ctx, *hc);
+        rc = stage_chunk(ctx, *hc, ctx->d_chunk, ctx->stream);
         if (rc) break;
         if (hipMemcpyAsync(d_list, pixel_pass + 2 * base, 2 * m * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream) != hipSuccess) {
             rc = fail(ctx, ILM_E_DEVICE, "list upload");
@@ -1476,24 +1537,26 @@ static void server_loop(ilm_ctx* ctx) {
             if (!issuable.empty() && inflight.size() < 2) {
                 build_chunk(ctx, issuable, *hc, in_chunk);
                 for (int w = 0; w < hc->n_work; w++) hc->work[w].pass0 += in_chunk[hc->work[w].req]->p.first_sample;
-                int rc = launch_chunk(ctx, *hc, jobs_need_ovf(ctx, in_chunk), nullptr);
+                int pk = 0;
+                int rc = launch_chunk(ctx, *hc, jobs_need_ovf(ctx, in_chunk), nullptr, &pk);
+                const Pipe q = ctx_pipe(ctx, pk);
                 Inflight f;
                 f.seg_slot = ctx->seg_ring_pos++ % ILM_RING;   /* at most two chunks in flight */
                 ctx->h_seg_ring[f.seg_slot] = 0;
-                (void)hipMemcpyAsync(ctx->h_seg_ring + f.seg_slot, &ctx->ctl->segments, sizeof(unsigned long long),
-                                     hipMemcpyDeviceToHost, ctx->stream);
+                (void)hipMemcpyAsync(ctx->h_seg_ring + f.seg_slot, &q.ctl->segments, sizeof(unsigned long long),
+                                     hipMemcpyDeviceToHost, q.stream);
                 (void)hipEventCreateWithFlags(&f.ev, hipEventDisableTiming);
                 for (Job* j : in_chunk) {
                     if (!job_has_work(j)) {
                         const int64_t wh = (int64_t)j->p.width * j->p.height;
-                        if (ilm_k_tonemap(j->d_accum, j->d_px, (int)wh, j->p.spp, j->p.tonemap, ctx->stream) != hipSuccess)
+                        if (ilm_k_tonemap(j->d_accum, j->d_px, (int)wh, j->p.spp, j->p.tonemap, q.stream) != hipSuccess)
                             j->failed = true;
-                        (void)hipMemcpyAsync(j->h_accum, j->d_accum, wh * sizeof(float4), hipMemcpyDeviceToHost, ctx->stream);
-                        (void)hipMemcpyAsync(j->h_px, j->d_px, wh * sizeof(ilm_pixel), hipMemcpyDeviceToHost, ctx->stream);
+                        (void)hipMemcpyAsync(j->h_accum, j->d_accum, wh * sizeof(float4), hipMemcpyDeviceToHost, q.stream);
+                        (void)hipMemcpyAsync(j->h_px, j->d_px, wh * sizeof(ilm_pixel), hipMemcpyDeviceToHost, q.stream);
                         f.finishing.push_back(j);
                     }
                 }
-                (void)hipEventRecord(f.ev, ctx->stream);
+                (void)hipEventRecord(f.ev, q.stream);
                 for (Job* j : in_chunk) j->last_ev_seq = ctx->chunk_seq;
                 ctx->chunk_seq++;
                 if (rc) {

@@ -591,7 +591,7 @@ __global__ void k_chunk_end(DCtl* ctl, const DChunk* chunk, unsigned long long* 
         const int D = gp(chunk)->depth_steps;
         for (int d = 0; d < D; d++) s += gp(ctl)->count[d];
         gpw(ctl)->segments = s;
-        *gpw(seg_total) += s;
+        atomicAdd(gpw(seg_total), s);   /* chunks of the two pipelines may end together */
     }
 }
 

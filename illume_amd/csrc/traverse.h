@@ -823,9 +823,6 @@ __global__ EXT_BOUNDS void k_extend(KParams P, int depth, int* ovf_base) {
     /* guided self-scheduling: reservations shrink with the rays left in the queue (down to one
      * batch), so the last waves of a launch finish together; ext_fetch is the upper bound */
     const uint32_t guide = gridDim.x * 2u;
-    CamU U;
-    const bool cu = camu_load(P, U);
-    const uint64_t su = cu ? (uint64_t)uni_p(gp(reqs[U.req].scene)) : 0ull;
     /* the first reservation is fixed by the wave's index; the cursor hands out the rest (one
      * address's atomics serialise, so a wave with nothing to do should not touch it) */
     const uint32_t first = gridDim.x * fetch;

@@ -230,6 +230,10 @@ typedef struct ilm_opts {
      * traversal for the wave, every lane testing its own ray; 0 = default: on, 1 = on, 2 = off,
      * each lane walking alone). Results are identical either way. */
     int32_t camera_packets;
+    /* chunk pipelines (1 or 2; 0 = 2): consecutive chunks alternate between two sets of path
+     * buffers on two streams, so a chunk's last, short launches run beside the next chunk's
+     * first ones (paths_in_flight paths each). Results are identical either way. */
+    int32_t pipes;
 } ilm_opts;
 
 typedef struct ilm_render_info {

@@ -74,6 +74,41 @@ def test_path_order_leaves_results_unchanged(ctx, scenes, order):
             assert i["segments"] == i_ref["segments"]
 
 
+def test_two_pipelines_match_one(scenes):
+    """Chunks alternating between two pipelines (two streams, accumulations kept in chunk order
+    by events) give the same sums, pixels and segment counts as one pipeline: a render split into
+    many chunks (some holding a pixel range of one pass, so consecutive chunks add into the same
+    pixels), a batch of the mix, and served requests, each with and without captured graphs."""
+    spec, st, _ = scenes["gold"]
+    p = _p(st, width=96, height=64, spp=9, max_depth=7)   # 6144 px per pass: two chunks per pass
+    specs = {k: v[0] for k, v in scenes.items()}
+    results = {}
+    for pipes in (1, 2):
+        for graphs in (0, 1):
+            with illume_amd.Context(0, pipes=pipes, use_graphs=graphs, paths_in_flight=4096) as c:
+                sc = {k: c.scene(v) for k, v in specs.items()}
+                a, px, info = c.render(sc["gold"], p)
+                batch, binfo = c.render_batch([sc[k] for k, _ in MIX], [_p(scenes[k][1], **kw) for k, kw in MIX])
+                hs = [c.submit(sc[k], _p(scenes[k][1], **kw)) for k, kw in MIX]
+                served = []
+                for h in hs:
+                    h.wait(60000)
+                    served.append(h.result())
+                    h.free()
+                for s_ in sc.values():
+                    s_.free()
+                results[(pipes, graphs)] = (a, px, info["segments"], info["chunks"], batch, binfo["segments"], served)
+    ref = results[(1, 0)]
+    assert ref[3] == 18
+    for key, r in results.items():
+        assert np.array_equal(r[0], ref[0]) and np.array_equal(r[1], ref[1]), key
+        assert r[2] == ref[2] and r[5] == ref[5], key
+        for (ba, bp), (ra, rp) in zip(r[4], ref[4]):
+            assert np.array_equal(ba, ra) and np.array_equal(bp, rp), key
+        for (sa, sp), (ra, rp) in zip(r[6], ref[4]):
+            assert np.array_equal(sa, ra) and np.array_equal(sp, rp), key
+
+
 def test_batch_equals_individual_renders(ctx, scenes):
     ss = [scenes[n][2] for n, _ in MIX]
     ps = [_p(scenes[n][1], **kw) for n, kw in MIX]
@@ -258,12 +293,12 @@ def test_long_queues_match_across_reservations_and_oracle(scenes):
 def test_stats_poll_and_submit_do_not_wait_for_the_chunk_in_flight():
     """ilm_stats_get reads host-side counters and the scheduler never waits on the device while
     holding the scheduling lock, so a telemetry poll, a status poll and a submission return
-    promptly while a long request occupies the device (subsurface scene, 1024x1024 x 512 spp:
-    chunks of 2^26 paths that take several hundred ms each)."""
+    promptly while a long request occupies the device (subsurface scene, 1024x1024 x 2048 spp:
+    32 chunks of 2^26 paths, about 100 ms each; the request is cancelled at the end)."""
     spec, st = S.subsurface(env_size=(256, 128))
     with illume_amd.Context(0) as c:
         s = c.scene(spec)
-        long = c.submit(s, _p(st, width=1024, height=1024, spp=512))
+        long = c.submit(s, _p(st, width=1024, height=1024, spp=2048))
         for _ in range(5000):
             if long.poll()["state"] == "running":
                 break
