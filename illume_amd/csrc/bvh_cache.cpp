@@ -142,7 +142,7 @@ bool layouts_valid(MeshBVH& b) {
             visited++;
             depth = std::max(depth, it.depth);
             const QNode& q = b.qnodes[it.node];
-            if (q.nchild < 1 || q.nchild > 4) return false;
+            if (q.nchild < 2 || q.nchild > 4) return false;   /* the 8-bit box test assumes two or more */
             for (int k = 0; k < 4; k++) {
                 const int32_t c = q.child[k];
                 if (k >= q.nchild) {

@@ -191,8 +191,10 @@ __device__ inline void box4q(const float4 w0, const float4 w1, const float4 w2, 
         const float tn = __builtin_fmaxf(__builtin_fmaxf(xn, yn), __builtin_fmaxf(zn, 0.0f));
         const float tf = __builtin_fminf(__builtin_fminf(xf, yf), zf);
         /* evaluated without short-circuit: a branch per child would split the walk into
-         * exec-masked blocks (and make the compiler re-canonicalise the slab values in each) */
-        const bool hit = (k < nchild) & (tn <= tf * BOX_K) & (tn <= tm);
+         * exec-masked blocks (and make the compiler re-canonicalise the slab values in each).
+         * Every node has at least two children (the builder collapses BVH2 nodes; cached trees
+         * are checked on load), so only slots 2 and 3 test nchild. */
+        const bool hit = (k < 2 || k < nchild) & (tn <= __builtin_fminf(tf * BOX_K, tm));
         tn_out[k] = hit ? tn : __builtin_inff();
         hitmask |= (uint32_t)hit << k;
     }
@@ -212,6 +214,25 @@ struct Stack {
             if (sp < LDS_STACK) lds[sp * EXT_BLOCK] = v;
             else if (sp < LDS_STACK + OVF_STACK) *spill(sp) = v;
             sp = sp < LDS_STACK + OVF_STACK ? sp + 1 : sp;
+        }
+    }
+    /* Push the sorted far children of a node visit that hit nhit (1..4) children: c3 if nhit > 3,
+     * then c2 if nhit > 2, then c1 if nhit > 1. While three more entries fit in LDS the lane
+     * writes rows sp..sp+2 unconditionally (rows past the new top are not live) and moves sp
+     * once, instead of three masked pushes. */
+    __device__ inline void push_far(int nhit, int c1, int c2, int c3) {
+        if (sp <= LDS_STACK - 3) {
+            const int e0 = nhit > 3 ? c3 : nhit > 2 ? c2 : c1;
+            const int e1 = nhit > 3 ? c2 : c1;
+            LDSAS int* p = lds + sp * EXT_BLOCK;
+            p[0] = e0;
+            p[EXT_BLOCK] = e1;
+            p[2 * EXT_BLOCK] = c1;
+            sp += nhit - 1;
+        } else {
+            if (nhit > 3) push(c3);
+            if (nhit > 2) push(c2);
+            if (nhit > 1) push(c1);
         }
     }
     __device__ inline int pop() {
@@ -482,10 +503,7 @@ __device__ inline bool inner_step_words(const float4 n0, const float4 n1, const 
         ILM_CE(k1, c1, k3, c3)
         ILM_CE(k1, c1, k2, c2)
 #undef ILM_CE
-        const int nhit = __builtin_popcount(hm);
-        if (nhit > 3) st.push(c3);
-        if (nhit > 2) st.push(c2);
-        if (nhit > 1) st.push(c1);
+        st.push_far(__builtin_popcount(hm), c1, c2, c3);
         node = c0;
         return true;
     } else {
