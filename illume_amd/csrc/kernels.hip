@@ -435,6 +435,9 @@ __global__ SHADE_BOUNDS void k_shade(KParams P, int depth) {
         SRu.mats = uni_p(gp(Su->mats));
         SRu.mediums = uni_p(gp(Su->mediums));
     }
+    /* first depth of a chunk of one work item: its camera values are read once per wave */
+    CamU U;
+    const bool cu = CAM && camu_load(P, U);
     const GLOBAL float4* ino = gp(P.in.o);
     const GLOBAL float4* ind = gp(P.in.d);
     const GLOBAL float4* inc4 = gp(P.in.c);
@@ -455,7 +458,8 @@ __global__ SHADE_BOUNDS void k_shade(KParams P, int depth) {
         int hinst = -1;
         if (valid) {
             if (CAM) {
-                camera_ray(P, i, o4, d4, rg);
+                if (cu) camera_ray_u(U, i, o4, d4, rg);
+                else camera_ray(P, i, o4, d4, rg);
                 c4 = make_float4(1.0f, 1.0f, 1.0f, u2f(i));
             } else {
                 o4 = ino[i]; d4 = ind[i]; c4 = inc4[i];
