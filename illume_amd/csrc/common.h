@@ -97,11 +97,12 @@ struct DInst {
     const float4* tris;
     int root;
     int qroot;
-    /* the mesh's vertex data, for shading (bytes 144..175) */
+    /* the mesh's data for shading (bytes 144..175): the shading vertices of each triangle, then
+     * texture coordinates (with idx, only for meshes that have them) */
+    const float4* svert;
+    const float* uv;
     const int* idx;
     const float* pos;
-    const float* nrm;
-    const float* uv;
 };
 static_assert(sizeof(DInst) == 176, "DInst is 11 16-byte words");
 #define ILM_INST_FACE_NORMALS 1
@@ -115,6 +116,8 @@ struct DMesh {
     const float* pos;       /* 3 per vertex */
     const float* nrm;       /* 3 per vertex, or null */
     const float* uv;        /* 2 per vertex, or null */
+    const float4* svert;    /* 3 per original triangle: its vertex normals, or with face normals its
+                               vertex positions (w = 0), so shading reads them in one request */
     int root;               /* root child code (FP32 layout) */
     int qroot;              /* root child code (quantised layout) */
     int face_normals;

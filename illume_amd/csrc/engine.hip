@@ -237,7 +237,19 @@ int ilm_scene_upload(ilm_ctx* ctx, const ilm_scene_desc* d, ilm_scene** out) {
         up = nullptr;
         if (!m.face_normals) HIPTRY(ctx, dupload(s.get(), &np, m.normals, 3 * (size_t)m.num_vertices));
         if (m.has_texcoords) HIPTRY(ctx, dupload(s.get(), &up, m.texcoords, 2 * (size_t)m.num_vertices));
+        /* shading vertices by original triangle index: the same floats the kernel would gather
+         * through the index array */
+        std::vector<float4> sv(3 * (size_t)m.num_triangles);
+        const float* va = m.face_normals ? m.positions : m.normals;
+        for (int t = 0; t < m.num_triangles; t++)
+            for (int k = 0; k < 3; k++) {
+                const float* v = &va[3 * (size_t)m.indices[3 * t + k]];
+                sv[3 * (size_t)t + k] = make_float4(v[0], v[1], v[2], 0.0f);
+            }
+        float4* svp;
+        HIPTRY(ctx, dupload(s.get(), &svp, sv.data(), sv.size()));
         dm.nodes = bn; dm.qnodes = qn; dm.tris = tp; dm.idx = ip; dm.pos = pp; dm.nrm = np; dm.uv = up;
+        dm.svert = svp;
         dm.root = bvh.root;
         dm.qroot = bvh.qroot;
         dm.face_normals = m.face_normals != 0;
@@ -263,10 +275,10 @@ int ilm_scene_upload(ilm_ctx* ctx, const ilm_scene_desc* d, ilm_scene** out) {
         di.tris = dmeshes[di.mesh].tris;
         di.root = dmeshes[di.mesh].root;
         di.qroot = dmeshes[di.mesh].qroot;
+        di.svert = dmeshes[di.mesh].svert;
+        di.uv = dmeshes[di.mesh].uv;
         di.idx = dmeshes[di.mesh].idx;
         di.pos = dmeshes[di.mesh].pos;
-        di.nrm = dmeshes[di.mesh].nrm;
-        di.uv = dmeshes[di.mesh].uv;
         di.flags = (d->meshes[di.mesh].face_normals ? ILM_INST_FACE_NORMALS : 0) |
                    (d->meshes[di.mesh].has_texcoords ? ILM_INST_TEXCOORDS : 0);
         for (int k = 0; k < 16; k++) {

@@ -155,34 +155,35 @@ __device__ inline int shade_path(const GLOBAL DScene* S, const ShadeRefs& SR, co
     float hu, hv;
     int mat;
     if (hinst >= 0) {
-        /* the instance record's flags, material and vertex pointers are requested together
-         * (bytes 96..111 and 144..159), so the vertex fetches wait on one round trip, not on the
-         * record and then a mesh record */
+        /* the instance record's flags, material and shading pointers are requested together
+         * (bytes 96..111 and 144..159), and the triangle's three shading vertices then come in
+         * one request: the normal waits on two round trips, not on the record, a mesh record,
+         * the index array and the vertex arrays */
         const GLOBAL DInst* I = SR.inst + hinst;
         const GLOBAL int4* iw = (const GLOBAL int4*)I;
         const int4 misc = iw[6], vp0 = iw[9];
         asm volatile("" ::"v"(misc.x), "v"(vp0.x));
         const int flags = misc.z;
-        const GLOBAL int* idx = (const GLOBAL int*)(((uint64_t)(uint32_t)vp0.y << 32) | (uint32_t)vp0.x);
-        const GLOBAL float* pos = (const GLOBAL float*)(((uint64_t)(uint32_t)vp0.w << 32) | (uint32_t)vp0.z);
+        const GLOBAL float4* sv = (const GLOBAL float4*)(((uint64_t)(uint32_t)vp0.y << 32) | (uint32_t)vp0.x);
         const int prim = __float_as_int(h4.w);
         const float bu = h4.y, bv = h4.z;
-        const int i0 = idx[3 * prim], i1 = idx[3 * prim + 1], i2 = idx[3 * prim + 2];
+        const float4 s0 = sv[3 * prim], s1 = sv[3 * prim + 1], s2 = sv[3 * prim + 2];
         if (flags & ILM_INST_FACE_NORMALS) {
             /* cross(v1 - v0, v2 - v0) of the triangle's vertices, as src/primitives/mesh.cu:541 */
-            V3 p0 = mk3(pos[3 * i0], pos[3 * i0 + 1], pos[3 * i0 + 2]);
-            V3 p1 = mk3(pos[3 * i1], pos[3 * i1 + 1], pos[3 * i1 + 2]);
-            V3 p2 = mk3(pos[3 * i2], pos[3 * i2 + 1], pos[3 * i2 + 2]);
+            V3 p0 = mk3(s0.x, s0.y, s0.z);
+            V3 p1 = mk3(s1.x, s1.y, s1.z);
+            V3 p2 = mk3(s2.x, s2.y, s2.z);
             n = normalize3(cross3(sub3(p1, p0), sub3(p2, p0)));
         } else {
-            const GLOBAL float* nrm = gp(I->nrm);
-            V3 n0 = mk3(nrm[3 * i0], nrm[3 * i0 + 1], nrm[3 * i0 + 2]);
-            V3 n1 = mk3(nrm[3 * i1], nrm[3 * i1 + 1], nrm[3 * i1 + 2]);
-            V3 n2 = mk3(nrm[3 * i2], nrm[3 * i2 + 1], nrm[3 * i2 + 2]);
+            V3 n0 = mk3(s0.x, s0.y, s0.z);
+            V3 n1 = mk3(s1.x, s1.y, s1.z);
+            V3 n2 = mk3(s2.x, s2.y, s2.z);
             n = normalize3(add3(mul3(n0, 1.0f - bu - bv), add3(mul3(n1, bu), mul3(n2, bv))));
         }
         if (flags & ILM_INST_TEXCOORDS) {
-            const GLOBAL float* uvp = gp(I->uv);
+            const GLOBAL float* uvp = (const GLOBAL float*)(((uint64_t)(uint32_t)vp0.w << 32) | (uint32_t)vp0.z);
+            const GLOBAL int* idx = gp(I->idx);
+            const int i0 = idx[3 * prim], i1 = idx[3 * prim + 1], i2 = idx[3 * prim + 2];
             float w = 1.0f - bu - bv;
             hu = (uvp[2 * i0] * w + uvp[2 * i1] * bu) + uvp[2 * i2] * bv;
             hv = (uvp[2 * i0 + 1] * w + uvp[2 * i1 + 1] * bu) + uvp[2 * i2 + 1] * bv;
