@@ -14,8 +14,8 @@ measures time.
     ext_fetch        rays an extend wave reserves from the queue at a time (work-queue granularity)
     leaf_wait_pct    per-step refill: lanes waiting at BLAS leaves (as a percentage of the other
                      busy lanes) before a wave spends a trip on triangle tests
-    shade_sort       1: each 512-path shading block runs its paths grouped by material stand-in
-                     (staged through LDS), 2: in queue order
+    shade_sort       1: each 512-path shading block past the first depth runs its paths grouped by
+                     material stand-in (staged through LDS), 2: in queue order
     camera_packets   1: the first depth traces each wave's 64 camera rays as one packet, 2: per lane
     sample_group     consecutive path ids per pixel (ray coherence of the first bounces)
     tile_rows        pixel walk in bands of this many rows, column by column (wave footprint)

@@ -82,6 +82,7 @@ struct Job {
     int samples_done = 0;
     int64_t last_ev_seq = -1;
     int refs = 1;
+    int device = 0;                 /* of the context (served form), for freeing the pinned results */
     std::condition_variable cv;
     ~Job();
 };
@@ -119,6 +120,10 @@ struct ilm_ctx {
     std::atomic<int> refs{1};
     int device = 0;
     hipStream_t stream = nullptr;
+    /* host<->device copies made by caller threads (scene uploads, result readback). A blocking
+     * copy on the legacy stream while the scheduler thread captures a graph fails with
+     * hipErrorStreamCaptureImplicit, so those copies go through this non-blocking stream. */
+    hipStream_t copy_stream = nullptr;
     ilm_opts opts;
     int n_cu = 0;
     int ext_grid = 0, shade_grid = 0, gen_grid = 0;

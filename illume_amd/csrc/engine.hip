@@ -48,6 +48,16 @@ static std::mutex g_scratch_mu;                     /* g_scratch_ptr */
 static void* g_scratch_ptr[ILM_MAX_DEVICES];
 /* held for the whole of an ilm_render_noalloc call: calls on one GPU share its array */
 static std::mutex g_noalloc_mu[ILM_MAX_DEVICES];
+/* held while a graph is captured on the device, and around the calls that wait for the whole
+ * device (hipFree): the runtime refuses those while any stream of the device is capturing, and
+ * contexts on one device may capture and free from different threads. A leaf lock. */
+static std::mutex g_capture_mu[ILM_MAX_DEVICES];
+static std::mutex& capture_mu(int device) { return g_capture_mu[device >= 0 && device < ILM_MAX_DEVICES ? device : 0]; }
+static void dfree(int device, void* p) {
+    if (!p) return;
+    std::lock_guard<std::mutex> lk(capture_mu(device));
+    (void)hipFree(p);
+}
 
 /* This device's address of the scratch array, looked up once per device with the device
  * current (the caller has made it current). */
@@ -100,13 +110,19 @@ static hipError_t dalloc(ilm_scene* s, void** p, size_t bytes) {
     }
     return e;
 }
+/* Blocking copy through the context's copy stream (never the legacy stream). */
+static hipError_t copy_sync(ilm_ctx* ctx, void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, ctx->copy_stream);
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(ctx->copy_stream);
+}
 template <class T>
 static hipError_t dupload(ilm_scene* s, T** dst, const T* src, size_t count) {
     *dst = nullptr;
     if (count == 0) return hipSuccess;
     hipError_t e = dalloc(s, (void**)dst, count * sizeof(T));
     if (e != hipSuccess) return e;
-    return hipMemcpy(*dst, src, count * sizeof(T), hipMemcpyHostToDevice);
+    return copy_sync(s->ctx, *dst, src, count * sizeof(T), hipMemcpyHostToDevice);
 }
 
 static int upload_texture(ilm_ctx* ctx, ilm_scene* s, const ilm_texture_desc& t, DTex& d) {
@@ -433,7 +449,7 @@ static void scene_release(ilm_scene* s, bool sched_locked = false) {
     if (s->refs.fetch_sub(1) != 1) return;
     ilm_ctx* ctx = s->ctx;
     (void)hipSetDevice(ctx->device);
-    for (void* p : s->allocs) (void)hipFree(p);
+    for (void* p : s->allocs) dfree(ctx->device, p);
     s->allocs.clear();
     (void)hipGetLastError();
     if (sched_locked) {
@@ -459,7 +475,7 @@ int ilm_debug_instance_matrices(const ilm_scene* s, int32_t i, float* out48) {
 /* ======================= context ======================= */
 
 static void ctx_free_paths(ilm_ctx* ctx) {
-    for (void* p : ctx->path_allocs) (void)hipFree(p);
+    for (void* p : ctx->path_allocs) dfree(ctx->device, p);
     ctx->path_allocs.clear();
     ctx->own_bytes -= ctx->path_bytes;
     ctx->path_bytes = 0;
@@ -546,6 +562,7 @@ int ilm_open(int device, const ilm_opts* opts, ilm_ctx** out) {
     HIPTRY(ctx, hipGetDeviceProperties(&prop, device));
     ctx->n_cu = prop.multiProcessorCount;
     HIPTRY(ctx, hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    HIPTRY(ctx, hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking));
     ctx->npipes = ctx->opts.pipes == 1 ? 1 : 2;
     if (ctx->npipes == 2) HIPTRY(ctx, hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking));
     for (int i = 0; i < 2; i++) HIPTRY(ctx, hipEventCreateWithFlags(&ctx->acc_ev[i], hipEventDisableTiming));
@@ -573,8 +590,9 @@ int ilm_open(int device, const ilm_opts* opts, ilm_ctx** out) {
     }
     HIPTRY(ctx, A((void**)&ctx->d_seg_total, 2 * sizeof(unsigned long long)));
     HIPTRY(ctx, A((void**)&ctx->d_trav, 2 * sizeof(unsigned long long)));
-    HIPTRY(ctx, hipMemset(ctx->d_seg_total, 0, 2 * sizeof(unsigned long long)));
-    HIPTRY(ctx, hipMemset(ctx->d_trav, 0, 2 * sizeof(unsigned long long)));
+    HIPTRY(ctx, hipMemsetAsync(ctx->d_seg_total, 0, 2 * sizeof(unsigned long long), ctx->stream));
+    HIPTRY(ctx, hipMemsetAsync(ctx->d_trav, 0, 2 * sizeof(unsigned long long), ctx->stream));
+    HIPTRY(ctx, hipStreamSynchronize(ctx->stream));
     for (int i = 0; i < ILM_RING; i++) {
         HIPTRY(ctx, hipHostMalloc((void**)&ctx->h_chunk[i], sizeof(DChunk), hipHostMallocDefault));
         HIPTRY(ctx, hipEventCreateWithFlags(&ctx->ring_ev[i], hipEventDisableTiming));
@@ -598,11 +616,11 @@ static void ctx_destroy(ilm_ctx* ctx) {
     if (ctx->stream2) (void)hipStreamSynchronize(ctx->stream2);
     for (auto& kv : ctx->graphs) (void)hipGraphExecDestroy(kv.second);
     ctx_free_paths(ctx);
-    if (ctx->ovf) (void)hipFree(ctx->ovf);
-    for (auto& kv : ctx->bufs.free) (void)hipFree(kv.second);
+    dfree(ctx->device, ctx->ovf);
+    for (auto& kv : ctx->bufs.free) dfree(ctx->device, kv.second);
     ctx->bufs.free.clear();
-    for (void* p : ctx->allocs) (void)hipFree(p);
-    if (ctx->seg_count) (void)hipFree(ctx->seg_count);
+    for (void* p : ctx->allocs) dfree(ctx->device, p);
+    dfree(ctx->device, ctx->seg_count);
     for (int i = 0; i < ILM_RING; i++) {
         if (ctx->h_chunk[i]) (void)hipHostFree(ctx->h_chunk[i]);
         if (ctx->ring_ev[i]) (void)hipEventDestroy(ctx->ring_ev[i]);
@@ -615,6 +633,7 @@ static void ctx_destroy(ilm_ctx* ctx) {
     }
     if (ctx->fill_ev) (void)hipEventDestroy(ctx->fill_ev);
     if (ctx->stream2) (void)hipStreamDestroy(ctx->stream2);
+    if (ctx->copy_stream) (void)hipStreamDestroy(ctx->copy_stream);
     (void)hipStreamDestroy(ctx->stream);
     (void)hipGetLastError();
     delete ctx;
@@ -662,7 +681,7 @@ int ilm_tune(ilm_ctx* ctx, const ilm_opts* o) {
             const size_t nb = (size_t)ctx->npipes * grid * ilm_k_ext_block() * ilm_k_ovf_stack() * sizeof(int);
             int* fresh = nullptr;
             HIPTRY(ctx, hipMalloc((void**)&fresh, nb));
-            (void)hipFree(ctx->ovf);
+            dfree(ctx->device, ctx->ovf);
             ctx->own_bytes += (int64_t)nb - (int64_t)ovf_bytes(ctx);
             ctx->ovf = fresh;
             ctx->ovf_grid = grid;
@@ -827,11 +846,14 @@ static int run_graph(ilm_ctx* ctx, int key, const Pipe& q, const KParams& P, int
     if (it == ctx->graphs.end()) {
         hipGraph_t g = nullptr;
         hipGraphExec_t ge = nullptr;
-        HIPTRY(ctx, hipStreamBeginCapture(q.stream, hipStreamCaptureModeThreadLocal));
-        int rc2 = finish ? enqueue_finish(ctx, q, P) : enqueue_depths(ctx, q, P, D, nullptr);
-        hipError_t ec = hipStreamEndCapture(q.stream, &g);
-        if (rc2) return rc2;
-        if (ec != hipSuccess) return fail(ctx, ILM_E_DEVICE, std::string("graph capture: ") + hipGetErrorString(ec));
+        {
+            std::lock_guard<std::mutex> cap(capture_mu(ctx->device));
+            HIPTRY(ctx, hipStreamBeginCapture(q.stream, hipStreamCaptureModeThreadLocal));
+            int rc2 = finish ? enqueue_finish(ctx, q, P) : enqueue_depths(ctx, q, P, D, nullptr);
+            hipError_t ec = hipStreamEndCapture(q.stream, &g);
+            if (rc2) return rc2;
+            if (ec != hipSuccess) return fail(ctx, ILM_E_DEVICE, std::string("graph capture: ") + hipGetErrorString(ec));
+        }
         HIPTRY(ctx, hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
         (void)hipGraphDestroy(g);
         it = ctx->graphs.emplace(key, ge).first;
@@ -919,8 +941,9 @@ static int job_prepare(ilm_ctx* ctx, Job* j) {
 /* Return the request's buffers to the pool (stream order keeps a later user of the same buffer
  * behind every kernel already queued on it). */
 static void job_release_device(Job* j) {
-    if (j->d_accum && !(j->pool && j->pool->give(j->d_accum, j->accum_cap))) (void)hipFree(j->d_accum);
-    if (j->d_px && !(j->pool && j->pool->give(j->d_px, j->px_cap))) (void)hipFree(j->d_px);
+    const int dev = j->scene ? j->scene->ctx->device : 0;
+    if (j->d_accum && !(j->pool && j->pool->give(j->d_accum, j->accum_cap))) dfree(dev, j->d_accum);
+    if (j->d_px && !(j->pool && j->pool->give(j->d_px, j->px_cap))) dfree(dev, j->d_px);
     j->d_accum = nullptr;
     j->d_px = nullptr;
 }
@@ -1149,10 +1172,10 @@ static int copy_job_outputs(ilm_ctx* ctx, Job* j, float* accum_rgb, ilm_pixel* o
     const int64_t wh = (int64_t)j->p.width * j->p.height;
     if (accum_rgb) {
         std::vector<float4> a4(wh);
-        HIPTRY(ctx, hipMemcpy(a4.data(), j->d_accum, wh * sizeof(float4), hipMemcpyDeviceToHost));
+        HIPTRY(ctx, copy_sync(ctx, a4.data(), j->d_accum, wh * sizeof(float4), hipMemcpyDeviceToHost));
         accum4_to_rgb(a4, accum_rgb, wh);
     }
-    if (out) HIPTRY(ctx, hipMemcpy(out, j->d_px, wh * sizeof(ilm_pixel), hipMemcpyDeviceToHost));
+    if (out) HIPTRY(ctx, copy_sync(ctx, out, j->d_px, wh * sizeof(ilm_pixel), hipMemcpyDeviceToHost));
     return ILM_OK;
 }
 
@@ -1475,7 +1498,7 @@ int ilm_debug_path_radiance(ilm_ctx* ctx, ilm_scene* scene, const ilm_render_par
             if (segments) segments[base + i] = sc[i];
         }
     }
-    (void)hipFree(d_list);
+    dfree(ctx->device, d_list);
     return rc;
 }
 
@@ -1685,8 +1708,14 @@ int ilm_submit(ilm_ctx* ctx, ilm_scene* scene, const ilm_render_params* params, 
     j->scene = scene;
     j->p = *params;
     j->bytes = wh * (2 * (int64_t)sizeof(float4) + (int64_t)sizeof(ilm_pixel));
-    if (hipHostMalloc((void**)&j->h_accum, wh * sizeof(float4), hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc((void**)&j->h_px, wh * sizeof(ilm_pixel), hipHostMallocDefault) != hipSuccess) {
+    j->device = ctx->device;
+    bool pinned;
+    {
+        std::lock_guard<std::mutex> cap(capture_mu(ctx->device));
+        pinned = hipHostMalloc((void**)&j->h_accum, wh * sizeof(float4), hipHostMallocDefault) == hipSuccess &&
+                 hipHostMalloc((void**)&j->h_px, wh * sizeof(ilm_pixel), hipHostMallocDefault) == hipSuccess;
+    }
+    if (!pinned) {
         delete j;   /* ~Job frees whichever buffer was allocated */
         return fail(ctx, ILM_E_NOMEM, "pinned result buffers");
     }
@@ -1787,6 +1816,8 @@ void ilm_req_free(ilm_req* r) {
 }
 
 Job::~Job() {
+    if (!h_accum && !h_px) return;
+    std::lock_guard<std::mutex> lk(capture_mu(device));
     if (h_accum) (void)hipHostFree(h_accum);
     if (h_px) (void)hipHostFree(h_px);
 }

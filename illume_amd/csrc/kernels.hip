@@ -699,10 +699,12 @@ hipError_t ilm_k_extend(const KParams& P, int depth, int* ovf, int grid, hipStre
     return hipGetLastError();
 }
 hipError_t ilm_k_shade(const KParams& P, int depth, int grid, hipStream_t s) {
+    /* the first depth's paths are in path-id order, consecutive ids being samples of one pixel
+     * (or of neighbouring pixels), so a wave already meets one or two materials: sorting them
+     * through LDS costs more than it saves there (measured), and only later depths sort */
     const bool cam = P.cam0 && depth == 0;
-    if (P.shade_sort && cam) ILM_LAUNCH((k_shade<1, 1>), dim3(grid), dim3(SHADE_BLOCK), 0, s, P, depth);
+    if (cam) ILM_LAUNCH((k_shade<0, 1>), dim3(grid), dim3(SHADE_BLOCK), 0, s, P, depth);
     else if (P.shade_sort) ILM_LAUNCH((k_shade<1, 0>), dim3(grid), dim3(SHADE_BLOCK), 0, s, P, depth);
-    else if (cam) ILM_LAUNCH((k_shade<0, 1>), dim3(grid), dim3(SHADE_BLOCK), 0, s, P, depth);
     else ILM_LAUNCH((k_shade<0, 0>), dim3(grid), dim3(SHADE_BLOCK), 0, s, P, depth);
     return hipGetLastError();
 }

@@ -219,7 +219,8 @@ typedef struct ilm_opts {
      * waiting at leaves are at least this percentage of its other busy lanes (0 = 100) */
     int32_t leaf_wait_pct;
     /* shade the paths of each 512-path block grouped by medium / miss / sphere / instance
-     * (0 = default: on, 1 = on, 2 = off) */
+     * (0 = default: on, 1 = on, 2 = off); the first depth, whose paths are in pixel-sample
+     * order, is shaded in that order either way */
     int32_t shade_sort;
     /* chunk packing across requests (served and batched forms): 0 = in arrival order, each
      * request taking all it can; 1 = fair share (requests needing less than an equal share get
