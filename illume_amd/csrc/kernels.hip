@@ -421,7 +421,9 @@ __global__ SHADE_BOUNDS void k_shade(KParams P, int depth) {
     __shared__ uint2 s_r[SB];
     __shared__ int s_i[SB];
     __shared__ uint16_t s_ord[SB];
-    __shared__ uint32_t s_wk[SORT ? SHADE_WAVES * SHADE_KEYS : 1];
+    /* survivors leave grouped by direction octant when sorting, and at the first depth */
+    constexpr bool OCT = SORT || CAM;
+    __shared__ uint32_t s_wk[OCT ? SHADE_WAVES * SHADE_KEYS : 1];
     GLOBAL DCtl* ctl = gpw(P.ctl);
     const uint32_t count = ctl->count[depth];
     const int lane = threadIdx.x & 63;
@@ -505,7 +507,7 @@ __global__ SHADE_BOUNDS void k_shade(KParams P, int depth) {
         }
         const bool alive = status == ST_ALIVE;
         uint32_t slot;
-        if (SORT) {
+        if (OCT) {
             /* survivors leave the block grouped by direction octant, so the extend waves that
              * take them 64 at a time trace rays from nearby origins in similar directions; one
              * atomic reserves the block's range */
