@@ -400,10 +400,12 @@ __device__ inline uint32_t block_split(uint32_t key, uint32_t* s_wk) {
     return s_wk[wid * NK + key] + rank;
 }
 
-#define SHADE_KEYS 17
-#define SHADE_KEY_INVALID 16
+#define SHADE_KEYS 18
+#define SHADE_KEY_INVALID 17
 __device__ inline uint32_t shade_key(int medium, int hinst) {
-    if (medium >= 0) return 15u;
+    /* inside a medium, a ray whose capped walk found no surface scatters (the cap is the
+     * scatter distance), one that found a surface usually reaches it: two different code paths */
+    if (medium >= 0) return hinst == -1 ? 15u : 16u;
     if (hinst == -1) return 14u;
     if (hinst < -1) return 9u + ((uint32_t)(-2 - hinst) & 3u);
     return (uint32_t)hinst % 9u;
