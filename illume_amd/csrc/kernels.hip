@@ -509,7 +509,7 @@ __global__ SHADE_BOUNDS void k_shade(KParams P, int depth) {
         }
         const bool alive = status == ST_ALIVE;
         uint32_t slot;
-        if (OCT) {
+        if constexpr (OCT) {
             /* survivors leave the block grouped by direction octant, so the extend waves that
              * take them 64 at a time trace rays from nearby origins in similar directions; one
              * atomic reserves the block's range */
@@ -550,47 +550,90 @@ __global__ SHADE_BOUNDS void k_shade(KParams P, int depth) {
     }
 }
 
-/* Kahan-compensated sum of the chunk's path radiances, pass order per pixel. */
-__global__ __launch_bounds__(256) void k_accumulate(KParams P) {
+/* Kahan-compensated sum of the chunk's path radiances, pass order per pixel. One wave per
+ * block takes 64 consecutive accumulate indices (pixels) at a time. Path ids come in runs of
+ * `sgroup` passes per pixel, so when a wave's 64 pixels lie in one work item and the runs are at
+ * least 8 long, the wave reads each 8-pass batch of its pixels cooperatively: 8 lanes per pixel
+ * read that pixel's 128 contiguous bytes, so a load instruction touches 8 cache lines rather than
+ * 64, and the batch is handed to the pixel's lane through LDS. Otherwise each lane reads its own
+ * pixel's radiances. Either way a lane adds its pixel's passes in pass order. */
+#define ACC_BATCH 8
+#define ACC_ROW (ACC_BATCH + 1)   /* float4 per pixel row in LDS: one pad against bank conflicts */
+__device__ inline void kahan_add(float4& s, float4& c, const float4 x) {
+    float y, t;
+    y = x.x - c.x; t = s.x + y; c.x = (t - s.x) - y; s.x = t;
+    y = x.y - c.y; t = s.y + y; c.y = (t - s.y) - y; s.y = t;
+    y = x.z - c.z; t = s.z + y; c.z = (t - s.z) - y; s.z = t;
+}
+__global__ __launch_bounds__(64) void k_accumulate(KParams P) {
+    __shared__ float4 stage[64 * ACC_ROW];
     const GLOBAL DChunk* C = gp(P.chunk);
     const uint32_t total_px = C->total_px;
     const GLOBAL float4* contrib = gp(P.contrib);
-    for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < total_px; g += gridDim.x * blockDim.x) {
+    const uint32_t lane = threadIdx.x;
+    /* the loop bound is uniform over the wave: every lane reaches the LDS barriers */
+    for (uint32_t g0 = blockIdx.x * 64u; g0 < total_px; g0 += gridDim.x * 64u) {
+        const uint32_t g = g0 + lane;
+        const bool valid = g < total_px;
+        const uint32_t gq = valid ? g : g0;
         int lo = 0, hi = C->n_work - 1;
         while (lo < hi) {
             int mid = (lo + hi + 1) >> 1;
-            if (C->work[mid].acc_base <= g) lo = mid; else hi = mid - 1;
+            if (C->work[mid].acc_base <= gq) lo = mid; else hi = mid - 1;
         }
         const GLOBAL DWork* W = C->work + lo;
         const GLOBAL DReq* R = C->reqs + W->req;
-        const uint32_t q = g - W->acc_base;
+        const uint32_t q = gq - W->acc_base;
         const uint32_t p = tile_pixel_r(W->px0 + q, R);
         GLOBAL float4* acc = gpw(R->accum);
         GLOBAL float4* cmp = gpw(R->comp);
-        float4 s = acc[p], c = cmp[p];
+        float4 s = make_float4(0.0f, 0.0f, 0.0f, 0.0f), c = s;
+        if (valid) { s = acc[p]; c = cmp[p]; }
         const int npass = W->npass;
         const uint32_t npx = W->npx, pid0 = W->pid_base, sg = W->sgroup;
-        /* eight loads in flight before the dependent Kahan chain: with samples grouped per
-         * pixel each lane walks its own run of contributions, and issuing a whole 128-byte line's
-         * worth at once keeps those lines from being evicted between single loads */
-        for (int k0 = 0; k0 < npass; k0 += 8) {
-            float4 xs[8];
+        /* one work item for the whole wave, all 64 pixels present, runs of at least a batch */
+        const bool coop = __ballot(valid && lo == __builtin_amdgcn_readfirstlane(lo)) == ~0ull && sg >= ACC_BATCH;
+        if (coop) {
+            const uint32_t q0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)q);
+            const uint32_t sub = lane >> 3, j = lane & 7u;
+            for (int k0 = 0; k0 < npass; k0 += ACC_BATCH) {
+                /* pass k0 + j of pixel q0 + 8 i + sub; a run of sg passes never splits a batch */
+                const uint32_t base = pid0 + ((uint32_t)k0 / sg) * npx * sg + ((uint32_t)k0 % sg) + j;
+                float4 xs[8];
 #pragma unroll
-            for (int j = 0; j < 8; j++)
-                if (k0 + j < npass) xs[j] = contrib[pid0 + slice_local((uint32_t)(k0 + j), q, npx, sg)];
-                else xs[j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                for (int i = 0; i < 8; i++) {
+                    xs[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                    if (k0 + (int)j < npass) xs[i] = contrib[base + (q0 + 8u * i + sub) * sg];
+                }
 #pragma unroll
-            for (int j = 0; j < 8; j++) {
-                if (k0 + j >= npass) break;
-                const float4 x = xs[j];
-                float y, t;
-                y = x.x - c.x; t = s.x + y; c.x = (t - s.x) - y; s.x = t;
-                y = x.y - c.y; t = s.y + y; c.y = (t - s.y) - y; s.y = t;
-                y = x.z - c.z; t = s.z + y; c.z = (t - s.z) - y; s.z = t;
+                for (int i = 0; i < 8; i++) stage[(8u * i + sub) * ACC_ROW + j] = xs[i];
+                __syncthreads();
+#pragma unroll
+                for (int jj = 0; jj < ACC_BATCH; jj++) {
+                    if (k0 + jj >= npass) break;
+                    kahan_add(s, c, stage[lane * ACC_ROW + jj]);
+                }
+                __syncthreads();   /* the next batch rewrites the stage */
+            }
+        } else if (valid) {
+            /* eight loads in flight before the dependent Kahan chain */
+            for (int k0 = 0; k0 < npass; k0 += 8) {
+                float4 xs[8];
+#pragma unroll
+                for (int jj = 0; jj < 8; jj++)
+                    if (k0 + jj < npass) xs[jj] = contrib[pid0 + slice_local((uint32_t)(k0 + jj), q, npx, sg)];
+                    else xs[jj] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll
+                for (int jj = 0; jj < 8; jj++) {
+                    if (k0 + jj >= npass) break;
+                    kahan_add(s, c, xs[jj]);
+                }
             }
         }
-        acc[p] = s;
-        cmp[p] = c;
+        if (valid) {
+            acc[p] = s;
+            cmp[p] = c;
+        }
     }
 }
 
@@ -713,7 +756,7 @@ hipError_t ilm_k_shade(const KParams& P, int depth, int grid, hipStream_t s) {
     return hipGetLastError();
 }
 hipError_t ilm_k_accumulate(const KParams& P, int grid, hipStream_t s) {
-    ILM_LAUNCH(k_accumulate, dim3(grid), dim3(256), 0, s, P);
+    ILM_LAUNCH(k_accumulate, dim3(grid * 4), dim3(64), 0, s, P);
     return hipGetLastError();
 }
 hipError_t ilm_k_chunk_end(const KParams& P, unsigned long long* seg_total, hipStream_t s) {
