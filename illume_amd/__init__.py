@@ -127,7 +127,7 @@ class Context:
 
     TUNABLE = ("bvh_layout", "extend_mode", "refill_min", "ext_fetch", "leaf_wait_pct", "shade_sort", "sched_fair",
                "camera_packets",
-               "sample_group", "tile_rows", "chunk_passes", "paths_in_flight")
+               "sample_group", "tile_rows", "chunk_passes", "paths_in_flight", "lone_split")
 
     def tune(self, **kw):
         """Change the tunable options (TUNABLE) of an idle context; others keep their values."""

@@ -97,7 +97,8 @@ class Opts(C.Structure):
                 ("sample_group", C.c_int32), ("tile_rows", C.c_int32), ("chunk_passes", C.c_int32),
                 ("bvh_cache_dir", C.c_char_p), ("refill_min", C.c_int32), ("ext_fetch", C.c_int32),
                 ("leaf_wait_pct", C.c_int32), ("shade_sort", C.c_int32),
-                ("sched_fair", C.c_int32), ("camera_packets", C.c_int32), ("pipes", C.c_int32)]
+                ("sched_fair", C.c_int32), ("camera_packets", C.c_int32), ("pipes", C.c_int32),
+                ("lone_split", C.c_int32)]
 
 
 class RenderInfo(C.Structure):

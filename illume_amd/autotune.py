@@ -22,6 +22,8 @@ measures time.
     paths_in_flight  paths per chunk (queue size): larger chunks amortise launch tails,
                      smaller ones bound the memory of the path state
     chunk_passes     passes of one request per chunk (0: as many as fit)
+    lone_split       1: a lone request renders as one chunk, 2: as two chunks of half its pixels
+                     on the two pipelines (0, the default: two when its max_depth is 24 or more)
 
 The search is coordinate descent from the context's current settings: each knob in turn is set
 to every candidate value with the others held, and the fastest is kept if it beats the current
@@ -43,6 +45,7 @@ DEFAULT_SPACE = (
     ("camera_packets", (1, 2)),
     ("ext_fetch", (128, 256, 512, 1024, 2048)),
     ("paths_in_flight", (1 << 24, 1 << 25, 1 << 26)),
+    ("lone_split", (1, 2)),
 )
 
 

@@ -72,6 +72,7 @@ struct Job {
     int64_t bytes = 0;              /* device bytes while admitted */
     int next_pass = 0;              /* issue cursor: pass, pixel */
     int64_t next_px = 0;
+    int64_t split_npx = 0;          /* > 0: issued as pixel ranges of this many slots, all passes each */
     /* served form */
     float4* h_accum = nullptr;      /* pinned results */
     ilm_pixel* h_px = nullptr;

@@ -516,7 +516,7 @@ static bool opts_valid(const ilm_opts& o) {
              o.leaf_wait_pct < 0 || o.leaf_wait_pct > 10000 || o.shade_sort < 0 || o.shade_sort > 2 ||
              o.sched_fair < 0 || o.sched_fair > 1 || o.camera_packets < 0 || o.camera_packets > 2 ||
              o.chunk_passes < 0 || o.paths_in_flight < 0 || o.hbm_budget_bytes < 0 || o.max_queued < 0 ||
-             o.pipes < 0 || o.pipes > 2);
+             o.pipes < 0 || o.pipes > 2 || o.lone_split < 0 || o.lone_split > 2);
 }
 
 /* extend scheduling actually used (ILM_EXTEND_DEFAULT is the per-step refill) */
@@ -658,7 +658,7 @@ void ilm_close(ilm_ctx* ctx) {
 }
 
 /* Retune an idle context: bvh_layout, extend_mode, refill_min, ext_fetch, leaf_wait_pct, shade_sort,
- * sched_fair, camera_packets, sample_group, tile_rows, chunk_passes and paths_in_flight take the values in *o (the
+ * sched_fair, camera_packets, sample_group, tile_rows, chunk_passes, paths_in_flight and lone_split take the values in *o (the
  * other fields are ignored). Every scene carries both node layouts, so a layout change needs no
  * upload. Path buffers are reallocated when the chunk size changes; captured graphs are dropped. */
 int ilm_tune(ilm_ctx* ctx, const ilm_opts* o) {
@@ -700,6 +700,7 @@ int ilm_tune(ilm_ctx* ctx, const ilm_opts* o) {
     ctx->opts.camera_packets = o->camera_packets;
     ctx->shade_grid = ctx->n_cu * ilm_k_shade_occupancy(ctx->opts.shade_sort != 2);
     ctx->opts.chunk_passes = o->chunk_passes;
+    ctx->opts.lone_split = o->lone_split;
     const int64_t cap = cap_of(*o);
     if (cap != ctx->cap) {
         ctx_free_paths(ctx);
@@ -1003,7 +1004,16 @@ static void add_slice(const ilm_ctx* ctx, Job* j, int64_t lim, int nreq, DChunk&
     std::memset(&w, 0, sizeof(w));
     w.req = nreq;
     w.pass0 = j->next_pass;
-    if (j->next_px == 0 && wh <= lim) {
+    if (j->split_npx > 0) {
+        /* a range of pixel slots with all of the request's passes (lim is not binding: the
+         * caller chose split_npx so that one range fills at most a chunk) */
+        const int64_t npx = std::min<int64_t>(wh - j->next_px, j->split_npx);
+        w.npass = j->p.spp;
+        w.px0 = (uint32_t)j->next_px;
+        w.npx = (uint32_t)npx;
+        j->next_px += npx;
+        if (j->next_px >= wh) { j->next_px = 0; j->next_pass = j->p.spp; }
+    } else if (j->next_px == 0 && wh <= lim) {
         int64_t np = std::min<int64_t>(j->p.spp - j->next_pass, lim / wh);
         if (ctx->opts.chunk_passes > 0) np = std::min<int64_t>(np, ctx->opts.chunk_passes);
         w.npass = (int)np;
@@ -1019,7 +1029,7 @@ static void add_slice(const ilm_ctx* ctx, Job* j, int64_t lim, int nreq, DChunk&
         if (j->next_px >= wh) { j->next_px = 0; j->next_pass += 1; }
     }
     w.sgroup = 1;
-    if (w.px0 == 0) {
+    {
         const uint32_t sg_max = ctx->opts.sample_group > 0 ? (uint32_t)ctx->opts.sample_group : 64u;
         while (w.sgroup * 2 <= sg_max && w.npass % (w.sgroup * 2) == 0) w.sgroup *= 2;
     }
@@ -1092,6 +1102,10 @@ static void accum4_to_rgb(const std::vector<float4>& a4, float* rgb, int64_t n) 
     for (int64_t i = 0; i < n; i++) { rgb[3 * i] = a4[i].x; rgb[3 * i + 1] = a4[i].y; rgb[3 * i + 2] = a4[i].z; }
 }
 
+/* a lone request of at least this many paths is rendered as two chunks (run_jobs_sync) */
+#define ILM_SPLIT_MIN_PATHS (1LL << 22)
+#define ILM_SPLIT_MIN_DEPTH 24
+
 /* Run jobs to completion on the calling thread (plain and batched forms). */
 static int run_jobs_sync(ilm_ctx* ctx, std::vector<Job*>& jobs, ilm_render_info* info) {
     std::lock_guard<std::mutex> dev(ctx->dev_mu);
@@ -1101,6 +1115,22 @@ static int run_jobs_sync(ilm_ctx* ctx, std::vector<Job*>& jobs, ilm_render_info*
         if (rc) return rc;
         /* pass i of the render seeds wang_hash(first_sample + i) */
         j->next_pass = 0;
+    }
+    /* A lone request that fits one chunk may run as two chunks of half its pixels (all passes
+     * each) on the two pipelines, so that each chunk's latency-bound last launches run beside
+     * the other's (lone_split). Splitting by pixels keeps a pixel's samples together in the path
+     * order, which the first depth's camera packets rely on; splitting by passes would halve the
+     * sample group. Each pixel's passes stay in one chunk, so the sums are unchanged. Deep
+     * renders gain (config 2, depth 45: +3 %); shallow ones lose a little to the two chunks'
+     * full launches competing (config 1, depth 16: -1.6 %), hence the default's depth test. */
+    if (jobs.size() == 1 && ctx->npipes == 2 && !ctx->opts.profile && !ctx->opts.count_tests &&
+        ctx->opts.chunk_passes == 0 && ctx->opts.lone_split != 1) {
+        Job* j = jobs[0];
+        const int64_t wh = (int64_t)j->p.width * j->p.height;
+        const int64_t paths = wh * j->p.spp;
+        const bool deep = ctx->opts.lone_split == 2 || j->p.max_depth >= ILM_SPLIT_MIN_DEPTH;
+        if (deep && paths <= ctx->cap && paths >= ILM_SPLIT_MIN_PATHS)
+            j->split_npx = (((wh + 1) / 2) + 63) & ~(int64_t)63;
     }
     ProfileSink prof;
     hipEvent_t e0, e1;

@@ -235,6 +235,11 @@ typedef struct ilm_opts {
      * buffers on two streams, so a chunk's last, short launches run beside the next chunk's
      * first ones (paths_in_flight paths each). Results are identical either way. */
     int32_t pipes;
+    /* a lone request (plain form) of at least 2^22 paths that fits one chunk, with two
+     * pipelines: 0 = default (split when its max_depth is at least 24), 1 = never, 2 = always
+     * split into two chunks of half its pixels (all passes each) that run side by side, so
+     * each chunk's latency-bound last launches overlap the other's. Results are identical. */
+    int32_t lone_split;
 } ilm_opts;
 
 typedef struct ilm_render_info {
@@ -294,7 +299,7 @@ const char* ilm_last_error(ilm_ctx* ctx);
 int ilm_stats_get(ilm_ctx* ctx, ilm_stats* out);
 /* Retune an idle context (no counterpart in the reference, whose launch shape is fixed):
  * bvh_layout, extend_mode, refill_min, ext_fetch, leaf_wait_pct, shade_sort, sched_fair,
- * camera_packets, sample_group, tile_rows, chunk_passes and paths_in_flight
+ * camera_packets, sample_group, tile_rows, chunk_passes, paths_in_flight and lone_split
  * take the values in *opts; other fields are ignored. ILM_E_BUSY while served requests are queued or running.
  * The autotuner (illume_amd/autotune.py) drives this between short trial renders. */
 int ilm_tune(ilm_ctx* ctx, const ilm_opts* opts);

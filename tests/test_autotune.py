@@ -5,7 +5,8 @@ from illume_amd.autotune import autotune
 
 class FakeCtx:
     def __init__(self):
-        self.s = dict(bvh_layout=0, camera_packets=0, extend_mode=0, refill_min=0, ext_fetch=0, leaf_wait_pct=0, shade_sort=0, sched_fair=0, sample_group=0, tile_rows=0, chunk_passes=0, paths_in_flight=0)
+        self.s = dict(bvh_layout=0, camera_packets=0, extend_mode=0, refill_min=0, ext_fetch=0, leaf_wait_pct=0, shade_sort=0, sched_fair=0, sample_group=0, tile_rows=0, chunk_passes=0, paths_in_flight=0,
+                      lone_split=0)
         self.tunes = 0
 
     def settings(self):

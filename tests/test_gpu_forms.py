@@ -109,6 +109,41 @@ def test_two_pipelines_match_one(scenes):
             assert np.array_equal(sa, ra) and np.array_equal(sp, rp), key
 
 
+def test_lone_render_split_in_two_matches_one_chunk(scenes):
+    """A lone request of at least 2^22 paths can run as two chunks of half its pixels (all passes
+    each) on the two pipelines (lone_split): sums, pixels and segment counts equal the one-chunk
+    render, with and without captured graphs, and for a pass count that is not a power of two (a
+    sample group below the pass count). By default only renders of max_depth 24 or more split."""
+    spec, st, _ = scenes["gold"]
+    for spp in (16, 24):
+        p = _p(st, width=512, height=384, spp=spp, max_depth=6)   # 3.1 M and 4.7 M paths
+        out = {}
+        for pipes in (1, 2):
+            for graphs in (0, 1):
+                with illume_amd.Context(0, pipes=pipes, use_graphs=graphs, lone_split=2) as c:
+                    sc = c.scene(spec)
+                    out[(pipes, graphs)] = c.render(sc, p)
+                    sc.free()
+        ref = out[(1, 0)]
+        big = 512 * 384 * spp >= 1 << 22
+        for key, (a, px, info) in out.items():
+            assert np.array_equal(a, ref[0]) and np.array_equal(px, ref[1]), (spp, key)
+            assert info["segments"] == ref[2]["segments"], (spp, key)
+            assert info["chunks"] == (2 if key[0] == 2 and big else 1), (spp, key, info["chunks"])
+    deep = _p(st, width=512, height=384, spp=24, max_depth=24)
+    res = {}
+    for split in (0, 1):
+        with illume_amd.Context(0, lone_split=split) as c:
+            sc = c.scene(spec)
+            res[split] = c.render(sc, deep)
+            res[(split, "shallow")] = c.render(sc, _p(st, width=512, height=384, spp=24, max_depth=6))[2]["chunks"]
+            sc.free()
+    assert res[0][2]["chunks"] == 2 and res[1][2]["chunks"] == 1
+    assert res[(0, "shallow")] == 1 and res[(1, "shallow")] == 1
+    assert np.array_equal(res[0][0], res[1][0]) and np.array_equal(res[0][1], res[1][1])
+    assert res[0][2]["segments"] == res[1][2]["segments"]
+
+
 def test_batch_equals_individual_renders(ctx, scenes):
     ss = [scenes[n][2] for n, _ in MIX]
     ps = [_p(scenes[n][1], **kw) for n, kw in MIX]
