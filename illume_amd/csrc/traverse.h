@@ -106,16 +106,14 @@ __device__ inline void write_hit(const KParams& P, uint32_t i, const Best& B) {
     gpw(P.hits.inst)[i] = B.inst;
 }
 
-/* Reciprocal direction for the slab test. Components below 1e-30 in magnitude (axis-parallel
- * rays) become +-1e-30 first: with a true 1/0 a plane through the origin gives 0 * inf = NaN and
- * a box the ray crosses could be rejected; with 1e-30 the slab distances (p - o) / d are +-huge
- * (or exactly 0), which keeps parallel slabs exact and the products finite for |p - o| < 3e8.
- * Only the box test uses this; triangles and spheres see the true direction. */
-__device__ inline float safe_dir(float x) { return __builtin_fabsf(x) < 1e-30f ? __builtin_copysignf(1e-30f, x) : x; }
-__device__ inline V3 rcp3(V3 d) {
-    return mk3(__builtin_amdgcn_rcpf(safe_dir(d.x)), __builtin_amdgcn_rcpf(safe_dir(d.y)),
-               __builtin_amdgcn_rcpf(safe_dir(d.z)));
-}
+/* Reciprocal direction for the slab test, clamped to [-1e30, 1e30] (one med3 after the rcp):
+ * components below about 1e-30 in magnitude (axis-parallel rays, +-0 included, the sign kept)
+ * get +-1e30. With a true 1/0 a plane through the origin gives 0 * inf = NaN and a box the ray
+ * crosses could be rejected; with 1e30 the slab distances (p - o) / d are +-huge (or exactly 0),
+ * which keeps parallel slabs exact and the products finite for |p - o| < 3e8. Only the box test
+ * uses this; triangles and spheres see the true direction. */
+__device__ inline float safe_rcp(float x) { return __builtin_amdgcn_fmed3f(__builtin_amdgcn_rcpf(x), -1e30f, 1e30f); }
+__device__ inline V3 rcp3(V3 d) { return mk3(safe_rcp(d.x), safe_rcp(d.y), safe_rcp(d.z)); }
 
 /* Two child boxes of an FP32 BVH2 node against the ray (o, 1/d). A slab distance is formed as
  * (p - o) * (1/d): each of the two roundings and v_rcp_f32's error is relative to the distance
@@ -163,7 +161,7 @@ __device__ inline void box4q(const float4 w0, const float4 w1, const float4 w2, 
      * rounded twice, a plane's error is relative to its distance (covered by the BOX_ margins)
      * plus at most 2^-23 * 255 * |A|, which the builder's outward pad of every child box
      * (bvh.cpp QPAD_REL, 2^-20 of the node extent) covers four times over. |1/d| <= 1e30
-     * (safe_dir) and s <= 2^ILM_QUANT_MAX_EXP (scenes that need a larger quantum do not get the
+     * (safe_rcp) and s <= 2^ILM_QUANT_MAX_EXP (scenes that need a larger quantum do not get the
      * 8-bit layout) keep A below 2^124; a subnormal A only adds an absolute error below 2^-140. */
     const float ax = sx * id.x, ay = sy * id.y, az = sz * id.z;
     const float bx = (w0.x - o.x) * id.x, by = (w0.y - o.y) * id.y, bz = (w0.z - o.z) * id.z;
