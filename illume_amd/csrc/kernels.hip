@@ -441,8 +441,8 @@ __global__ SHADE_BOUNDS void k_shade(KParams P, int depth) {
         SRu.mediums = uni_p(gp(Su->mediums));
     }
     /* first depth of a chunk of one work item: its camera values are read once per wave */
-    CamU U;
-    const bool cu = CAM && camu_load(P, U);
+    CamSrc cs;
+    const bool cu = CAM && camsrc_load(P, cs);
     const GLOBAL float4* ino = gp(P.in.o);
     const GLOBAL float4* ind = gp(P.in.d);
     const GLOBAL float4* inc4 = gp(P.in.c);
@@ -463,7 +463,7 @@ __global__ SHADE_BOUNDS void k_shade(KParams P, int depth) {
         int hinst = -1;
         if (valid) {
             if (CAM) {
-                if (cu) camera_ray_u(U, i, o4, d4, rg);
+                if (cu) camera_ray_u(camu_fetch(cs), i, o4, d4, rg);
                 else camera_ray(P, i, o4, d4, rg);
                 c4 = make_float4(1.0f, 1.0f, 1.0f, u2f(i));
             } else {

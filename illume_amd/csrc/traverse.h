@@ -322,9 +322,11 @@ __device__ inline void camera_ray(const KParams& P, uint32_t i, float4& o4, floa
 }
 
 /* camera_ray for a chunk of one work item (a whole plain render is one): the item, its request's
- * pixel walk and image plane, and the scene's camera are the same for every path, so they are
- * read once per wave into registers and a camera ray is then formed without a memory round trip
- * (same operations, same results as camera_ray). */
+ * pixel walk and image plane, and the scene's camera are the same for every path. Their
+ * addresses are kept once per wave (CamSrc, three uniform pointers) and the values are fetched
+ * with scalar loads where a ray is formed (camu_fetch), so they do not hold some thirty SGPRs
+ * across the kernel's loop, which made the compiler spill SGPRs into VGPR lanes (a vector
+ * instruction per spill and per reload); same operations, same results as camera_ray. */
 struct CamU {
     uint32_t pid_base, px0, npx, sg_log2;
     int pass0;
@@ -336,41 +338,54 @@ struct CamU {
     float cam[12];
     float dof, aperture;
 };
-__device__ inline UDiv uni_div(UDiv d) {
-    UDiv r;
-    r.lo = (uint32_t)uni_i((int)d.lo);
-    r.hi = (uint32_t)uni_i((int)d.hi);
-    return r;
-}
-__device__ inline float uni_fl(float v) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v))); }
-/* false (U unset) unless the chunk is one work item of generated camera rays */
-__device__ inline bool camu_load(const KParams& P, CamU& U) {
+struct CamSrc {
+    uint64_t w, r, s;   /* the work item, its request and the request's scene */
+};
+#define CONSTAS __attribute__((address_space(4)))
+/* false (src unset) unless the chunk is one work item of generated camera rays */
+__device__ inline bool camsrc_load(const KParams& P, CamSrc& src) {
     if (P.list) return false;
     const GLOBAL DChunk* C = gp(P.chunk);
     if (uni_i(C->n_work) != 1) return false;
     const GLOBAL DWork* W = C->work;
-    U.pid_base = (uint32_t)uni_i((int)W->pid_base);
-    U.px0 = (uint32_t)uni_i((int)W->px0);
-    U.npx = (uint32_t)uni_i((int)W->npx);
-    U.sg_log2 = (uint32_t)uni_i((int)W->sg_log2);
-    U.pass0 = uni_i(W->pass0);
-    U.span_div = uni_div(W->span_div);
-    U.req = uni_i(W->req);
-    const GLOBAL DReq* R = C->reqs + U.req;
-    U.width = uni_i(R->width);
-    U.pixel_size = uni_fl(R->pixel_size);
-    U.left = uni_fl(R->left);
-    U.top = uni_fl(R->top);
-    U.tile_th = (uint32_t)uni_i((int)R->tile_th);
-    U.tile_full = (uint32_t)uni_i((int)R->tile_full);
-    U.w_div = uni_div(R->w_div);
-    U.band_div = uni_div(R->band_div);
-    U.th_div = uni_div(R->th_div);
-    const GLOBAL DScene* S = gp(R->scene);
-    for (int k = 0; k < 12; k++) U.cam[k] = uni_fl(S->cam[k]);
-    U.dof = uni_fl(S->dof);
-    U.aperture = uni_fl(S->aperture);
+    const GLOBAL DReq* R = C->reqs + uni_i(W->req);
+    src.w = (uint64_t)uni_p(W);
+    src.r = (uint64_t)uni_p(R);
+    src.s = (uint64_t)uni_p(gp(R->scene));
     return true;
+}
+/* The camera values, by scalar loads. The empty asm makes the addresses opaque, so the loads are
+ * not hoisted out of the caller's loop (which would bring the register pressure back). */
+__device__ inline CamU camu_fetch(CamSrc src) {
+    asm volatile("" : "+s"(src.w), "+s"(src.r), "+s"(src.s));
+    const CONSTAS DWork* W = (const CONSTAS DWork*)src.w;
+    const CONSTAS DReq* R = (const CONSTAS DReq*)src.r;
+    const CONSTAS DScene* S = (const CONSTAS DScene*)src.s;
+    CamU U;
+    U.pid_base = W->pid_base;
+    U.px0 = W->px0;
+    U.npx = W->npx;
+    U.sg_log2 = W->sg_log2;
+    U.pass0 = W->pass0;
+    U.span_div.lo = W->span_div.lo;
+    U.span_div.hi = W->span_div.hi;
+    U.req = W->req;
+    U.width = R->width;
+    U.pixel_size = R->pixel_size;
+    U.left = R->left;
+    U.top = R->top;
+    U.tile_th = R->tile_th;
+    U.tile_full = R->tile_full;
+    U.w_div.lo = R->w_div.lo;
+    U.w_div.hi = R->w_div.hi;
+    U.band_div.lo = R->band_div.lo;
+    U.band_div.hi = R->band_div.hi;
+    U.th_div.lo = R->th_div.lo;
+    U.th_div.hi = R->th_div.hi;
+    for (int k = 0; k < 12; k++) U.cam[k] = S->cam[k];
+    U.dof = S->dof;
+    U.aperture = S->aperture;
+    return U;
 }
 __device__ inline void camera_ray_u(const CamU& U, uint32_t i, float4& o4, float4& d4, uint2& rg) {
     const uint32_t l = i - U.pid_base;
@@ -750,9 +765,9 @@ __global__ __launch_bounds__(EXT_BLOCK) __attribute__((amdgpu_waves_per_eu(7, 8)
     const GLOBAL DReq* reqs = gp(P.chunk)->reqs;
     TravCount tc = {0u, 0u};
     const uint32_t guide = gridDim.x * 2u;
-    CamU U;
-    const bool cu = camu_load(P, U);
-    const uint64_t su = cu ? (uint64_t)uni_p(gp(reqs[U.req].scene)) : 0ull;
+    CamSrc cs;
+    const bool cu = camsrc_load(P, cs);
+    const uint64_t su = cu ? cs.s : 0ull;
     /* the first reservation is fixed by the wave's index; the cursor hands out the rest (one
      * address's atomics serialise, so a wave with nothing to do should not touch it) */
     const uint32_t first = gridDim.x * fetch;
@@ -791,7 +806,7 @@ __global__ __launch_bounds__(EXT_BLOCK) __attribute__((amdgpu_waves_per_eu(7, 8)
         best_init(B);
         if (valid) {
             if (cu) {
-                camera_ray_u(U, r, o4, d4, rg);
+                camera_ray_u(camu_fetch(cs), r, o4, d4, rg);
                 sbits = su;
             } else {
                 camera_ray(P, r, o4, d4, rg);
