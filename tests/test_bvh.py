@@ -187,8 +187,9 @@ def _forge(path, edit):
 def test_bvh_cache_rejects_entries_with_a_matching_hash_but_a_bad_tree(tmp_path):
     """The payload hash only catches accidents: an entry whose checksum is recomputed after its
     tree was changed must still be refused (and rebuilt) when a child index is out of range, a
-    child points back up the tree, a leaf range runs past the triangles, or the header
-    understates a depth that bounds the device traversal stack."""
+    child points back up the tree, a leaf range runs past the triangles, an 8-bit node claims
+    fewer than two children, or the header understates a depth that bounds the device
+    traversal stack."""
     from illume_amd import bvh_device_layouts
     m = S.displaced_icosphere(4, 3)
     fresh = bvh_device_layouts(m["positions"], m["indices"])
@@ -215,10 +216,16 @@ def test_bvh_cache_rejects_entries_with_a_matching_hash_but_a_bad_tree(tmp_path)
     def shallow(h, n, q, o):
         h["depth"] = 1                       # understated FP32 depth
 
+    def one_child(h, n, q, o):
+        # an 8-bit node claiming a single child (the kernel's box test assumes two or more)
+        w = int(np.int32(q[0, 3]).view(np.uint32))
+        q[0, 3] = np.uint32((w & 0x00ffffff) | (1 << 24)).view(np.int32)
+        q[0, 13:16] = 0x7fffffff
+
     def qshallow(h, n, q, o):
         h["qdepth"] = 0
 
-    for edit in (bad_child, cycle, leaf_past_end, shallow, qshallow):
+    for edit in (bad_child, cycle, leaf_past_end, shallow, qshallow, one_child):
         path.write_bytes(good)
         _forge(path, edit)
         got = bvh_device_layouts(m["positions"], m["indices"], cache_dir=str(tmp_path))
