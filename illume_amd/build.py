@@ -54,7 +54,7 @@ def build_lib(verbose=False):
             if verbose and out.strip():
                 print(out)
     tmp = OUT_LIB + ".tmp"
-    _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-o", tmp] + objs + ["-lpthread"])
+    _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-o", tmp] + objs + ["-lpthread", "-lz"])
     os.replace(tmp, OUT_LIB)
     tmp = OUT_CLI + ".tmp"
     _run(["g++", "-std=c++17", "-O2", "-Wall", os.path.join(CSRC, "illume_cli.cpp"), "-o", tmp,

@@ -13,10 +13,13 @@
  * uninitialised; with aperture 0 the focus distance does not change ray directions, while 0
  * would make every camera ray direction 0/0).
  * Additive keys (no reference counterpart): material "eta"/"k" (explicit conductor IOR) and
- * "distribution": "ggx"; bitmap textures read PFM (float) and binary PPM (8-bit) files.
+ * "distribution": "ggx"; bitmap textures also read PFM (float) files. The reference reads images
+ * through stb_image: here PNG, binary PPM/PGM (8-bit texels) and Radiance HDR (float texels)
+ * are read by the decoders below, with stb's conventions for each.
  * The OBJ reader accepts faces with more than three vertices (fan triangulation) and long
  * lines; vertex merging assigns indices in first-occurrence order as src/primitives/mesh.cu:264-340.
  */
+#include <algorithm>
 #include <cctype>
 #include <chrono>
 #include <cmath>
@@ -30,6 +33,8 @@
 #include <string>
 #include <unordered_map>
 #include <vector>
+
+#include <zlib.h>
 
 #include "../../include/illume_c.h"
 
@@ -217,18 +222,265 @@ bool read_pfm(const std::string& path, int& w, int& h, std::vector<float>& rgba)
         }
     return true;
 }
-bool read_ppm(const std::string& path, int& w, int& h, std::vector<uint8_t>& rgba) {
+bool read_file(const std::string& path, std::vector<uint8_t>& buf) {
     FILE* f = fopen(path.c_str(), "rb");
     if (!f) return false;
-    char magic[3] = {0};
-    int mx = 0;
-    if (fscanf(f, "%2s %d %d %d", magic, &w, &h, &mx) != 4 || strcmp(magic, "P6") || mx != 255 || w <= 0 || h <= 0) { fclose(f); return false; }
-    fgetc(f);
-    std::vector<uint8_t> raw((size_t)w * h * 3);
-    if (fread(raw.data(), 1, raw.size(), f) != raw.size()) { fclose(f); return false; }
+    buf.clear();
+    uint8_t tmp[1 << 16];
+    size_t n;
+    while ((n = fread(tmp, 1, sizeof(tmp), f)) > 0) buf.insert(buf.end(), tmp, tmp + n);
     fclose(f);
-    rgba.assign((size_t)w * h * 4, 255);
-    for (size_t i = 0; i < (size_t)w * h; i++) for (int c = 0; c < 3; c++) rgba[4 * i + c] = raw[3 * i + c];
+    return true;
+}
+
+/* Largest bitmap texture read from a file, in texels. */
+const int64_t MAX_TEXELS = (int64_t)1 << 28;
+
+/* Binary PNM: P6 (RGB) and P5 (grey), maxval 1..255, '#' comments in the header. Bytes are
+ * taken as they are (not rescaled by maxval), as stb_image's PNM reader gives them to
+ * ImageIO::load_ldr (src/imageio.cpp:6-13). */
+bool read_pnm(const std::vector<uint8_t>& b, int& w, int& h, std::vector<uint8_t>& rgba, std::string& why) {
+    size_t p = 2;
+    auto field = [&](long& v) {
+        for (;;) {
+            while (p < b.size() && isspace(b[p])) p++;
+            if (p < b.size() && b[p] == '#') { while (p < b.size() && b[p] != '\n') p++; continue; }
+            break;
+        }
+        if (p >= b.size() || !isdigit(b[p])) return false;
+        v = 0;
+        while (p < b.size() && isdigit(b[p]) && v < (1L << 30)) v = v * 10 + (b[p++] - '0');
+        return true;
+    };
+    if (b.size() < 2 || b[0] != 'P' || (b[1] != '6' && b[1] != '5')) { why = "not a binary PNM"; return false; }
+    const int ch = b[1] == '6' ? 3 : 1;
+    long W = 0, H = 0, mx = 0;
+    if (!field(W) || !field(H) || !field(mx)) { why = "bad PNM header"; return false; }
+    if (W <= 0 || H <= 0 || W * H > MAX_TEXELS) { why = "bad PNM size"; return false; }
+    if (mx < 1 || mx > 255) { why = "PNM maxval above 255 is not read"; return false; }
+    p++;   /* one whitespace byte ends the header */
+    const size_t need = (size_t)W * H * ch;
+    if (p + need > b.size()) { why = "PNM data truncated"; return false; }
+    w = (int)W; h = (int)H;
+    rgba.assign((size_t)W * H * 4, 255);
+    for (size_t i = 0; i < (size_t)W * H; i++)
+        for (int c = 0; c < 3; c++) rgba[4 * i + c] = b[p + i * ch + (ch == 3 ? c : 0)];
+    return true;
+}
+
+/* Radiance RGBE (.hdr), the format ImageIO::load_hdr reads through stb_image's loader
+ * (src/imageio.cpp:15-21; src/texture.cu:49-55 sends every path ending in "hdr" there):
+ * a "#?RADIANCE" or "#?RGBE" first line, header lines up to an empty line, one of which must be
+ * FORMAT=32-bit_rle_rgbe, then the resolution line "-Y <h> +X <w>" (rows top to bottom). Pixel
+ * data is flat RGBE when w < 8 or w >= 32768, or when the first scanline does not start with
+ * the adaptive run-length header (2, 2, w >> 8, w & 255); otherwise every scanline carries that
+ * header and then its four channels one after the other, each as runs: a count byte above 128
+ * repeats the next byte count - 128 times, a count of 1..128 is followed by that many literal
+ * bytes. A texel is (r, g, b) * 2^(e - 136), all 0 when e = 0, alpha 1. A scanline after the
+ * first without its run-length header is an error here (stb restarts the image as flat data). */
+bool read_hdr(const std::vector<uint8_t>& b, int& w, int& h, std::vector<float>& rgba, std::string& why) {
+    size_t p = 0;
+    auto line = [&](std::string& out) {
+        out.clear();
+        while (p < b.size() && b[p] != '\n') out.push_back((char)b[p++]);
+        if (p >= b.size()) return false;
+        p++;
+        return true;
+    };
+    std::string ln;
+    if (!line(ln) || (ln != "#?RADIANCE" && ln != "#?RGBE")) { why = "not a Radiance HDR file"; return false; }
+    bool fmt = false;
+    for (;;) {
+        if (!line(ln)) { why = "HDR header not terminated"; return false; }
+        if (ln.empty()) break;
+        if (ln == "FORMAT=32-bit_rle_rgbe") fmt = true;
+    }
+    if (!fmt) { why = "HDR format is not 32-bit_rle_rgbe"; return false; }
+    long H = 0, W = 0;
+    if (!line(ln) || sscanf(ln.c_str(), "-Y %ld +X %ld", &H, &W) != 2) { why = "HDR orientation other than -Y +X"; return false; }
+    if (W <= 0 || H <= 0 || W * H > MAX_TEXELS) { why = "bad HDR size"; return false; }
+    w = (int)W; h = (int)H;
+    rgba.assign((size_t)W * H * 4, 0.0f);
+    auto put = [&](size_t i, const uint8_t* e) {
+        float* o = &rgba[4 * i];
+        if (e[3]) {
+            const float f = ldexpf(1.0f, (int)e[3] - 136);
+            o[0] = (float)e[0] * f; o[1] = (float)e[1] * f; o[2] = (float)e[2] * f;
+        }
+        o[3] = 1.0f;
+    };
+    const size_t n = (size_t)W * H;
+    bool flat = W < 8 || W >= 32768;
+    if (!flat && (p + 4 > b.size() || b[p] != 2 || b[p + 1] != 2 || (b[p + 2] & 0x80))) flat = true;
+    if (flat) {
+        if (p + 4 * n > b.size()) { why = "HDR data truncated"; return false; }
+        for (size_t i = 0; i < n; i++) put(i, &b[p + 4 * i]);
+        return true;
+    }
+    std::vector<uint8_t> row((size_t)W * 4);
+    for (long y = 0; y < H; y++) {
+        if (p + 4 > b.size() || b[p] != 2 || b[p + 1] != 2 || (b[p + 2] & 0x80)) { why = "HDR scanline without run-length header"; return false; }
+        if (((long)b[p + 2] << 8 | b[p + 3]) != W) { why = "HDR scanline length differs from the width"; return false; }
+        p += 4;
+        for (int c = 0; c < 4; c++) {
+            long x = 0;
+            while (x < W) {
+                if (p >= b.size()) { why = "HDR data truncated"; return false; }
+                int cnt = b[p++];
+                if (cnt > 128) {
+                    cnt -= 128;
+                    if (cnt > W - x || p >= b.size()) { why = "bad HDR run"; return false; }
+                    const uint8_t v = b[p++];
+                    for (int k = 0; k < cnt; k++) row[4 * (x++) + c] = v;
+                } else {
+                    if (cnt == 0 || cnt > W - x || p + cnt > b.size()) { why = "bad HDR run"; return false; }
+                    for (int k = 0; k < cnt; k++) row[4 * (x++) + c] = b[p++];
+                }
+            }
+        }
+        for (long x = 0; x < W; x++) put((size_t)y * W + x, &row[4 * x]);
+    }
+    return true;
+}
+
+/* PNG to 8-bit RGBA, as stb_image's stbi_load(..., 4) gives ImageIO::load_ldr
+ * (src/imageio.cpp:6-13): every colour type (grey, RGB, palette, grey + alpha, RGBA), bit depths
+ * 1-16, both interlace methods, the five scanline filters. Grey below 8 bits is scaled to
+ * 0..255 (times 255, 85 or 17); 16-bit samples keep their high byte; grey is replicated into
+ * R, G and B; alpha is 255 where the image has none (tRNS is not applied: rendering reads RGB
+ * only). IDAT data is inflated with zlib; chunk CRCs are not checked, and an unknown critical
+ * chunk is an error, as in stb. */
+uint32_t be32(const uint8_t* q) { return (uint32_t)q[0] << 24 | (uint32_t)q[1] << 16 | (uint32_t)q[2] << 8 | q[3]; }
+
+bool read_png(const std::vector<uint8_t>& b, int& w, int& h, std::vector<uint8_t>& rgba, std::string& why) {
+    static const uint8_t sig[8] = {0x89, 'P', 'N', 'G', '\r', '\n', 0x1a, '\n'};
+    if (b.size() < 8 || memcmp(b.data(), sig, 8)) { why = "not a PNG file"; return false; }
+    size_t p = 8;
+    uint32_t W = 0, H = 0;
+    int depth = 0, ctype = -1, interlace = 0;
+    std::vector<uint8_t> pal, z;
+    bool ihdr = false, iend = false;
+    while (!iend) {
+        if (p + 12 > b.size()) { why = "PNG truncated"; return false; }
+        const uint32_t len = be32(&b[p]);
+        const uint8_t* t = &b[p + 4];
+        if (len > b.size() - p - 12) { why = "PNG chunk past the end of the file"; return false; }
+        const uint8_t* d = &b[p + 8];
+        const std::string type((const char*)t, 4);
+        if (!ihdr && type != "IHDR") { why = "PNG does not start with IHDR"; return false; }
+        if (type == "IHDR") {
+            if (len != 13 || ihdr) { why = "bad IHDR"; return false; }
+            W = be32(d); H = be32(d + 4); depth = d[8]; ctype = d[9]; interlace = d[12];
+            if (d[10] != 0 || d[11] != 0 || interlace > 1) { why = "unknown PNG compression, filter or interlace method"; return false; }
+            ihdr = true;
+        } else if (type == "PLTE") {
+            if (len % 3 || len > 768) { why = "bad PLTE"; return false; }
+            pal.assign(d, d + len);
+        } else if (type == "IDAT") {
+            z.insert(z.end(), d, d + len);
+        } else if (type == "IEND") {
+            iend = true;
+        } else if (!(t[0] & 32)) {
+            why = "unknown critical PNG chunk " + type;
+            return false;
+        }
+        p += 12 + (size_t)len;
+    }
+    const bool okdepth = (ctype == 0 && (depth == 1 || depth == 2 || depth == 4 || depth == 8 || depth == 16)) ||
+                         (ctype == 3 && (depth == 1 || depth == 2 || depth == 4 || depth == 8)) ||
+                         ((ctype == 2 || ctype == 4 || ctype == 6) && (depth == 8 || depth == 16));
+    if (!okdepth) { why = "bad PNG colour type / bit depth"; return false; }
+    if (W == 0 || H == 0 || (int64_t)W * H > MAX_TEXELS) { why = "bad PNG size"; return false; }
+    if (ctype == 3 && pal.empty()) { why = "palette PNG without PLTE"; return false; }
+    const int chans = ctype == 2 ? 3 : ctype == 4 ? 2 : ctype == 6 ? 4 : 1;
+    const int bpp_bits = chans * depth;
+    const size_t fbpp = (size_t)std::max(1, bpp_bits / 8);   /* filter distance in bytes */
+    /* Adam7 passes: start x, start y, step x, step y (one pass of the whole image otherwise) */
+    static const int A7[7][4] = {{0, 0, 8, 8}, {4, 0, 8, 8}, {0, 4, 4, 8}, {2, 0, 4, 4}, {0, 2, 2, 4}, {1, 0, 2, 2}, {0, 1, 1, 2}};
+    const int npass = interlace ? 7 : 1;
+    size_t raw_len = 0;
+    for (int k = 0; k < npass; k++) {
+        const uint32_t pw = interlace ? (W - A7[k][0] + A7[k][2] - 1) / A7[k][2] * (W > (uint32_t)A7[k][0]) : W;
+        const uint32_t ph = interlace ? (H - A7[k][1] + A7[k][3] - 1) / A7[k][3] * (H > (uint32_t)A7[k][1]) : H;
+        if (pw && ph) raw_len += (size_t)ph * (1 + ((size_t)pw * bpp_bits + 7) / 8);
+    }
+    std::vector<uint8_t> raw(raw_len);
+    {
+        z_stream zs;
+        memset(&zs, 0, sizeof(zs));
+        if (inflateInit(&zs) != Z_OK) { why = "zlib inflateInit failed"; return false; }
+        zs.next_in = z.data();
+        zs.avail_in = (uInt)z.size();
+        zs.next_out = raw.data();
+        zs.avail_out = (uInt)raw.size();
+        const int zr = inflate(&zs, Z_FINISH);
+        const size_t got = raw.size() - zs.avail_out;
+        inflateEnd(&zs);
+        if ((zr != Z_STREAM_END && zr != Z_BUF_ERROR && zr != Z_OK) || got < raw.size()) { why = "PNG image data inflates short or corrupt"; return false; }
+    }
+    w = (int)W; h = (int)H;
+    rgba.assign((size_t)W * H * 4, 255);
+    static const int gscale[17] = {0, 255, 85, 0, 17, 0, 0, 0, 1, 0, 0, 0, 0, 0, 0, 0, 1};
+    size_t off = 0;
+    std::vector<uint8_t> prev, cur;
+    for (int k = 0; k < npass; k++) {
+        const uint32_t x0 = interlace ? A7[k][0] : 0, y0 = interlace ? A7[k][1] : 0;
+        const uint32_t sx = interlace ? A7[k][2] : 1, sy = interlace ? A7[k][3] : 1;
+        const uint32_t pw = W > x0 ? (W - x0 + sx - 1) / sx : 0, ph = H > y0 ? (H - y0 + sy - 1) / sy : 0;
+        if (!pw || !ph) continue;
+        const size_t stride = ((size_t)pw * bpp_bits + 7) / 8;
+        prev.assign(stride, 0);
+        cur.resize(stride);
+        for (uint32_t r = 0; r < ph; r++) {
+            const int ft = raw[off];
+            const uint8_t* src = &raw[off + 1];
+            off += 1 + stride;
+            for (size_t i = 0; i < stride; i++) {
+                const int a = i >= fbpp ? cur[i - fbpp] : 0, up = prev[i], c = i >= fbpp ? prev[i - fbpp] : 0;
+                int pred;
+                switch (ft) {
+                case 0: pred = 0; break;
+                case 1: pred = a; break;
+                case 2: pred = up; break;
+                case 3: pred = (a + up) >> 1; break;
+                case 4: {
+                    const int pp = a + up - c, pa = abs(pp - a), pb = abs(pp - up), pc = abs(pp - c);
+                    pred = (pa <= pb && pa <= pc) ? a : (pb <= pc ? up : c);
+                    break;
+                }
+                default: why = "bad PNG filter type"; return false;
+                }
+                cur[i] = (uint8_t)(src[i] + pred);
+            }
+            const uint32_t y = y0 + r * sy;
+            for (uint32_t q = 0; q < pw; q++) {
+                const uint32_t x = x0 + q * sx;
+                uint8_t* o = &rgba[((size_t)y * W + x) * 4];
+                int s[4];
+                for (int c = 0; c < chans; c++) {
+                    if (depth == 8) s[c] = cur[(size_t)q * chans + c];
+                    else if (depth == 16) s[c] = cur[((size_t)q * chans + c) * 2];   /* high byte */
+                    else {
+                        const size_t bit = (size_t)q * depth;
+                        s[c] = (cur[bit >> 3] >> (8 - depth - (int)(bit & 7))) & ((1 << depth) - 1);
+                    }
+                }
+                if (ctype == 3) {
+                    if ((size_t)s[0] * 3 + 2 >= pal.size()) { why = "PNG palette index out of range"; return false; }
+                    o[0] = pal[3 * s[0]]; o[1] = pal[3 * s[0] + 1]; o[2] = pal[3 * s[0] + 2];
+                } else if (chans <= 2) {
+                    const int g = depth < 8 ? s[0] * gscale[depth] : s[0];
+                    o[0] = o[1] = o[2] = (uint8_t)g;
+                    if (chans == 2) o[3] = (uint8_t)s[1];
+                } else {
+                    o[0] = (uint8_t)s[0]; o[1] = (uint8_t)s[1]; o[2] = (uint8_t)s[2];
+                    if (chans == 4) o[3] = (uint8_t)s[3];
+                }
+            }
+            prev.swap(cur);
+            cur.resize(stride);
+        }
+    }
     return true;
 }
 
@@ -273,15 +525,26 @@ ilm_texture_desc texture_from_json(const JV& j, ilm_desc* D, const std::string& 
         std::unique_ptr<TexOwn> own(new TexOwn());
         int w = 0, h = 0;
         bool ok = false;
+        std::string why = "cannot open the file";
+        std::vector<uint8_t> bytes;
+        const bool opened = read_file(path, bytes);
         if (path.size() > 4 && path.substr(path.size() - 4) == ".pfm") {
             ok = read_pfm(path, w, h, own->f);
+            if (!ok) why = "not a readable PFM file";
             t.is_hdr = 1;
-        } else if (path.size() > 4 && path.substr(path.size() - 4) == ".ppm") {
-            ok = read_ppm(path, w, h, own->u8);
+        } else if (opened && path.size() >= 3 && path.substr(path.size() - 3) == "hdr") {
+            /* float texels, bilinear, as src/texture.cu:49-55 */
+            ok = read_hdr(bytes, w, h, own->f, why);
+            t.is_hdr = 1;
+        } else if (opened) {
+            /* 8-bit texels, point sampled, as src/texture.cu:57-63; the format is told by content */
+            if (bytes.size() >= 8 && bytes[0] == 0x89 && bytes[1] == 'P') ok = read_png(bytes, w, h, own->u8, why);
+            else if (bytes.size() >= 2 && bytes[0] == 'P' && (bytes[1] == '6' || bytes[1] == '5')) ok = read_pnm(bytes, w, h, own->u8, why);
+            else why = "unsupported image format (PNG, binary PPM/PGM, Radiance HDR and PFM are read)";
             t.is_hdr = 0;
         }
         if (!ok) {
-            fprintf(stderr, "texture_from_json: cannot read %s, using black\n", path.c_str());
+            fprintf(stderr, "texture_from_json: cannot read %s (%s), using black\n", path.c_str(), why.c_str());
             return tex_const(0, 0, 0);
         }
         t.type = ILM_TEX_BITMAP;

@@ -3,8 +3,8 @@
 * a thin-lens camera (aperture > 0: the lens sample of src/renderer.cu:97-102, src/math/sample.cu
   sample_circle) on the Cornell box, per-path radiance and a whole render against the oracle;
 * a scene read by the scene-JSON loader whose textures and meshes come from files: an 8-bit PPM
-  albedo (point sampled, src/texture.cu:112,156) on a textured OBJ quad and on a sphere (sphere
-  uv, src/primitives/sphere.cu:74-75), a float PFM environment (bilinear, env uv
+  or PNG albedo (point sampled, src/texture.cu:112,156) on a textured OBJ quad and on a sphere (sphere
+  uv, src/primitives/sphere.cu:74-75), a float PFM or Radiance HDR environment (bilinear, env uv
   src/renderer.cu:380-381), a generated 8,000-triangle OBJ with vertex normals (smooth shading)
   in rough gold, and a thin-lens camera; device and oracle render the same loaded descriptor.
 """
@@ -17,6 +17,7 @@ import illume_amd
 import illume_amd.scenes as S
 from illume_amd._abi import render_params
 from oracle.pyoracle import OracleScene
+from image_files import hdr_bytes, png_bytes, rgbe_encode
 
 pytestmark = pytest.mark.gpu
 
@@ -97,21 +98,29 @@ def _write_pfm(path, rgb):
         f.write(np.ascontiguousarray(rgb[::-1], dtype="<f4").tobytes())
 
 
-def _textured_json(tmp_path):
+def _textured_json(tmp_path, png_hdr=False):
+    """png_hdr: the albedo as an interlaced PNG and the environment as a run-length Radiance HDR
+    file (the formats the reference's own scenes use) instead of PPM and PFM."""
     rng = np.random.default_rng(5)
-    _write_ppm(tmp_path / "albedo.ppm", rng.integers(20, 250, (7, 13, 3)))
+    albedo = rng.integers(20, 250, (7, 13, 3))
     env = S.sky_env(48, 24, sun_radiance=60.0, sun_diam_deg=12.0)[..., :3]
-    _write_pfm(tmp_path / "sky.pfm", env)
+    if png_hdr:
+        (tmp_path / "albedo.png").write_bytes(png_bytes(albedo, 2, 8, interlace=1))
+        (tmp_path / "sky.hdr").write_bytes(hdr_bytes(rgbe_encode(env)))
+    else:
+        _write_ppm(tmp_path / "albedo.ppm", albedo)
+        _write_pfm(tmp_path / "sky.pfm", env)
+    alb, sky = ("albedo.png", "sky.hdr") if png_hdr else ("albedo.ppm", "sky.pfm")
     (tmp_path / "quad.obj").write_text(OBJ_QUAD)
     ball = S.displaced_icosphere(20, 11)
     ball["normals"] = ball["positions"] / np.linalg.norm(ball["positions"], axis=1, keepdims=True)
     S.write_obj(ball, str(tmp_path / "ball.obj"), normals=True)
     js = {
-        "environment_map": {"type": "bitmap", "file": "sky.pfm"},
+        "environment_map": {"type": "bitmap", "file": sky},
         "materials": [
-            {"name": "tex", "type": "lambert", "albedo": {"type": "bitmap", "file": "albedo.ppm", "scale": [3, 2]}},
-            {"name": "ball", "type": "lambert", "albedo": {"type": "bitmap", "file": "albedo.ppm"}},
-            {"name": "glossy", "type": "plastic", "ior": 1.5, "albedo": {"type": "bitmap", "file": "albedo.ppm"}},
+            {"name": "tex", "type": "lambert", "albedo": {"type": "bitmap", "file": alb, "scale": [3, 2]}},
+            {"name": "ball", "type": "lambert", "albedo": {"type": "bitmap", "file": alb}},
+            {"name": "glossy", "type": "plastic", "ior": 1.5, "albedo": {"type": "bitmap", "file": alb}},
             {"name": "metal", "type": "roughconductor", "material": "Au", "roughness": 0.25},
         ],
         "meshes": [{"name": "quad", "file": "quad.obj", "has_texcoords": True},
@@ -135,8 +144,9 @@ def _textured_json(tmp_path):
     return str(path)
 
 
-def test_file_textures_and_env_from_json_match_oracle(ctx, tmp_path):
-    js = illume_amd.load_json(_textured_json(tmp_path))
+@pytest.mark.parametrize("png_hdr", [False, True], ids=["ppm_pfm", "png_hdr"])
+def test_file_textures_and_env_from_json_match_oracle(ctx, tmp_path, png_hdr):
+    js = illume_amd.load_json(_textured_json(tmp_path, png_hdr))
     d = js.desc
     m = [d.materials[i] for i in range(d.num_materials)]
     assert m[0].albedo.type == 2 and m[0].albedo.is_hdr == 0 and (m[0].albedo.width, m[0].albedo.height) == (13, 7)
@@ -146,8 +156,9 @@ def test_file_textures_and_env_from_json_match_oracle(ctx, tmp_path):
     sc = js.upload(ctx)
     orc = OracleScene.from_desc(d, owner=js)
     p = render_params(width=st.width, height=st.height, spp=8, max_depth=6, ray_bias=st.ray_bias, tonemap=st.tonemap)
-    _path_parity(ctx, sc, orc, p, 1500, 22, "file-textured paths")
-    _compare_render(ctx, sc, orc, p, "file-textured render")
+    label = "png/hdr" if png_hdr else "ppm/pfm"
+    _path_parity(ctx, sc, orc, p, 1500, 22, label + " file-textured paths")
+    _compare_render(ctx, sc, orc, p, label + " file-textured render")
     sc.free()
     orc.free()
     js.free()
