@@ -683,17 +683,20 @@ __device__ inline void packet_trace(const GLOBAL DScene* S, V3 O, V3 D, Best& B,
             if (any) {
                 const int lead = (int)__builtin_ctzll(__ballot(hm != 0u));
                 const uint32_t lh = (uint32_t)__builtin_amdgcn_readlane((int)hm, lead);
-                float k[4];
+                /* sort keys as unsigned ints: entry distances are >= 0 (the sign bit is cleared
+                 * for -0), so their bit patterns order as the floats do, and the network below
+                 * runs on the scalar unit (the keys are uniform) */
+                uint32_t k[4];
                 int c[4] = {uni_i(ch.x), uni_i(ch.y), uni_i(ch.z), uni_i(ch.w)};
 #pragma unroll
                 for (int j = 0; j < 4; j++) {
-                    const float lt = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(tn[j]), lead));
-                    k[j] = !((any >> j) & 1u) ? __builtin_inff() : (((lh >> j) & 1u) ? lt : ILM_FLT_MAX);
+                    const uint32_t lt = (uint32_t)__builtin_amdgcn_readlane(__float_as_int(tn[j]), lead) & 0x7fffffffu;
+                    k[j] = !((any >> j) & 1u) ? 0x7f800000u : (((lh >> j) & 1u) ? lt : 0x7f7fffffu);   /* +inf, FLT_MAX */
                 }
 #define ILM_CE(a, b)                                                                  \
                 {                                                                     \
                     const bool sw = k[b] < k[a];                                      \
-                    const float tk = sw ? k[b] : k[a]; k[b] = sw ? k[a] : k[b]; k[a] = tk; \
+                    const uint32_t tk = sw ? k[b] : k[a]; k[b] = sw ? k[a] : k[b]; k[a] = tk; \
                     const int tc_ = sw ? c[b] : c[a]; c[b] = sw ? c[a] : c[b]; c[a] = tc_; \
                 }
                 ILM_CE(0, 1)
