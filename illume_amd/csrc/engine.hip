@@ -1121,8 +1121,9 @@ static int run_jobs_sync(ilm_ctx* ctx, std::vector<Job*>& jobs, ilm_render_info*
      * the other's (lone_split). Splitting by pixels keeps a pixel's samples together in the path
      * order, which the first depth's camera packets rely on; splitting by passes would halve the
      * sample group. Each pixel's passes stay in one chunk, so the sums are unchanged. Deep
-     * renders gain (config 2, depth 45: +3 %); shallow ones lose a little to the two chunks'
-     * full launches competing (config 1, depth 16: -1.6 %), hence the default's depth test. */
+     * renders gain (960x720, 32 spp, depth 45 in a subsurface medium: +3 %); shallow ones lose
+     * a little to the two chunks' full launches competing (1024x1024, 64 spp, depth 16: -1.6 %),
+     * hence the default's depth test. */
     if (jobs.size() == 1 && ctx->npipes == 2 && !ctx->opts.profile && !ctx->opts.count_tests &&
         ctx->opts.chunk_passes == 0 && ctx->opts.lone_split != 1) {
         Job* j = jobs[0];
