@@ -309,7 +309,13 @@ int ilm_scene_upload(ilm_ctx* ctx, const ilm_scene_desc* desc, ilm_scene** out);
 void ilm_scene_free(ilm_scene* scene);
 int64_t ilm_scene_device_bytes(const ilm_scene* scene);
 
-/* Reference-schema scene JSON (+ OBJ meshes) to a host descriptor; host only. */
+/* Reference-schema scene JSON (+ OBJ meshes) to a host descriptor; host only. Replaces
+ * HostScene's JSON constructor (src/scene/scene.cpp:11-99). Bitmap textures and environment maps
+ * are read from PNG or binary PPM/PGM files (8-bit texels) and, for paths ending in "hdr", from
+ * Radiance RGBE files (float texels), as the reference's stb_image path gives them
+ * (src/imageio.cpp:6-21, src/texture.cu:49-63); PFM files give float texels as well. A texture
+ * file that cannot be read becomes black, with a message on stderr; an OBJ file that cannot be
+ * read fails the load with ILM_E_IO. */
 int ilm_desc_load_json(const char* path, ilm_desc** out);
 const ilm_scene_desc* ilm_desc_scene(const ilm_desc* d);
 int ilm_desc_settings(const ilm_desc* d, ilm_render_params* out);
