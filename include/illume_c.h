@@ -216,7 +216,7 @@ typedef struct ilm_opts {
      * reservations as the queue drains (down to 64 rays) */
     int32_t ext_fetch;
     /* per-step refill: a wave runs its BLAS-leaf triangle tests only on trips where the lanes
-     * waiting at leaves are at least this percentage of its other busy lanes (0 = 100) */
+     * waiting at leaves are at least this percentage of its other busy lanes (0 = 60) */
     int32_t leaf_wait_pct;
     /* shade the paths of each 512-path block grouped by medium / miss / sphere / instance
      * (0 = default: on, 1 = on, 2 = off); the first depth, whose paths are in pixel-sample
