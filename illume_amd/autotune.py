@@ -39,7 +39,7 @@ DEFAULT_SPACE = (
     ("bvh_layout", (0, 1)),
     ("sample_group", (1, 4, 16, 64)),
     ("tile_rows", (1, 4, 8)),
-    (("extend_mode", "refill_min"), ((1, 0), (2, 16), (2, 32), (3, 8), (3, 16), (3, 48))),
+    (("extend_mode", "refill_min"), ((1, 0), (2, 16), (2, 32), (3, 8), (3, 16), (3, 32), (3, 48))),
     ("leaf_wait_pct", (40, 50, 60, 80, 100, 150)),
     ("shade_sort", (1, 2)),
     ("camera_packets", (1, 2)),

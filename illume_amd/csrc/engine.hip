@@ -765,7 +765,7 @@ static KParams base_params(ilm_ctx* ctx, const Pipe& q) {
     P.tri_test = ctx->opts.tri_test;
     P.extend_mode = ext_mode_of(ctx->opts);
     P.tile_rows = ctx->opts.tile_rows > 0 ? ctx->opts.tile_rows : 4;
-    P.refill_min = ctx->opts.refill_min > 0 ? ctx->opts.refill_min : 32;
+    P.refill_min = ctx->opts.refill_min > 0 ? ctx->opts.refill_min : 24;
     P.ext_fetch = ctx->opts.ext_fetch > 0 ? ctx->opts.ext_fetch : 256;
     P.leaf_pct = ctx->opts.leaf_wait_pct > 0 ? ctx->opts.leaf_wait_pct : 60;
     P.shade_sort = ctx->opts.shade_sort == 2 ? 0 : 1;

@@ -207,7 +207,7 @@ typedef struct ilm_opts {
      * device layouts; replaces the reference's unversioned .bvh files, src/accel/bvh.cpp:399-439),
      * or NULL for none. The string is copied at ilm_open. */
     const char* bvh_cache_dir;
-    /* refill modes: number of idle lanes of a wave that triggers a refill (0 = 32) */
+    /* refill modes: number of idle lanes of a wave that triggers a refill (0 = 24) */
     int32_t refill_min;
     /* rays an extend wave reserves from the queue at a time (multiple of 64; 0 = 256): its first
      * reservation by its index, later ones from the shared cursor. In the per-step and leaf
